@@ -1,0 +1,188 @@
+"""Benchmark: traced rays/s and CDF records/s of the batched drop workload.
+
+Workload (one step = one pass of the hot path over the whole batch, inputs and
+outputs resident in HBM): oblate water drops, radius 2 mm, four seeded
+Chebyshev shape terms (axis ratio ~0.85, convex), 2^26 incident rays per
+record on a stratified grid, 64 inclinations evenly spaced in [-90, 90] deg x
+3 wavelengths (450/550/650 nm, IAPWS R9-97 water at 20 C) = 192 records,
+up to 9 internal interactions per ray, 900 x 1800 (theta, phi) cells, FP32
+tracing, FP64 CDF; zero + trace + CDF replayed as one hipGraph per step.
+
+Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# Water refractive index, IAPWS R9-97 with IAPWS-95 density at 20 C and
+# 101325 Pa, at the three workload wavelengths.
+WATER_20C = {450.0: 1.339607940680526, 550.0: 1.3346825974540153, 650.0: 1.3316678740505974}
+# Mean Table-I row at 2 mm (include/rainbow/raindrop_shape.hpp:29-34 gives
+# c0..c3 = -0.0458, -0.0335, -0.1211, -0.0227 at 2.0 mm); the workload draws
+# +-5 % around it with seed 2 and checks convexity in the library.
+BASE_COEFFS = (-0.0458, -0.0335, -0.1211, -0.0227)
+
+# Operation census of the drop kernel per event (flops, one per IEEE
+# operation including div/sqrt; see DESIGN.md "Roofline"): entry of a drop
+# ray, one internal interaction, one emission (power + FP64 binning).
+CENSUS = {
+    "drop": {"miss": 10, "entry": 1230, "hit": 1150, "emit": 102},
+    "sphere": {"miss": 10, "entry": 250, "hit": 270, "emit": 102},
+}
+PEAK_TFLOPS_FP32 = 157.3   # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(records: int, rays_log2: int, nt: int, np_: int):
+    import ice_crystal_phase_amd as ip
+    rng = np.random.default_rng(2)
+    coeffs = [c * (1 + 0.05 * (2 * rng.random() - 1)) for c in BASE_COEFFS]
+    side = int(round(math.sqrt(1 << rays_log2)))
+    n_alpha = max(1, records // 3)
+    recs = []
+    for lam in (450.0, 550.0, 650.0):
+        for k in range(n_alpha):
+            alpha = -90.0 + (k + 0.5) * 180.0 / n_alpha
+            recs.append(ip.drop_record(coeffs, WATER_20C[lam], side, nt, np_, radius_mm=2.0,
+                                       inclination_deg=alpha, max_internal_hits=9, wavelength_nm=lam))
+            if len(recs) == records:
+                return recs, coeffs, side
+    return recs, coeffs, side
+
+
+def census_flops(stats_list, kinds):
+    total = 0.0
+    for st, kind in zip(stats_list, kinds):
+        c = CENSUS[kind]
+        misses = st.status[8]
+        hit = st.rays - misses
+        total += misses * c["miss"] + hit * c["entry"] + st.internal_hits * c["hit"] + st.samples * c["emit"]
+    return total
+
+
+def cpu_baseline(rec, nt: int, np_: int, budget_s: float = 12.0):
+    """Single-core oracle (FP32 mirror of the kernel) on a bounded sample of
+    record 0's rays; returns rays/s and the sample description."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_bind as ob
+    r = ob.drop_record(coeffs=tuple(rec.shape_coefficients), sphere=False,
+                       alpha_deg=math.degrees(math.atan2(-rec.incident[1], rec.incident[0])),
+                       n_int=rec.interior_index, grid=rec.grid_w, max_hits=rec.max_internal_hits)
+    total = rec.grid_w * rec.grid_h
+    start = total // 2 - rec.grid_w * (rec.grid_h // 8)   # rows through the drop's middle
+    n = 1 << 14
+    t0 = time.perf_counter()
+    ob.drop_hist(r, nt, np_, precision=32, qbits=ob.quantum_bits(total), first=start, count=n)
+    dt = time.perf_counter() - t0
+    n2 = int(min(rec.grid_w * (rec.grid_h // 4), max(n, n * (budget_s / max(dt, 1e-6)))))
+    t0 = time.perf_counter()
+    ob.drop_hist(r, nt, np_, precision=32, qbits=ob.quantum_bits(total), first=start, count=n2)
+    dt = time.perf_counter() - t0
+    return n2 / dt, n2, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--records", type=int, default=192)
+    ap.add_argument("--rays-log2", type=int, default=26)
+    ap.add_argument("--theta", type=int, default=900)
+    ap.add_argument("--phi", type=int, default=1800)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-rays", type=int, default=0)
+    ap.add_argument("--blocks", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
+              "and not launched (--gpus must be 1).", file=sys.stderr)
+        return 2
+
+    import ice_crystal_phase_amd as ip
+
+    recs, coeffs, side = workload(args.records, args.rays_log2, args.theta, args.phi)
+    ctx = ip.Context(0)
+    batch = ip.Batch(ctx, recs)
+    if args.tile_rays or args.blocks:
+        batch.set_schedule(args.tile_rays, args.blocks)
+    flags = ip.RUN_TRACE | ip.RUN_CDF | (0 if args.no_graph else ip.RUN_GRAPH)
+    for _ in range(args.warmup):
+        batch.run(flags)
+    trace_ms, cdf_ms, step_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(flags)
+        a, b, c = batch.timing()
+        trace_ms.append(a)
+        cdf_ms.append(b)
+        step_ms.append(c)
+    wall = time.perf_counter() - t0
+    stats = [batch.stats(k) for k in range(len(recs))]
+    rays_total = sum(int(s.rays) for s in stats)
+    ms_step = wall * 1e3 / args.steps
+    rays_per_s = rays_total / (ms_step / 1e3)
+    trace_avg = float(np.mean(trace_ms)) if trace_ms[0] >= 0 else float("nan")
+    cdf_avg = float(np.mean(cdf_ms)) if cdf_ms[0] >= 0 else float("nan")
+    flops = census_flops(stats, ["drop"] * len(stats))
+    achieved = flops / (trace_avg / 1e3) / 1e12 if trace_avg == trace_avg else None
+    # CDF HBM figure (algorithmic bytes: read 8 B per cell, write phi + theta)
+    nt, np_ = args.theta, args.phi
+    cdf_bytes = len(recs) * (8 * nt * np_ + 8 * (np_ + 1) + 8 * np_ * (nt + 1))
+    cdf_gbs = cdf_bytes / (cdf_avg / 1e3) / 1e9 if cdf_avg == cdf_avg else None
+    _, _, rep0 = batch.cdf(0)
+    out = {
+        "metric": "rays/s + CDF records/s on 1 MI355X vs host CPU; surrogate records/s at TV<=1e-2",
+        "value": rays_per_s,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 trace, f64 bin/CDF",
+        "data": "synthetic",
+        "config": {"workload": f"oblate drop r=2mm, 4 Chebyshev terms (seed 2), {side}x{side} rays/record, "
+                               f"{len(recs)} records ({len(recs)//3} alpha x 3 lambda), {nt}x{np_} cells, "
+                               f"<=9 internal hits, {'hipGraph' if not args.no_graph else 'eager'}",
+                   "records": len(recs), "rays_per_record": side * side, "theta": nt, "phi": np_,
+                   "shape_coefficients": [round(c, 6) for c in coeffs]},
+        "records_per_s": len(recs) / (ms_step / 1e3),
+        "cdf_records_per_s": len(recs) / (cdf_avg / 1e3) if cdf_avg == cdf_avg else None,
+        "trace_ms_per_step": trace_avg,
+        "cdf_ms_per_step": cdf_avg,
+        "cdf_hbm_gbs": cdf_gbs,
+        "g_record0": rep0.g_stored,
+        "roofline": {"kernel": "icephase_trace_drop", "bound": "valu", "achieved": achieved,
+                     "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
+                     "frac": (achieved / PEAK_TFLOPS_FP32) if achieved else None,
+                     "traffic": None, "flops_per_launch": flops},
+    }
+    if not args.no_cpu_baseline:
+        rps, n, dt = cpu_baseline(recs[len(recs) // 2], nt, np_)
+        out["cpu_baseline"] = {"value": rps, "unit": "rays/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} rays of record {len(recs)//2} (FP32 oracle mirror, same grid, "
+                                         f"binning and fixed-point histogram) in {dt:.1f} s"}
+        out["gpu_over_cpu"] = rays_per_s / rps
+    batch.close()
+    ctx.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,408 @@
+// FP64 CDF construction for gfx950, batched over records (blockIdx.y).
+//
+// Semantics follow the reference host CDF (hex_phase/src/phase_table.cpp:111-187,
+// build_host_cdf) and its GPU counterpart's audit (device/phase_cdf.cu:164-236):
+//   w = m / largest; per phi column an ORDERED binary64 prefix over theta (exact
+//   plateaus, monotone); ordered marginal over phi; zero column -> u edges;
+//   endpoints exactly 0 and 1; audit of (d phi)(d theta) against m / total.
+// Every stored CDF value is produced by the same sequence of IEEE operations as
+// the reference's ordered loops, so the arrays are bitwise the reference's for
+// the same cell masses. Reductions that only feed the report (total mass, g,
+// L1, TV) run in parallel with compensated per-lane sums.
+//
+// Layout: masses are theta-major [nt][np]; a column scan reads one row per
+// step with consecutive lanes on consecutive phi (coalesced). theta is
+// phi-major [np][nt+1]; each wave stages a 64 x 32 tile in LDS and writes it
+// back as contiguous 256-byte column segments.
+#include "kernels.hpp"
+
+namespace icephase {
+
+constexpr int kRedThreads = 256;
+constexpr int kColWave = 64;
+constexpr int kRowChunk = 32;
+
+struct Neumaier {
+    double s, c;
+    __device__ void add(double x)
+    {
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+// Cell mass of the integer source: (double)q + pole share in the end rows
+// (same operation order as the oracle's orc_bins_to_mass).
+__device__ inline double cell_mass(const CdfJob& J, uint32_t i, uint32_t j)
+{
+    if (!J.bins) return J.mass[(uint64_t)i * J.np + j];
+    double m = (double)J.bins[(uint64_t)i * J.np + j];
+    if (i == 0) m = m + (double)J.stats[kStatForward] / (double)J.np;
+    if (i + 1 == J.nt) m = m + (double)J.stats[kStatBackward] / (double)J.np;
+    return m;
+}
+
+// Mass of a stored cell: the fine mass, or the coarse buffer after aggregation.
+__device__ inline double stored_mass(const CdfJob& J, uint32_t i, uint32_t j)
+{
+    if (J.coarse) return J.coarse[(uint64_t)i * J.onp + j];
+    return cell_mass(J, i, j);
+}
+
+__device__ inline double block_sum(double v, double* sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    return t;
+}
+__device__ inline double block_max(double v, double* sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t = fmax(t, sh[k]);
+    return t;
+}
+
+// Conservative aggregation: one thread per stored cell, Kahan in (row, column)
+// order over its fine cells (phase_table.cpp:128-131 with math.hpp:120-128).
+extern "C" __global__ void __launch_bounds__(kRedThreads) icephase_cdf_coarsen(const CdfJob* __restrict__ jobs)
+{
+    const CdfJob J = jobs[blockIdx.y];
+    if (!J.coarse) return;
+    const uint32_t dt = J.nt / J.ont, dp = J.np / J.onp;
+    const uint64_t n = (uint64_t)J.ont * J.onp;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t ci = (uint32_t)(k / J.onp), cj = (uint32_t)(k % J.onp);
+        double v = 0, c = 0;
+        for (uint32_t i = ci * dt; i < (ci + 1) * dt; ++i)
+            for (uint32_t j = cj * dp; j < (cj + 1) * dp; ++j) {
+                const double y = cell_mass(J, i, j) - c, t = v + y;
+                c = (t - v) - y;
+                v = t;
+            }
+        J.coarse[k] = v;
+    }
+}
+
+// Pass 1: fine moments (mass, axial), stored moments, largest stored mass,
+// negative / non-finite count. partials: 8 doubles per block.
+extern "C" __global__ void __launch_bounds__(kRedThreads) icephase_cdf_moments(const CdfJob* __restrict__ jobs)
+{
+    __shared__ double sh[kRedThreads / 64];
+    const CdfJob J = jobs[blockIdx.y];
+    Neumaier mf{0, 0}, af{0, 0}, ms{0, 0}, as{0, 0};
+    double neg = 0, largest = 0;
+    const uint64_t n = (uint64_t)J.nt * J.np;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(k / J.np), j = (uint32_t)(k % J.np);
+        const double m = cell_mass(J, i, j);
+        if (!(m >= 0 && m <= DBL_MAX)) {
+            neg += 1;
+            continue;
+        }
+        mf.add(m);
+        af.add(m * (1 - J.edges[i] - J.edges[i + 1]));
+        largest = fmax(largest, m);
+    }
+    if (!J.coarse) {
+        ms = mf;
+        as = af;
+    }
+    const uint64_t on = J.coarse ? (uint64_t)J.ont * J.onp : 0;
+    if (J.coarse) largest = 0;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < on; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(k / J.onp), j = (uint32_t)(k % J.onp);
+        const double m = stored_mass(J, i, j);
+        if (!(m >= 0 && m <= DBL_MAX)) continue;
+        ms.add(m);
+        as.add(m * (1 - J.cedges[i] - J.cedges[i + 1]));
+        largest = fmax(largest, m);
+    }
+    const double v0 = block_max(largest, sh);
+    const double v1 = block_sum(mf.value(), sh);
+    const double v2 = block_sum(af.value(), sh);
+    const double v3 = block_sum(ms.value(), sh);
+    const double v4 = block_sum(as.value(), sh);
+    const double v5 = block_sum(neg, sh);
+    if (threadIdx.x == 0) {
+        double* p = J.partials + (uint64_t)blockIdx.x * 8;
+        p[0] = v0; p[1] = v1; p[2] = v2; p[3] = v3; p[4] = v4; p[5] = v5; p[6] = 0; p[7] = 0;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(kRedThreads)
+    icephase_cdf_moments_final(const CdfJob* __restrict__ jobs, uint32_t blocks)
+{
+    __shared__ double sh[kRedThreads / 64];
+    const CdfJob J = jobs[blockIdx.x];
+    double mx = 0;
+    Neumaier s1{0, 0}, s2{0, 0}, s3{0, 0}, s4{0, 0};
+    double neg = 0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+        const double* p = J.partials + (uint64_t)b * 8;
+        mx = fmax(mx, p[0]);
+        s1.add(p[1]); s2.add(p[2]); s3.add(p[3]); s4.add(p[4]);
+        neg += p[5];
+    }
+    const double v0 = block_max(mx, sh);
+    const double v1 = block_sum(s1.value(), sh);
+    const double v2 = block_sum(s2.value(), sh);
+    const double v3 = block_sum(s3.value(), sh);
+    const double v4 = block_sum(s4.value(), sh);
+    const double v5 = block_sum(neg, sh);
+    if (threadIdx.x == 0) {
+        J.scalars[kCdfLargest] = v0;
+        J.scalars[kCdfMassFine] = v1;
+        J.scalars[kCdfAxialFine] = v2;
+        J.scalars[kCdfMassStored] = v3;
+        J.scalars[kCdfAxialStored] = v4;
+        J.scalars[kCdfNegative] = v5;
+    }
+}
+
+// Coarsening total variation (report only; phase_table.cpp:136-143).
+extern "C" __global__ void __launch_bounds__(kRedThreads) icephase_cdf_tv(const CdfJob* __restrict__ jobs)
+{
+    __shared__ double sh[kRedThreads / 64];
+    const CdfJob J = jobs[blockIdx.y];
+    if (!J.coarse) return;
+    const uint32_t dt = J.nt / J.ont, dp = J.np / J.onp;
+    const double M = J.scalars[kCdfMassFine], Mc = J.scalars[kCdfMassStored];
+    Neumaier tv{0, 0};
+    const uint64_t n = (uint64_t)J.nt * J.np;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(k / J.np), j = (uint32_t)(k % J.np);
+        const uint32_t ci = i / dt, cj = j / dp;
+        const double fraction = (J.edges[i + 1] - J.edges[i]) / (J.cedges[ci + 1] - J.cedges[ci]) / dp;
+        tv.add(fabs(cell_mass(J, i, j) / M - J.coarse[(uint64_t)ci * J.onp + cj] / Mc * fraction));
+    }
+    const double v = block_sum(tv.value(), sh);
+    if (threadIdx.x == 0) J.partials[(uint64_t)blockIdx.x * 8 + 6] = v;
+}
+
+extern "C" __global__ void __launch_bounds__(kRedThreads)
+    icephase_cdf_tv_final(const CdfJob* __restrict__ jobs, uint32_t blocks)
+{
+    __shared__ double sh[kRedThreads / 64];
+    const CdfJob J = jobs[blockIdx.x];
+    if (!J.coarse) {
+        if (threadIdx.x == 0) J.scalars[kCdfTv] = 0;
+        return;
+    }
+    Neumaier s{0, 0};
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) s.add(J.partials[(uint64_t)b * 8 + 6]);
+    const double v = block_sum(s.value(), sh);
+    if (threadIdx.x == 0) J.scalars[kCdfTv] = 0.5 * v;
+}
+
+// Pass 2: per stored phi column, the ordered sum of w = m / largest.
+extern "C" __global__ void __launch_bounds__(kRedThreads) icephase_cdf_colsum(const CdfJob* __restrict__ jobs)
+{
+    const CdfJob J = jobs[blockIdx.y];
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J.onp) return;
+    const double largest = J.scalars[kCdfLargest];
+    double s = 0;
+    uint32_t under = 0;
+    for (uint32_t i = 0; i < J.ont; ++i) {
+        const double m = stored_mass(J, i, j);
+        const double w = m / largest;
+        under += (m > 0 && !(w > 0)) ? 1u : 0u;
+        s += w;
+    }
+    J.cols[j] = s;
+    if (under) atomicAdd(&J.scalars[kCdfUnderflow], (double)under);
+}
+
+// Pass 3: ordered marginal over phi and its normalization (phase_table.cpp:162-176).
+extern "C" __global__ void __launch_bounds__(64) icephase_cdf_marginal(const CdfJob* __restrict__ jobs)
+{
+    const CdfJob J = jobs[blockIdx.x];
+    if (threadIdx.x != 0) return;
+    double acc = 0;
+    J.phi[0] = 0;
+    uint32_t j = 0;
+    for (; j + 16 <= J.onp; j += 16) {
+        double c[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) c[k] = J.cols[j + k];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            acc = acc + c[k];
+            J.phi[j + k + 1] = acc;
+        }
+    }
+    for (; j < J.onp; ++j) {
+        acc = acc + J.cols[j];
+        J.phi[j + 1] = acc;
+    }
+    const double total = acc;
+    J.scalars[kCdfTotal] = total;
+    for (j = 1; j < J.onp; ++j) J.phi[j] = J.phi[j] / total;
+    J.phi[J.onp] = 1;
+}
+
+// Pass 4: ordered prefix again (same operations -> same values), normalized
+// into phi-major theta through an LDS transpose, with the audit of each cell.
+extern "C" __global__ void __launch_bounds__(kColWave) icephase_cdf_theta(const CdfJob* __restrict__ jobs)
+{
+    __shared__ double tile[kColWave][kRowChunk + 1];
+    const CdfJob J = jobs[blockIdx.y];
+    const uint32_t j0 = blockIdx.x * kColWave;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t j = j0 + lane;
+    const bool active = j < J.onp;
+    const uint32_t nt = J.ont;
+    const uint64_t stride = (uint64_t)nt + 1;
+    const double largest = J.scalars[kCdfLargest];
+    const double total = J.scalars[kCdfTotal];
+    const double Mc = J.scalars[kCdfMassStored];
+    const double col = active ? J.cols[j] : 0.0;
+    const double a0 = active ? J.phi[j] : 0.0, a1 = active ? J.phi[j + 1] : 0.0;
+    const double da = a1 - a0;
+    Neumaier l1{0, 0}, lost{0, 0}, rec{0, 0}, ax{0, 0};
+    double maxcell = 0, lostcells = 0, malformed = 0;
+    if (active) {
+        J.theta[(uint64_t)j * stride] = 0.0;
+        malformed += (!(a1 >= a0) || !(total > 0) || !(total <= DBL_MAX)) ? 1 : 0;
+    }
+    double s = 0, prev = 0;  // prefix and the normalized value at row i
+    for (uint32_t i0 = 0; i0 < nt; i0 += kRowChunk) {
+        const uint32_t rows = nt - i0 < (uint32_t)kRowChunk ? nt - i0 : (uint32_t)kRowChunk;
+        if (active) {
+            for (uint32_t r = 0; r < rows; ++r) {
+                const uint32_t i = i0 + r;
+                const double m = stored_mass(J, i, j);
+                s += m / largest;
+                double next;
+                if (i + 1 == nt)
+                    next = 1.0;
+                else
+                    next = col > 0 ? s / col : J.cedges[i + 1];
+                tile[lane][r] = next;
+                const double q = da * (next - prev);
+                const double p = m / Mc;
+                malformed += (!(q >= 0 && q <= DBL_MAX) || next < prev) ? 1 : 0;
+                const double err = fabs(p - q);
+                l1.add(err);
+                maxcell = fmax(maxcell, err);
+                if (p > 0 && q == 0) {
+                    lost.add(p);
+                    lostcells += 1;
+                }
+                rec.add(q);
+                ax.add(q * (1 - J.cedges[i] - J.cedges[i + 1]));
+                prev = next;
+            }
+        }
+        __syncthreads();
+        // transposed store: lanes 0-31 column 2c, lanes 32-63 column 2c+1
+        const uint32_t r = lane & 31, half = lane >> 5;
+        for (uint32_t c = 0; c < (uint32_t)kColWave / 2; ++c) {
+            const uint32_t cc = 2 * c + half;
+            const uint32_t jj = j0 + cc;
+            if (jj < J.onp && r < rows) J.theta[(uint64_t)jj * stride + i0 + r + 1] = tile[cc][r];
+        }
+        __syncthreads();
+    }
+    // wave reduction of the audit terms
+    double v[8] = {l1.value(), lost.value(), rec.value(), ax.value(), maxcell, lostcells, malformed, 0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(v[k], off, 64);
+            v[k] = k == 4 ? fmax(v[k], o) : v[k] + o;
+        }
+    }
+    if (lane == 0) {
+        double* p = J.partials + (uint64_t)blockIdx.x * 8;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) p[k] = v[k];
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(kRedThreads)
+    icephase_cdf_audit_final(const CdfJob* __restrict__ jobs, uint32_t blocks)
+{
+    __shared__ double sh[kRedThreads / 64];
+    const CdfJob J = jobs[blockIdx.x];
+    Neumaier s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    double mx = 0, lc = 0, mal = 0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+        const double* p = J.partials + (uint64_t)b * 8;
+        s[0].add(p[0]); s[1].add(p[1]); s[2].add(p[2]); s[3].add(p[3]);
+        mx = fmax(mx, p[4]);
+        lc += p[5];
+        mal += p[6];
+    }
+    const double v0 = block_sum(s[0].value(), sh);
+    const double v1 = block_sum(s[1].value(), sh);
+    const double v2 = block_sum(s[2].value(), sh);
+    const double v3 = block_sum(s[3].value(), sh);
+    const double v4 = block_max(mx, sh);
+    const double v5 = block_sum(lc, sh);
+    const double v6 = block_sum(mal, sh);
+    if (threadIdx.x == 0) {
+        J.scalars[kCdfL1] = v0;
+        J.scalars[kCdfLost] = v1;
+        J.scalars[kCdfRecovered] = v2;
+        J.scalars[kCdfAxialCdf] = v3;
+        J.scalars[kCdfMaxCell] = v4;
+        J.scalars[kCdfLostCells] = v5;
+        J.scalars[kCdfMalformed] = v6;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(64) icephase_cdf_clear(const CdfJob* __restrict__ jobs)
+{
+    const CdfJob J = jobs[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < kCdfScalars; k += blockDim.x) J.scalars[k] = 0;
+}
+
+}  // namespace icephase
+
+namespace icephase {
+
+// The whole CDF sequence for njobs records on one stream. reduce_blocks is the
+// x-extent of the reduction grids (partials hold 8 doubles per block).
+hipError_t launch_cdf(const CdfJob* d_jobs, uint32_t njobs, uint32_t max_onp, bool any_coarse, uint32_t reduce_blocks,
+                      hipStream_t s)
+{
+    const dim3 red(reduce_blocks, njobs);
+    hipLaunchKernelGGL(icephase_cdf_clear, dim3(njobs), dim3(64), 0, s, d_jobs);
+    if (any_coarse) hipLaunchKernelGGL(icephase_cdf_coarsen, red, dim3(kRedThreads), 0, s, d_jobs);
+    hipLaunchKernelGGL(icephase_cdf_moments, red, dim3(kRedThreads), 0, s, d_jobs);
+    hipLaunchKernelGGL(icephase_cdf_moments_final, dim3(njobs), dim3(kRedThreads), 0, s, d_jobs, reduce_blocks);
+    if (any_coarse) {
+        hipLaunchKernelGGL(icephase_cdf_tv, red, dim3(kRedThreads), 0, s, d_jobs);
+        hipLaunchKernelGGL(icephase_cdf_tv_final, dim3(njobs), dim3(kRedThreads), 0, s, d_jobs, reduce_blocks);
+    }
+    hipLaunchKernelGGL(icephase_cdf_colsum, dim3((max_onp + kRedThreads - 1) / kRedThreads, njobs), dim3(kRedThreads),
+                       0, s, d_jobs);
+    hipLaunchKernelGGL(icephase_cdf_marginal, dim3(njobs), dim3(64), 0, s, d_jobs);
+    const uint32_t col_blocks = (max_onp + kColWave - 1) / kColWave;
+    hipLaunchKernelGGL(icephase_cdf_theta, dim3(col_blocks, njobs), dim3(kColWave), 0, s, d_jobs);
+    hipLaunchKernelGGL(icephase_cdf_audit_final, dim3(njobs), dim3(kRedThreads), 0, s, d_jobs, col_blocks);
+    return hipGetLastError();
+}
+
+uint32_t cdf_partial_blocks(uint32_t reduce_blocks, uint32_t max_onp)
+{
+    const uint32_t col_blocks = (max_onp + kColWave - 1) / kColWave;
+    return reduce_blocks > col_blocks ? reduce_blocks : col_blocks;
+}
+
+}  // namespace icephase

@@ -1,0 +1,384 @@
+// Axisymmetric water-drop tracer, FP32, device side.
+//
+// Geometry: r(mu) = 1 + c0 + c1 mu + sum_{n>=2} c_n T_n(mu), mu = -y/r, in units
+// of the equal-volume radius (include/rainbow/raindrop_shape.hpp:70-100). The
+// reference intersects this surface with an interval-bracketed Newton solver
+// inside an OptiX custom primitive (raindrop_intersection.hpp:116-382); this
+// kernel instead uses what convexity gives for free, with no BVH:
+//   * sphere: closed-form entry from the grid coordinates, closed-form chord;
+//   * drop: the inside of a convex body along a line is one interval, so the
+//     entry root is bracketed by the bounding-sphere entry and any interior
+//     point (the foot of the perpendicular, else a golden-section search), and
+//     the exit root by a small self offset and the bounding-sphere exit; each
+//     bracket is solved by Newton with bisection fallback.
+// Splitting is per ray, FP32, two-column Jones (the Mueller M11 row for
+// unpolarized input), flux-normalized (t_flux = sqrt(nt ct / ni ci) t_E, as in
+// hex_trace/include/ice_crystal/flux_interface.hpp:26-88). Emission follows the
+// linear chain of hex_trace/include/ice_crystal/hex_trace_core.hpp:101-163:
+// external R at entry, T at every internal encounter, continue with R.
+//
+// Every operation is a separately rounded IEEE single-precision operation
+// (built with -ffp-contract=off, correctly rounded division and sqrt), so the
+// CPU mirror in the oracle reproduces each emitted direction and power bitwise.
+#pragma once
+
+#include "phase_bin.hpp"
+
+namespace icephase {
+namespace drop {
+
+struct f3 {
+    float x, y, z;
+};
+__device__ inline f3 F3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ inline f3 operator+(f3 a, f3 b) { return F3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline f3 operator-(f3 a, f3 b) { return F3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline f3 operator*(f3 a, float s) { return F3(a.x * s, a.y * s, a.z * s); }
+__device__ inline f3 operator/(f3 a, float s) { return F3(a.x / s, a.y / s, a.z / s); }
+__device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline f3 cross(f3 a, f3 b)
+{
+    return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline f3 unit(f3 a) { return a / sqrtf(dot(a, a)); }
+
+struct Shape {
+    float c[8];
+    float inner, outer;
+    uint32_t sphere;
+};
+
+__device__ inline float cheb_radius(const float* c, float mu, float& der)
+{
+    float prev = 1.0f, cur = mu, dprev = 0.0f, dcur = 1.0f;
+    float rad = (1.0f + c[0]) + c[1] * mu;
+    float d = c[1];
+#pragma unroll
+    for (int n = 2; n < 8; ++n) {
+        const float next = (2.0f * mu) * cur - prev;
+        const float dnext = (2.0f * cur + (2.0f * mu) * dcur) - dprev;
+        rad = rad + c[n] * next;
+        d = d + c[n] * dnext;
+        prev = cur;
+        cur = next;
+        dprev = dcur;
+        dcur = dnext;
+    }
+    der = d;
+    return rad;
+}
+
+__device__ inline float surface(const Shape& s, f3 p)
+{
+    const float r = sqrtf(dot(p, p));
+    if (s.sphere) return r - 1.0f;
+    if (r < s.inner * 0.25f) return r - s.inner;
+    const float mu = -p.y / r;
+    float der;
+    return r - cheb_radius(s.c, mu, der);
+}
+
+__device__ inline f3 gradient(const Shape& s, f3 p)
+{
+    const float r = sqrtf(dot(p, p));
+    const f3 u = p / r;
+    if (s.sphere) return u;
+    const float mu = -u.y;
+    float der;
+    (void)cheb_radius(s.c, mu, der);
+    const float k = der / r;
+    const f3 a = F3(0.0f - u.x * mu, -1.0f - u.y * mu, 0.0f - u.z * mu);
+    return u - a * k;
+}
+
+// Newton with bisection fallback on f(o + t d), t in [a,b]; f(a) > 0 iff a_pos.
+__device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, bool a_pos, float x)
+{
+    const float tol = 0x1p-21f;
+    if (!(x > a && x < b)) x = a + (b - a) * 0.5f;
+    for (int it = 0; it < 48; ++it) {
+        const f3 p = o + d * x;
+        const float fx = surface(s, p);
+        if (fx == 0.0f) return x;
+        if ((fx > 0.0f) == a_pos)
+            a = x;
+        else
+            b = x;
+        const float scale = fabsf(x) > 1.0f ? fabsf(x) : 1.0f;
+        if (b - a <= tol * scale) return x;
+        const float slope = dot(gradient(s, p), d);
+        const float cand = x - fx / slope;
+        if (cand > a && cand < b) {
+            const float step = fabsf(cand - x);
+            x = cand;
+            if (step <= tol * scale) return x;
+        } else {
+            x = a + (b - a) * 0.5f;
+        }
+    }
+    return x;
+}
+
+__device__ inline bool entry(const Shape& s, f3 o, f3 k, float u, float v, float& t)
+{
+    const float rho2 = u * u + v * v;
+    if (s.sphere) {
+        if (!(rho2 < 1.0f)) return false;
+        t = -sqrtf(1.0f - rho2);
+        return true;
+    }
+    const float out2 = s.outer * s.outer;
+    if (!(rho2 < out2)) return false;
+    const float ta = -sqrtf(out2 - rho2);
+    float tm = 0.0f;
+    const float fm = surface(s, o);
+    if (!(fm < 0.0f)) {
+        const float g = 0.6180339887498949f;
+        float lo = ta, hi = -ta;
+        float c1 = hi - g * (hi - lo), c2 = lo + g * (hi - lo);
+        float f1 = surface(s, o + k * c1);
+        float f2 = surface(s, o + k * c2);
+        bool found = false;
+        for (int it = 0; it < 28; ++it) {
+            if (f1 < 0.0f) {
+                tm = c1;
+                found = true;
+                break;
+            }
+            if (f2 < 0.0f) {
+                tm = c2;
+                found = true;
+                break;
+            }
+            if (f1 < f2) {
+                hi = c2;
+                c2 = c1;
+                f2 = f1;
+                c1 = hi - g * (hi - lo);
+                f1 = surface(s, o + k * c1);
+            } else {
+                lo = c1;
+                c1 = c2;
+                f1 = f2;
+                c2 = lo + g * (hi - lo);
+                f2 = surface(s, o + k * c2);
+            }
+        }
+        if (!found) return false;
+    }
+    const float rm = 1.0f + s.c[0];
+    const float g2 = rm * rm - rho2;
+    const float x0 = g2 > 0.0f ? -sqrtf(g2) : ta + (tm - ta) * 0.5f;
+    t = newton(s, o, k, ta, tm, true, x0);
+    return true;
+}
+
+__device__ inline bool chord(const Shape& s, f3 p, f3 d, float& t)
+{
+    const float b = dot(p, d);
+    if (s.sphere) {
+        const float c = dot(p, p) - 1.0f;
+        float disc = b * b - c;
+        if (disc < 0.0f) disc = 0.0f;
+        t = -b + sqrtf(disc);
+        return true;
+    }
+    const float cb = dot(p, p) - s.outer * s.outer;
+    float disc = b * b - cb;
+    if (disc < 0.0f) disc = 0.0f;
+    const float tb = -b + sqrtf(disc);
+    const float ts = 0x1p-16f;
+    if (!(ts < tb)) return false;
+    if (!(surface(s, p + d * ts) < 0.0f)) return false;
+    t = newton(s, p, d, ts, tb, false, -2.0f * b);
+    return true;
+}
+
+struct c32 {
+    float re, im;
+};
+__device__ inline c32 cmul(c32 a, c32 b) { return c32{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline c32 cscale(c32 a, float b) { return c32{a.re * b, a.im * b}; }
+__device__ inline c32 cadd(c32 a, c32 b) { return c32{a.re + b.re, a.im + b.im}; }
+__device__ inline float cn2(c32 a) { return a.re * a.re + a.im * a.im; }
+
+struct Jones {
+    c32 c0x, c0y, c1x, c1y;
+    __device__ float power() const { return 0.5f * (((cn2(c0x) + cn2(c0y)) + cn2(c1x)) + cn2(c1y)); }
+};
+
+struct Ray {
+    f3 dir, e0, e1;
+    Jones j;
+};
+struct Split {
+    Ray r, t;
+    bool ok, has_t, tir;
+};
+
+__device__ inline c32 tir_coefficient(float a, float b)
+{
+    const float s = fabsf(a) > fabsf(b) ? fabsf(a) : fabsf(b);
+    const float x = a / s, y = b / s, den = x * x + y * y;
+    return c32{(x * x - y * y) / den, (-2.0f * x * y) / den};
+}
+
+// n points into the incident medium.
+__device__ inline Split split(const Ray& in, f3 n, float ni, float nt)
+{
+    Split o;
+    o.ok = false;
+    o.has_t = false;
+    o.tir = false;
+    const f3 d = in.dir;
+    float ci = -dot(d, n);
+    if (!(ci >= -1e-3f && ni > 0.0f && nt > 0.0f)) return o;
+    if (ci < 0.0f) ci = 0.0f;
+    if (ci > 1.0f) ci = 1.0f;
+    f3 s = cross(d, n);
+    const float s2 = dot(s, s);
+    const float sine = sqrtf(s2);
+    if (s2 > 0x1p-40f)
+        s = s / sine;
+    else
+        s = unit(in.e0 - d * dot(d, in.e0));
+    const f3 pin = unit(cross(d, s));
+    const float a = dot(in.e0, s), b = dot(in.e1, s);
+    const float cc = dot(in.e0, pin), dd = dot(in.e1, pin);
+    Jones j;
+    j.c0x = cadd(cscale(in.j.c0x, a), cscale(in.j.c0y, b));
+    j.c0y = cadd(cscale(in.j.c0x, cc), cscale(in.j.c0y, dd));
+    j.c1x = cadd(cscale(in.j.c1x, a), cscale(in.j.c1y, b));
+    j.c1y = cadd(cscale(in.j.c1x, cc), cscale(in.j.c1y, dd));
+    const float eta = ni / nt;
+    const float st = eta * sine;
+    o.r.dir = unit(d + n * (2.0f * ci));
+    o.r.e0 = s;
+    o.r.e1 = unit(cross(o.r.dir, s));
+    c32 rs, rp;
+    if (st > 1.0f) {
+        const float beta = sqrtf((st - 1.0f) * (st + 1.0f));
+        rs = tir_coefficient(ni * ci, nt * beta);
+        rp = tir_coefficient(nt * ci, ni * beta);
+        o.tir = true;
+    } else {
+        const float ct = sqrtf((1.0f - st) * (1.0f + st));
+        const float ds = ni * ci + nt * ct, dp = nt * ci + ni * ct;
+        if (!(ds > 0.0f && dp > 0.0f)) return o;
+        rs = c32{(ni * ci - nt * ct) / ds, 0.0f};
+        rp = c32{(nt * ci - ni * ct) / dp, 0.0f};
+        const float num = (2.0f * sqrtf(ni * ci)) * sqrtf(nt * ct);
+        const float ts = num / ds, tp = num / dp;
+        o.t.dir = unit((d + n * ci) * eta - n * ct);
+        o.t.e0 = s;
+        o.t.e1 = unit(cross(o.t.dir, s));
+        o.t.j.c0x = cscale(j.c0x, ts);
+        o.t.j.c0y = cscale(j.c0y, tp);
+        o.t.j.c1x = cscale(j.c1x, ts);
+        o.t.j.c1y = cscale(j.c1y, tp);
+        o.has_t = true;
+    }
+    o.r.j.c0x = cmul(j.c0x, rs);
+    o.r.j.c0y = cmul(j.c0y, rp);
+    o.r.j.c1x = cmul(j.c1x, rs);
+    o.r.j.c1y = cmul(j.c1y, rp);
+    o.ok = true;
+    return o;
+}
+
+__device__ inline f3 outward_normal(const Shape& s, f3 p) { return unit(gradient(s, p)); }
+
+struct Config {
+    Shape shape;
+    f3 ki, e0, e1;
+    float n_int, n_ext, h, tail;
+    uint32_t w, hgt, max_hits;
+};
+
+enum Status : uint32_t {
+    kExhausted = 0,
+    kTail = 1,
+    kLimit = 2,
+    kUnresolved = 3,
+    kNumeric = 5,
+    kMiss = 8,
+};
+
+struct RayResult {
+    float unresolved;
+    uint32_t status, internal_hits, tir, outputs;
+};
+
+template <class Sink>
+__device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
+{
+    RayResult res{0.0f, kMiss, 0, 0, 0};
+    const uint32_t ix = (uint32_t)(id % c.w), iy = (uint32_t)(id / c.w);
+    const float u = ((float)(2u * ix + 1u) / (float)c.w - 1.0f) * c.h;
+    const float v = ((float)(2u * iy + 1u) / (float)c.hgt - 1.0f) * c.h;
+    const f3 o = c.e0 * u + c.e1 * v;
+    float t;
+    if (!entry(c.shape, o, c.ki, u, v, t)) return res;
+    f3 p = o + c.ki * t;
+    Ray ray;
+    ray.dir = c.ki;
+    ray.e0 = c.e0;
+    ray.e1 = c.e1;
+    ray.j = Jones{c32{1.0f, 0.0f}, c32{0.0f, 0.0f}, c32{0.0f, 0.0f}, c32{1.0f, 0.0f}};
+    Split sp = split(ray, outward_normal(c.shape, p), c.n_ext, c.n_int);
+    if (!sp.ok) {
+        res.status = kNumeric;
+        res.unresolved = 1.0f;
+        return res;
+    }
+    auto emit = [&](const Ray& r) {
+        const float pw = r.j.power();
+        if (pw > 0.0f) {
+            sink(r.dir, pw);
+            ++res.outputs;
+        }
+    };
+    emit(sp.r);
+    if (!sp.has_t) {
+        res.tir = 1;
+        res.status = kExhausted;
+        return res;
+    }
+    ray = sp.t;
+    res.status = kLimit;
+    for (uint32_t hit = 0; hit < c.max_hits; ++hit) {
+        const float pw = ray.j.power();
+        if (pw == 0.0f) {
+            res.status = kExhausted;
+            break;
+        }
+        if (pw <= c.tail) {
+            res.status = kTail;
+            break;
+        }
+        float tx;
+        if (!chord(c.shape, p, ray.dir, tx)) {
+            res.status = kUnresolved;
+            break;
+        }
+        const f3 q = p + ray.dir * tx;
+        const f3 nout = outward_normal(c.shape, q);
+        sp = split(ray, nout * -1.0f, c.n_int, c.n_ext);
+        if (!sp.ok) {
+            res.status = kNumeric;
+            break;
+        }
+        ++res.internal_hits;
+        if (sp.tir) ++res.tir;
+        if (sp.has_t) emit(sp.t);
+        ray = sp.r;
+        p = q;
+    }
+    const float rem = ray.j.power();
+    if (res.status == kLimit && rem == 0.0f) res.status = kExhausted;
+    res.unresolved = rem;
+    return res;
+}
+
+}  // namespace drop
+}  // namespace icephase

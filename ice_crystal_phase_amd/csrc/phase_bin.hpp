@@ -1,0 +1,160 @@
+// Direction -> (u, phi) cell binning in FP64 and the fixed-point quantizer,
+// shared by every trace kernel. Device code (host-callable for the C-ABI's
+// scalar helpers).
+//
+// phase_bin follows hex_phase/include/ice_crystal/phase_math.hpp:37-81: FP64
+// frame projection, pole test s <= 256 eps, stable u from (mu, s), half-open u
+// edges with the 64-eps / 1e-7-cell snap, phi = atan2(y, x) with the periodic
+// seam and the same snap rule. Two changes of method keep the result and cut
+// the cost on the GPU:
+//   * the u row is guessed from u = sin^2(pi i / 2n) in FP32 and then corrected
+//     against the FP64 edges, which returns the same row as the reference's
+//     binary search for any u in [0,1];
+//   * the phi column is taken from an FP32 atan2 when the FP32 column index is
+//     more than PHI_MARGIN away from every integer (its error is below 1e-3
+//     columns there, so floor() and the no-snap outcome are certain) and from
+//     the FP64 atan2 otherwise.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+namespace icephase {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kSqrt3 = 1.732050807568877293527446341505872367;
+
+struct d3 {
+    double x, y, z;
+};
+__host__ __device__ inline d3 D3(double x, double y, double z) { return d3{x, y, z}; }
+__host__ __device__ inline d3 operator+(d3 a, d3 b) { return D3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline d3 operator-(d3 a, d3 b) { return D3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline d3 operator-(d3 a) { return D3(-a.x, -a.y, -a.z); }
+__host__ __device__ inline d3 operator*(d3 a, double s) { return D3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline d3 operator/(d3 a, double s) { return D3(a.x / s, a.y / s, a.z / s); }
+__host__ __device__ inline double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline d3 cross(d3 a, d3 b)
+{
+    return D3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline bool finite64(double v) { return v >= -DBL_MAX && v <= DBL_MAX; }
+__host__ __device__ inline bool finite64(d3 a) { return finite64(a.x) && finite64(a.y) && finite64(a.z); }
+__host__ __device__ inline double gt_max(double a, double b) { return a > b ? a : b; }
+__host__ __device__ inline double max_abs(d3 a) { return gt_max(fabs(a.x), gt_max(fabs(a.y), fabs(a.z))); }
+
+// Max-abs scaled normalization (hex_trace/include/ice_crystal/math.hpp:38-47).
+__host__ __device__ inline bool unit_of(d3 a, d3& out)
+{
+    const double s = max_abs(a);
+    if (!(s > 0) || !finite64(s)) return false;
+    a = a / s;
+    const double n = sqrt(dot(a, a));
+    if (!(n > 0) || !finite64(n)) return false;
+    out = a / n;
+    return finite64(out);
+}
+__host__ __device__ inline bool is_unit64(d3 a)
+{
+    return finite64(a) && fabs(dot(a, a) - 1.0) <= 128 * DBL_EPSILON;
+}
+
+// Binning frame of one record: (e0, e1, ki), right handed, FP64.
+struct BinFrame {
+    d3 ki, e0, e1;
+};
+
+struct CellHit {
+    uint32_t index;  // i*np + j when kind == 0
+    int32_t kind;    // 0 cell, +1 forward pole, -1 backward pole, 2 invalid
+    uint32_t snapped;
+};
+
+constexpr double kPhiMargin = 2e-3;
+
+// edges: nt+1 FP64 u edges (any address space visible to the caller).
+template <class EdgeLoad>
+__host__ __device__ inline CellHit bin_direction(const BinFrame& g, uint32_t nt, uint32_t np, d3 d,
+                                                 EdgeLoad edge)
+{
+    CellHit h{0, 2, 0};
+    if (!is_unit64(d)) return h;
+    const double x = dot(g.e0, d), y = dot(g.e1, d), z = dot(g.ki, d);
+    const double n = sqrt(x * x + y * y + z * z);
+    if (!(n > 0) || !finite64(n) || fabs(n - 1) > 512 * DBL_EPSILON) return h;
+    const double mu = z / n, s = hypot(x, y) / n;
+    if (s <= 256 * DBL_EPSILON) {
+        h.kind = mu >= 0 ? 1 : -1;
+        return h;
+    }
+    const double u = mu >= 0 ? (0.5 * s * s) / (1 + mu) : 1 - (0.5 * s * s) / (1 - mu);
+    if (!(u >= 0 && u <= 1)) return h;
+    // Row: FP32 guess from u = sin^2(pi i/2n), corrected against the FP64 edges.
+    const float uf = (float)u;
+    int lo = (int)(asinf(sqrtf(uf)) * (float)(2.0 / kPi) * (float)nt);
+    lo = lo < 0 ? 0 : (lo > (int)nt - 1 ? (int)nt - 1 : lo);
+    while (lo > 0 && u < edge(lo)) --lo;
+    while (lo + 1 < (int)nt && u >= edge(lo + 1)) ++lo;
+    uint32_t snapped = 0;
+    for (int cand = 0; cand < 2; ++cand) {
+        const int e = lo + cand;
+        if (e == 0 || e >= (int)nt) continue;
+        const double ee = edge(e);
+        const double width = fmin(ee - edge(e - 1), edge(e + 1) - ee);
+        const double tol = fmin(64 * DBL_EPSILON * fmax(fabs(u), fabs(ee)), 1e-7 * width);
+        if (fabs(u - ee) <= tol) {
+            snapped = (u != ee);
+            lo = e;
+            break;
+        }
+    }
+    // Column.
+    uint32_t j;
+    {
+        const float phif = atan2f((float)y, (float)x);
+        const float idxf = (phif + (float)kPi) * (float)(1.0 / (2.0 * kPi)) * (float)np;
+        const float fl = floorf(idxf);
+        const float fr = idxf - fl;
+        if (fr > (float)kPhiMargin && fr < 1.0f - (float)kPhiMargin && fl >= 0.0f && fl < (float)np) {
+            j = (uint32_t)fl;
+        } else {
+            double phi = atan2(y, x);
+            if (phi >= kPi) phi -= 2 * kPi;
+            const double v = (phi + kPi) / (2 * kPi);
+            double index = v * np;
+            const double nearest = floor(index + 0.5);
+            if (fabs(index - nearest) <= fmin(64 * DBL_EPSILON * np, 1e-7)) {
+                snapped = snapped || (index != nearest);
+                index = nearest;
+            }
+            j = (uint32_t)index;
+            if (j == np) j = 0;
+            if (j >= np) return h;
+        }
+    }
+    h.index = (uint32_t)lo * np + j;
+    h.kind = 0;
+    h.snapped = snapped;
+    return h;
+}
+
+// Fixed-point quantum: rint(p * 2^qbits) for a power fraction p in [0, 1].
+__host__ __device__ inline uint64_t quantize(double p, uint32_t qbits)
+{
+    return (uint64_t)rint(ldexp(p, (int)qbits));
+}
+
+// Q = 62 - ceil(log2(rays)), clamped to [8, 52]: the sum over all rays of a record
+// (each ray's emitted quanta total at most 2^Q plus rounding) stays below 2^63.
+__host__ __device__ inline uint32_t quantum_bits(uint64_t rays)
+{
+    uint32_t lg = 0;
+    while (lg < 64 && (uint64_t(1) << lg) < rays) ++lg;
+    int q = 62 - (int)lg;
+    q = q > 52 ? 52 : q;
+    q = q < 8 ? 8 : q;
+    return (uint32_t)q;
+}
+
+}  // namespace icephase

@@ -1,0 +1,682 @@
+// C-ABI runtime of libicephase.so (declarations and reference citations in
+// include/icephase.h). One HIP device and one default-priority stream per
+// context; every entry point is synchronous; errors are returned as codes with
+// the message kept per context (or per thread for context-free calls).
+#include "host_records.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace icephase {
+hipError_t launch_zero(const DevRecord* d_recs, uint32_t nrec, uint32_t* d_queue, hipStream_t s);
+hipError_t launch_trace(int kind, bool wide, const DevRecord* d_recs, TileMap map, uint32_t* d_queue,
+                        uint32_t blocks, uint32_t threads, hipStream_t s);
+hipError_t launch_bin_dirs(const BinFrame& g, uint32_t nt, uint32_t np, const double* d_edges, uint64_t n,
+                           const double* d_dirs, uint32_t* d_index, int32_t* d_kind, hipStream_t s);
+hipError_t launch_cdf(const CdfJob* d_jobs, uint32_t njobs, uint32_t max_onp, bool any_coarse, uint32_t reduce_blocks,
+                      hipStream_t s);
+uint32_t cdf_partial_blocks(uint32_t reduce_blocks, uint32_t max_onp);
+}  // namespace icephase
+
+using namespace icephase;
+
+struct iph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    std::string err;
+    std::vector<iph_batch*> batches;  // live batches; destroyed with the context
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(iph_ctx* c, int code, const std::string& msg)
+{
+    if (c)
+        c->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+
+#define IPH_HIP(ctx, call)                                                                                  \
+    do {                                                                                                    \
+        const hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                               \
+            return fail((ctx), IPH_E_DEVICE,                                                                \
+                        std::string(#call) + " failed: " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
+                            std::to_string(__LINE__) + ")");                                                \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t count)
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (!count) return hipSuccess;
+        return hipMalloc(&p, count * sizeof(T));
+    }
+};
+
+struct CdfSetup {
+    uint32_t ont, onp;
+    double max_l1, max_lost, max_tv;
+};
+
+bool resolve_policy(const iph_cdf_policy* p, uint32_t nt, uint32_t np, CdfSetup& s, std::string& err)
+{
+    s.ont = p && p->out_theta ? p->out_theta : nt;
+    s.onp = p && p->out_phi ? p->out_phi : np;
+    s.max_l1 = p ? p->max_l1 : 1e-10;
+    s.max_lost = p ? p->max_lost_mass : 1e-12;
+    s.max_tv = p ? p->max_coarsening_tv : 1.0;
+    if (!s.ont || !s.onp || s.ont > nt || s.onp > np || nt % s.ont || np % s.onp) {
+        err = "Storage dimensions must divide histogram dimensions.";
+        return false;
+    }
+    if (!(std::isfinite(s.max_l1) && s.max_l1 >= 0 && s.max_l1 <= 2 && std::isfinite(s.max_lost) &&
+          s.max_lost >= 0 && s.max_lost <= 1 && std::isfinite(s.max_tv) && s.max_tv >= 0 && s.max_tv <= 1)) {
+        err = "Invalid CDF quality policy.";
+        return false;
+    }
+    return true;
+}
+
+// Acceptance rules of build_host_cdf (hex_phase/src/phase_table.cpp:111-187) and
+// PhaseCdf::build (src/phase_cdf.cpp:448-468) applied to the device scalars.
+iph_cdf_report make_report(const double* sc, const CdfSetup& s, bool coarse)
+{
+    iph_cdf_report r;
+    std::memset(&r, 0, sizeof r);
+    r.fine_mass = sc[kCdfMassFine];
+    r.stored_mass = sc[kCdfMassStored];
+    r.g_source = sc[kCdfAxialFine] / sc[kCdfMassFine];
+    r.aggregation_error = coarse ? std::fabs(sc[kCdfMassStored] / sc[kCdfMassFine] - 1) : 0.0;
+    r.coarsening_tv = sc[kCdfTv];
+    r.l1_error = sc[kCdfL1];
+    r.lost_mass = sc[kCdfLost];
+    r.max_cell_error = sc[kCdfMaxCell];
+    r.cdf_mass = sc[kCdfRecovered];
+    r.g_stored = sc[kCdfAxialCdf] / sc[kCdfRecovered];
+    r.lost_cells = (uint64_t)sc[kCdfLostCells];
+    r.malformed = (uint64_t)sc[kCdfMalformed];
+    r.weight_underflow = (uint64_t)sc[kCdfUnderflow];
+    r.negative = (uint64_t)sc[kCdfNegative];
+    if (r.negative)
+        r.status = 1;  // negative / non-finite mass
+    else if (!(sc[kCdfMassFine] > 0) || !std::isfinite(sc[kCdfMassFine]))
+        r.status = 2;  // empty histogram
+    else if (!std::isfinite(r.coarsening_tv) || r.coarsening_tv > s.max_tv || r.aggregation_error > 1e-12)
+        r.status = 3;  // coarsening audit
+    else if (r.weight_underflow)
+        r.status = 4;
+    else if (r.malformed || !(sc[kCdfTotal] > 0))
+        r.status = 5;
+    else if (r.l1_error > s.max_l1 || r.lost_mass > s.max_lost || std::fabs(r.cdf_mass - 1) > 1e-10 ||
+             !std::isfinite(r.g_stored) || !(r.g_stored > -1 && r.g_stored < 1))
+        r.status = 6;  // CDF mass / HG audit
+    return r;
+}
+
+const char* report_reason(int status)
+{
+    switch (status) {
+    case 1: return "Negative/nonfinite mass.";
+    case 2: return "Empty/nonfinite histogram.";
+    case 3: return "Conservative coarsening audit failed.";
+    case 4: return "CDF weight underflow.";
+    case 5: return "Malformed CDF; refusing repair.";
+    case 6: return "CDF mass / HG audit failed.";
+    default: return "ok";
+    }
+}
+
+}  // namespace
+
+struct iph_batch {
+    iph_ctx* ctx = nullptr;
+    uint32_t nrec = 0;
+    std::vector<iph_record> recs;
+    std::vector<DevRecord> host;
+    std::vector<CdfSetup> cdf;
+    std::vector<double> weight, area;
+    std::vector<std::vector<double>> edges_host, cedges_host;
+    // device
+    DevBuf<DevRecord> d_recs;
+    DevBuf<unsigned long long> d_bins, d_stats;
+    DevBuf<double> d_edges;
+    DevBuf<uint32_t> d_queue;
+    DevBuf<uint32_t> d_kind_recs;
+    DevBuf<uint64_t> d_tile_start;
+    DevBuf<CdfJob> d_jobs;
+    DevBuf<double> d_cdf;  // phi, theta, cols, scalars, partials, coarse
+    std::vector<size_t> off_phi, off_theta;
+    std::vector<size_t> off_bins;
+    std::vector<CdfJob> jobs;
+    TileMap map[2]{};
+    bool wide[2] = {false, false};
+    uint32_t max_onp = 1, reduce_blocks = 512;
+    bool any_coarse = false;
+    uint32_t blocks[2] = {0, 0};
+    uint64_t tile_rays = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_flags = 0;
+    bool graph_events = false;
+    double ms[3] = {0, 0, 0};
+    ~iph_batch()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+extern "C" {
+
+int iph_abi_version(void) { return IPH_ABI_VERSION; }
+
+uint32_t iph_struct_size(int which)
+{
+    switch (which) {
+    case 0: return (uint32_t)sizeof(iph_record);
+    case 1: return (uint32_t)sizeof(iph_hist_stats);
+    case 2: return (uint32_t)sizeof(iph_cdf_report);
+    case 3: return (uint32_t)sizeof(iph_cdf_policy);
+    default: return 0;
+    }
+}
+
+const char* iph_last_error(const iph_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int iph_ctx_create(int device, iph_ctx** out)
+{
+    if (!out) return fail(nullptr, IPH_E_INVALID, "out must not be NULL.");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return fail(nullptr, IPH_E_DEVICE, std::string("No HIP device available: ") + hipGetErrorString(e));
+    if (device < 0 || device >= count) return fail(nullptr, IPH_E_INVALID, "Invalid device ordinal.");
+    auto c = std::make_unique<iph_ctx>();
+    c->device = device;
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, IPH_E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+        return fail(nullptr, IPH_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cus = prop.multiProcessorCount;
+    *out = c.release();
+    return IPH_OK;
+}
+
+void iph_ctx_destroy(iph_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    // batches still alive are owned by the context: release them first
+    const std::vector<iph_batch*> live = ctx->batches;
+    for (iph_batch* b : live) iph_batch_destroy(b);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int iph_u_edges(uint32_t nt, double* out)
+{
+    if (!out) return fail(nullptr, IPH_E_INVALID, "out must not be NULL.");
+    std::vector<double> e;
+    std::string err;
+    if (!make_u_edges(nt, e, err)) return fail(nullptr, IPH_E_INVALID, err);
+    std::memcpy(out, e.data(), e.size() * sizeof(double));
+    return IPH_OK;
+}
+
+int iph_bin_directions(iph_ctx* ctx, uint32_t nt, uint32_t np, const double* frame, uint64_t n, const double* dirs,
+                       uint32_t* index, int32_t* kind)
+{
+    if (!ctx) return fail(nullptr, IPH_E_INVALID, "ctx must not be NULL.");
+    if (!frame || (n && (!dirs || !index || !kind)) || !nt || !np || (uint64_t)nt * np > 0xffffffffull)
+        return fail(ctx, IPH_E_INVALID, "Invalid binning arguments.");
+    if (!n) return IPH_OK;
+    std::vector<double> e;
+    std::string err;
+    if (!make_u_edges(nt, e, err)) return fail(ctx, IPH_E_INVALID, err);
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    BinFrame g;
+    g.ki = D3(frame[0], frame[1], frame[2]);
+    g.e0 = D3(frame[3], frame[4], frame[5]);
+    g.e1 = D3(frame[6], frame[7], frame[8]);
+    DevBuf<double> de, dd;
+    DevBuf<uint32_t> di;
+    DevBuf<int32_t> dk;
+    IPH_HIP(ctx, de.alloc(e.size()));
+    IPH_HIP(ctx, dd.alloc(3 * n));
+    IPH_HIP(ctx, di.alloc(n));
+    IPH_HIP(ctx, dk.alloc(n));
+    IPH_HIP(ctx, hipMemcpyAsync(de.p, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    IPH_HIP(ctx, hipMemcpyAsync(dd.p, dirs, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    IPH_HIP(ctx, launch_bin_dirs(g, nt, np, de.p, n, dd.p, di.p, dk.p, ctx->stream));
+    IPH_HIP(ctx, hipMemcpyAsync(index, di.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    IPH_HIP(ctx, hipMemcpyAsync(kind, dk.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return IPH_OK;
+}
+
+int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const iph_cdf_policy* policy,
+                     iph_batch** out)
+{
+    if (!ctx) return fail(nullptr, IPH_E_INVALID, "ctx must not be NULL.");
+    if (!out || !recs || nrec == 0) return fail(ctx, IPH_E_INVALID, "Need at least one record.");
+    *out = nullptr;
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    auto b = std::make_unique<iph_batch>();
+    b->ctx = ctx;
+    b->nrec = nrec;
+    b->recs.assign(recs, recs + nrec);
+    b->host.resize(nrec);
+    b->cdf.resize(nrec);
+    b->weight.resize(nrec);
+    b->area.resize(nrec);
+    b->edges_host.resize(nrec);
+    b->cedges_host.resize(nrec);
+    std::string err;
+    size_t bins_total = 0, edges_total = 0;
+    b->off_bins.resize(nrec);
+    std::vector<size_t> off_edges(nrec), off_cedges(nrec);
+    for (uint32_t r = 0; r < nrec; ++r) {
+        if (!prepare_record(recs[r], 0, b->host[r], b->weight[r], b->area[r], err))
+            return fail(ctx, IPH_E_INVALID, "record " + std::to_string(r) + ": " + err);
+        if (!resolve_policy(policy, recs[r].theta_count, recs[r].phi_count, b->cdf[r], err))
+            return fail(ctx, IPH_E_INVALID, "record " + std::to_string(r) + ": " + err);
+        if (!make_u_edges(recs[r].theta_count, b->edges_host[r], err)) return fail(ctx, IPH_E_INVALID, err);
+        const uint32_t dt = recs[r].theta_count / b->cdf[r].ont;
+        b->cedges_host[r].resize(b->cdf[r].ont + 1);
+        for (uint32_t i = 0; i <= b->cdf[r].ont; ++i) b->cedges_host[r][i] = b->edges_host[r][(size_t)i * dt];
+        b->off_bins[r] = bins_total;
+        bins_total += (size_t)recs[r].theta_count * recs[r].phi_count;
+        off_edges[r] = edges_total;
+        edges_total += b->edges_host[r].size();
+        off_cedges[r] = edges_total;
+        edges_total += b->cedges_host[r].size();
+        if (b->host[r].nt + 1 > 2048) b->wide[b->host[r].kind] = true;
+    }
+    IPH_HIP(ctx, b->d_recs.alloc(nrec));
+    IPH_HIP(ctx, b->d_bins.alloc(bins_total));
+    IPH_HIP(ctx, b->d_stats.alloc((size_t)nrec * kStatCount));
+    IPH_HIP(ctx, b->d_edges.alloc(edges_total));
+    IPH_HIP(ctx, b->d_queue.alloc(4));
+    {
+        std::vector<double> all(edges_total);
+        for (uint32_t r = 0; r < nrec; ++r) {
+            std::copy(b->edges_host[r].begin(), b->edges_host[r].end(), all.begin() + off_edges[r]);
+            std::copy(b->cedges_host[r].begin(), b->cedges_host[r].end(), all.begin() + off_cedges[r]);
+        }
+        IPH_HIP(ctx, hipMemcpy(b->d_edges.p, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (uint32_t r = 0; r < nrec; ++r) {
+        b->host[r].bins = b->d_bins.p + b->off_bins[r];
+        b->host[r].stats = b->d_stats.p + (size_t)r * kStatCount;
+        b->host[r].edges = b->d_edges.p + off_edges[r];
+    }
+    // CDF buffers
+    size_t cdf_total = 0;
+    b->off_phi.resize(nrec);
+    b->off_theta.resize(nrec);
+    std::vector<size_t> off_cols(nrec), off_sc(nrec), off_part(nrec), off_coarse(nrec);
+    for (uint32_t r = 0; r < nrec; ++r) b->max_onp = std::max(b->max_onp, b->cdf[r].onp);
+    const uint32_t pblocks = cdf_partial_blocks(b->reduce_blocks, b->max_onp);
+    for (uint32_t r = 0; r < nrec; ++r) {
+        const CdfSetup& s = b->cdf[r];
+        const bool coarse = s.ont != recs[r].theta_count || s.onp != recs[r].phi_count;
+        b->any_coarse = b->any_coarse || coarse;
+        b->off_phi[r] = cdf_total; cdf_total += s.onp + 1;
+        b->off_theta[r] = cdf_total; cdf_total += (size_t)s.onp * (s.ont + 1);
+        off_cols[r] = cdf_total; cdf_total += s.onp;
+        off_sc[r] = cdf_total; cdf_total += kCdfScalars;
+        off_part[r] = cdf_total; cdf_total += (size_t)pblocks * 8;
+        off_coarse[r] = cdf_total; cdf_total += coarse ? (size_t)s.ont * s.onp : 0;
+    }
+    IPH_HIP(ctx, b->d_cdf.alloc(cdf_total));
+    b->jobs.resize(nrec);
+    for (uint32_t r = 0; r < nrec; ++r) {
+        const CdfSetup& s = b->cdf[r];
+        const bool coarse = s.ont != recs[r].theta_count || s.onp != recs[r].phi_count;
+        CdfJob& J = b->jobs[r];
+        J.nt = recs[r].theta_count; J.np = recs[r].phi_count; J.ont = s.ont; J.onp = s.onp;
+        J.bins = b->host[r].bins; J.mass = nullptr; J.stats = b->host[r].stats;
+        J.edges = b->d_edges.p + off_edges[r]; J.cedges = b->d_edges.p + off_cedges[r];
+        J.coarse = coarse ? b->d_cdf.p + off_coarse[r] : nullptr;
+        J.phi = b->d_cdf.p + b->off_phi[r]; J.theta = b->d_cdf.p + b->off_theta[r];
+        J.cols = b->d_cdf.p + off_cols[r]; J.scalars = b->d_cdf.p + off_sc[r];
+        J.partials = b->d_cdf.p + off_part[r];
+    }
+    IPH_HIP(ctx, b->d_jobs.alloc(nrec));
+    IPH_HIP(ctx, hipMemcpy(b->d_jobs.p, b->jobs.data(), nrec * sizeof(CdfJob), hipMemcpyHostToDevice));
+    for (auto& e : b->ev) IPH_HIP(ctx, hipEventCreate(&e));
+    iph_batch* raw = b.release();
+    const int rc = iph_batch_set_schedule(raw, 0, 0);
+    if (rc != IPH_OK) {
+        delete raw;
+        return rc;
+    }
+    ctx->batches.push_back(raw);
+    *out = raw;
+    return IPH_OK;
+}
+
+void iph_batch_destroy(iph_batch* b)
+{
+    if (!b) return;
+    iph_ctx* ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    auto it = std::find(ctx->batches.begin(), ctx->batches.end(), b);
+    if (it != ctx->batches.end()) ctx->batches.erase(it);
+    delete b;
+}
+
+int iph_batch_set_schedule(iph_batch* b, uint64_t tile_rays, uint32_t blocks)
+{
+    if (!b) return fail(nullptr, IPH_E_INVALID, "batch must not be NULL.");
+    iph_ctx* ctx = b->ctx;
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    b->tile_rays = tile_rays;
+    std::vector<uint32_t> kind_recs[2];
+    std::vector<uint64_t> starts[2];
+    for (int k = 0; k < 2; ++k) starts[k].push_back(0);
+    for (uint32_t r = 0; r < b->nrec; ++r) {
+        DevRecord& R = b->host[r];
+        uint64_t t = tile_rays;
+        if (!t) {
+            // default: ~64 tiles per record, 2048..65536 rays per tile
+            t = R.rays / 64;
+            t = t < 2048 ? 2048 : (t > 65536 ? 65536 : t);
+        }
+        R.tile_rays = t;
+        kind_recs[R.kind].push_back(r);
+        starts[R.kind].push_back(starts[R.kind].back() + (R.rays + t - 1) / t);
+    }
+    IPH_HIP(ctx, hipMemcpy(b->d_recs.p, b->host.data(), b->nrec * sizeof(DevRecord), hipMemcpyHostToDevice));
+    size_t n_kind = kind_recs[0].size() + kind_recs[1].size();
+    IPH_HIP(ctx, b->d_kind_recs.alloc(n_kind));
+    IPH_HIP(ctx, b->d_tile_start.alloc(n_kind + 2));
+    size_t o_rec = 0, o_start = 0;
+    for (int k = 0; k < 2; ++k) {
+        TileMap& m = b->map[k];
+        m.n = (uint32_t)kind_recs[k].size();
+        m.total = starts[k].back();
+        m.recs_of_kind = b->d_kind_recs.p + o_rec;
+        m.tile_start = b->d_tile_start.p + o_start;
+        if (m.n) {
+            IPH_HIP(ctx, hipMemcpy(b->d_kind_recs.p + o_rec, kind_recs[k].data(), m.n * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice));
+            IPH_HIP(ctx, hipMemcpy(b->d_tile_start.p + o_start, starts[k].data(), (m.n + 1) * sizeof(uint64_t),
+                                   hipMemcpyHostToDevice));
+        }
+        o_rec += m.n;
+        o_start += m.n + 1;
+        const uint64_t want = blocks ? blocks : (uint64_t)ctx->cus * 8;
+        b->blocks[k] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, m.total ? m.total : 1));
+    }
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return IPH_OK;
+}
+
+static int enqueue(iph_batch* b, uint32_t flags, bool timed)
+{
+    iph_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    if (timed) IPH_HIP(ctx, hipEventRecord(b->ev[0], s));
+    if (flags & IPH_RUN_TRACE) {
+        IPH_HIP(ctx, launch_zero(b->d_recs.p, b->nrec, b->d_queue.p, s));
+        for (int k = 0; k < 2; ++k)
+            if (b->map[k].total)
+                IPH_HIP(ctx, launch_trace(k, b->wide[k], b->d_recs.p, b->map[k], b->d_queue.p + k, b->blocks[k], 256, s));
+    }
+    if (timed) IPH_HIP(ctx, hipEventRecord(b->ev[1], s));
+    if (flags & IPH_RUN_CDF) IPH_HIP(ctx, launch_cdf(b->d_jobs.p, b->nrec, b->max_onp, b->any_coarse, b->reduce_blocks, s));
+    if (timed) IPH_HIP(ctx, hipEventRecord(b->ev[2], s));
+    return IPH_OK;
+}
+
+static int check_cdf(iph_batch* b)
+{
+    iph_ctx* ctx = b->ctx;
+    for (uint32_t r = 0; r < b->nrec; ++r) {
+        double sc[kCdfScalars];
+        IPH_HIP(ctx, hipMemcpy(sc, b->jobs[r].scalars, sizeof sc, hipMemcpyDeviceToHost));
+        const bool coarse = b->jobs[r].coarse != nullptr;
+        const iph_cdf_report rep = make_report(sc, b->cdf[r], coarse);
+        if (rep.status)
+            return fail(ctx, IPH_E_AUDIT, "record " + std::to_string(r) + ": " + report_reason(rep.status));
+    }
+    return IPH_OK;
+}
+
+int iph_batch_run(iph_batch* b, uint32_t flags)
+{
+    if (!b) return fail(nullptr, IPH_E_INVALID, "batch must not be NULL.");
+    iph_ctx* ctx = b->ctx;
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t work = flags & (IPH_RUN_TRACE | IPH_RUN_CDF);
+    if (!work) return fail(ctx, IPH_E_INVALID, "Nothing to run (flags).");
+    if (flags & IPH_RUN_GRAPH) {
+        if (!b->graph || b->graph_flags != work) {
+            if (b->graph) {
+                (void)hipGraphExecDestroy(b->graph);
+                b->graph = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            // Event-record nodes inside the graph give per-stage device times of
+            // every replay; without them only the whole replay is timed.
+            b->graph_events = true;
+            IPH_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue(b, work, true);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc != IPH_OK || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                g = nullptr;
+                (void)hipGetLastError();
+                b->graph_events = false;
+                IPH_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                rc = enqueue(b, work, false);
+                e = hipStreamEndCapture(ctx->stream, &g);
+            }
+            if (rc != IPH_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(ctx, IPH_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            const hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess)
+                return fail(ctx, IPH_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            b->graph_flags = work;
+        }
+        if (b->graph_events) {
+            IPH_HIP(ctx, hipGraphLaunch(b->graph, ctx->stream));
+            IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            float t0 = 0, t1 = 0;
+            IPH_HIP(ctx, hipEventElapsedTime(&t0, b->ev[0], b->ev[1]));
+            IPH_HIP(ctx, hipEventElapsedTime(&t1, b->ev[1], b->ev[2]));
+            b->ms[0] = t0;
+            b->ms[1] = t1;
+            b->ms[2] = (double)t0 + t1;
+        } else {
+            IPH_HIP(ctx, hipEventRecord(b->ev[0], ctx->stream));
+            IPH_HIP(ctx, hipGraphLaunch(b->graph, ctx->stream));
+            IPH_HIP(ctx, hipEventRecord(b->ev[3], ctx->stream));
+            IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            float t = 0;
+            IPH_HIP(ctx, hipEventElapsedTime(&t, b->ev[0], b->ev[3]));
+            b->ms[0] = b->ms[1] = -1;
+            b->ms[2] = t;
+        }
+    } else {
+        const int rc = enqueue(b, work, true);
+        if (rc != IPH_OK) return rc;
+        IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        float t0 = 0, t1 = 0;
+        IPH_HIP(ctx, hipEventElapsedTime(&t0, b->ev[0], b->ev[1]));
+        IPH_HIP(ctx, hipEventElapsedTime(&t1, b->ev[1], b->ev[2]));
+        b->ms[0] = t0;
+        b->ms[1] = t1;
+        b->ms[2] = (double)t0 + t1;
+    }
+    if ((work & IPH_RUN_CDF) && !(flags & IPH_RUN_NO_AUDIT)) return check_cdf(b);
+    return IPH_OK;
+}
+
+int iph_batch_timing(const iph_batch* b, double* ms)
+{
+    if (!b || !ms) return fail(nullptr, IPH_E_INVALID, "Invalid arguments.");
+    std::memcpy(ms, b->ms, sizeof b->ms);
+    return IPH_OK;
+}
+
+int iph_batch_stats(iph_batch* b, uint32_t rec, iph_hist_stats* out)
+{
+    if (!b || !out || rec >= b->nrec) return fail(b ? b->ctx : nullptr, IPH_E_INVALID, "Invalid record index.");
+    iph_ctx* ctx = b->ctx;
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned long long st[kStatCount];
+    IPH_HIP(ctx, hipMemcpy(st, b->host[rec].stats, sizeof st, hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof *out);
+    out->q_forward = st[kStatForward];
+    out->q_backward = st[kStatBackward];
+    out->samples = st[kStatSamples];
+    out->invalid = st[kStatInvalid];
+    out->snapped = st[kStatSnapped];
+    for (int k = 0; k < 9; ++k) out->status[k] = st[kStatStatus0 + k];
+    out->q_escaped = st[kStatEscaped];
+    out->q_unresolved = st[kStatUnresolved];
+    out->internal_hits = st[kStatInternalHits];
+    out->rays = b->host[rec].rays;
+    out->qbits = b->host[rec].qbits;
+    out->ray_weight_mm2 = b->weight[rec];
+    out->projected_area_mm2 = b->area[rec] > 0 ? b->area[rec] : (double)(out->rays - out->status[8]) * b->weight[rec];
+    return IPH_OK;
+}
+
+int iph_batch_bins(iph_batch* b, uint32_t rec, uint64_t* out)
+{
+    if (!b || !out || rec >= b->nrec) return fail(b ? b->ctx : nullptr, IPH_E_INVALID, "Invalid record index.");
+    iph_ctx* ctx = b->ctx;
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)b->host[rec].nt * b->host[rec].np;
+    IPH_HIP(ctx, hipMemcpy(out, b->host[rec].bins, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return IPH_OK;
+}
+
+int iph_batch_cdf(iph_batch* b, uint32_t rec, double* phi, double* theta, iph_cdf_report* report)
+{
+    if (!b || rec >= b->nrec) return fail(b ? b->ctx : nullptr, IPH_E_INVALID, "Invalid record index.");
+    iph_ctx* ctx = b->ctx;
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    const CdfJob& J = b->jobs[rec];
+    if (phi) IPH_HIP(ctx, hipMemcpy(phi, J.phi, (J.onp + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (theta)
+        IPH_HIP(ctx, hipMemcpy(theta, J.theta, (size_t)J.onp * (J.ont + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (report) {
+        double sc[kCdfScalars];
+        IPH_HIP(ctx, hipMemcpy(sc, J.scalars, sizeof sc, hipMemcpyDeviceToHost));
+        *report = make_report(sc, b->cdf[rec], J.coarse != nullptr);
+    }
+    return IPH_OK;
+}
+
+int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
+                  double* phi, double* theta, iph_cdf_report* report)
+{
+    if (!ctx) return fail(nullptr, IPH_E_INVALID, "ctx must not be NULL.");
+    if (!mass || !phi || !theta || !nt || !np || (uint64_t)nt * np > 0xffffffffull)
+        return fail(ctx, IPH_E_INVALID, "Invalid CDF arguments.");
+    CdfSetup s;
+    std::string err;
+    if (!resolve_policy(policy, nt, np, s, err)) return fail(ctx, IPH_E_INVALID, err);
+    std::vector<double> e;
+    if (!make_u_edges(nt, e, err)) return fail(ctx, IPH_E_INVALID, err);
+    const uint32_t dt = nt / s.ont;
+    std::vector<double> ce(s.ont + 1);
+    for (uint32_t i = 0; i <= s.ont; ++i) ce[i] = e[(size_t)i * dt];
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    const bool coarse = s.ont != nt || s.onp != np;
+    const uint32_t red = 512;
+    const uint32_t pblocks = cdf_partial_blocks(red, s.onp);
+    const size_t n = (size_t)nt * np;
+    const size_t sz_theta = (size_t)s.onp * (s.ont + 1);
+    DevBuf<double> buf;
+    const size_t total = n + e.size() + ce.size() + (s.onp + 1) + sz_theta + s.onp + kCdfScalars + (size_t)pblocks * 8 +
+                         (coarse ? (size_t)s.ont * s.onp : 0);
+    IPH_HIP(ctx, buf.alloc(total));
+    double* p = buf.p;
+    CdfJob J;
+    std::memset(&J, 0, sizeof J);
+    J.nt = nt; J.np = np; J.ont = s.ont; J.onp = s.onp;
+    double* d_mass = p; p += n;
+    double* d_e = p; p += e.size();
+    double* d_ce = p; p += ce.size();
+    J.mass = d_mass; J.edges = d_e; J.cedges = d_ce;
+    J.phi = p; p += s.onp + 1;
+    J.theta = p; p += sz_theta;
+    J.cols = p; p += s.onp;
+    J.scalars = p; p += kCdfScalars;
+    J.partials = p; p += (size_t)pblocks * 8;
+    J.coarse = coarse ? p : nullptr;
+    IPH_HIP(ctx, hipMemcpyAsync(d_mass, mass, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    IPH_HIP(ctx, hipMemcpyAsync(d_e, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    IPH_HIP(ctx, hipMemcpyAsync(d_ce, ce.data(), ce.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    DevBuf<CdfJob> dj;
+    IPH_HIP(ctx, dj.alloc(1));
+    IPH_HIP(ctx, hipMemcpyAsync(dj.p, &J, sizeof J, hipMemcpyHostToDevice, ctx->stream));
+    IPH_HIP(ctx, launch_cdf(dj.p, 1, s.onp, coarse, red, ctx->stream));
+    double sc[kCdfScalars];
+    IPH_HIP(ctx, hipMemcpyAsync(phi, J.phi, (s.onp + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPH_HIP(ctx, hipMemcpyAsync(theta, J.theta, sz_theta * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPH_HIP(ctx, hipMemcpyAsync(sc, J.scalars, sizeof sc, hipMemcpyDeviceToHost, ctx->stream));
+    IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const iph_cdf_report rep = make_report(sc, s, coarse);
+    if (report) *report = rep;
+    if (rep.status) return fail(ctx, IPH_E_AUDIT, report_reason(rep.status));
+    return IPH_OK;
+}
+
+int iph_write_record(const char* out_dir, const char* metadata_json, const double* phi, const double* theta,
+                     uint32_t nt, uint32_t np)
+{
+    if (!out_dir || !metadata_json || !phi || !theta || !nt || !np)
+        return fail(nullptr, IPH_E_INVALID, "Invalid record arguments.");
+    std::vector<double> e;
+    std::string err;
+    if (!make_u_edges(nt, e, err)) return fail(nullptr, IPH_E_INVALID, err);
+    const int rc = write_record_dir(out_dir, metadata_json, phi, theta, e, nt, np, err);
+    if (rc != IPH_OK) return fail(nullptr, rc, err);
+    return IPH_OK;
+}
+
+}  // extern "C"

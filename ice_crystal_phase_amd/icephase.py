@@ -1,0 +1,308 @@
+"""ctypes binding of libicephase.so (C-ABI declared in include/icephase.h).
+
+This is the Python face of the same boundary the C++ command-line tools use.
+It raises when the library is missing or no HIP device is usable: there is no
+CPU fallback anywhere in the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import weakref
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libicephase.so"
+
+u32, u64, i32, f64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
+P = C.POINTER
+
+SHAPE_SPHERE, SHAPE_DROP, SHAPE_HEX = 0, 1, 2
+RUN_TRACE, RUN_CDF, RUN_GRAPH, RUN_NO_AUDIT = 1, 2, 4, 8
+
+STATUS_NAMES = ("exhausted", "tail_tolerance_reached", "interaction_limit", "ambiguous_boundary",
+                "invalid_input", "numerical_failure", "output_overflow", "replay_mismatch", "missed")
+
+
+class IcePhaseError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"icephase error {code}: {message}")
+        self.code = code
+
+
+class AuditError(IcePhaseError):
+    pass
+
+
+class Record(C.Structure):
+    _fields_ = [("shape", u32), ("max_internal_hits", u32), ("theta_count", u32), ("phi_count", u32),
+                ("grid_w", u32), ("grid_h", u32), ("rays", u64), ("seed", u64), ("radius_mm", f64),
+                ("length_mm", f64), ("shape_coefficients", f64 * 8), ("incident", f64 * 3),
+                ("interior_index", f64), ("exterior_index", f64), ("wavelength_nm", f64),
+                ("tail_tolerance", f64)]
+
+
+class HistStats(C.Structure):
+    _fields_ = [("q_forward", u64), ("q_backward", u64), ("samples", u64), ("invalid", u64), ("snapped", u64),
+                ("status", u64 * 9), ("q_escaped", u64), ("q_unresolved", u64), ("internal_hits", u64),
+                ("rays", u64), ("qbits", u32), ("pad", u32), ("ray_weight_mm2", f64),
+                ("projected_area_mm2", f64)]
+
+
+class CdfPolicy(C.Structure):
+    _fields_ = [("out_theta", u32), ("out_phi", u32), ("max_l1", f64), ("max_lost_mass", f64),
+                ("max_coarsening_tv", f64)]
+
+
+class CdfReport(C.Structure):
+    _fields_ = [("g_source", f64), ("g_stored", f64), ("coarsening_tv", f64), ("cdf_mass", f64),
+                ("l1_error", f64), ("lost_mass", f64), ("max_cell_error", f64), ("aggregation_error", f64),
+                ("fine_mass", f64), ("stored_mass", f64), ("lost_cells", u64), ("malformed", u64),
+                ("weight_underflow", u64), ("negative", u64), ("status", i32), ("pad", i32)]
+
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Load libicephase.so; raise (never fall back) when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise IcePhaseError(-2, f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                                "(make -C ice_crystal_phase_amd)")
+    L = C.CDLL(str(LIB_PATH))
+    L.iph_abi_version.restype = C.c_int
+    L.iph_struct_size.argtypes = [C.c_int]
+    L.iph_struct_size.restype = u32
+    L.iph_last_error.argtypes = [C.c_void_p]
+    L.iph_last_error.restype = C.c_char_p
+    L.iph_ctx_create.argtypes = [C.c_int, P(C.c_void_p)]
+    L.iph_ctx_destroy.argtypes = [C.c_void_p]
+    L.iph_ctx_destroy.restype = None
+    L.iph_u_edges.argtypes = [u32, P(f64)]
+    L.iph_bin_directions.argtypes = [C.c_void_p, u32, u32, P(f64), u64, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.iph_batch_create.argtypes = [C.c_void_p, P(Record), u32, P(CdfPolicy), P(C.c_void_p)]
+    L.iph_batch_destroy.argtypes = [C.c_void_p]
+    L.iph_batch_destroy.restype = None
+    L.iph_batch_run.argtypes = [C.c_void_p, u32]
+    L.iph_batch_set_schedule.argtypes = [C.c_void_p, u64, u32]
+    L.iph_batch_timing.argtypes = [C.c_void_p, P(f64)]
+    L.iph_batch_stats.argtypes = [C.c_void_p, u32, P(HistStats)]
+    L.iph_batch_bins.argtypes = [C.c_void_p, u32, C.c_void_p]
+    L.iph_batch_cdf.argtypes = [C.c_void_p, u32, C.c_void_p, C.c_void_p, P(CdfReport)]
+    L.iph_build_cdf.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(CdfPolicy), C.c_void_p, C.c_void_p,
+                                P(CdfReport)]
+    L.iph_write_record.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, u32, u32]
+    for name, st in (("Record", Record), ("HistStats", HistStats), ("CdfReport", CdfReport),
+                     ("CdfPolicy", CdfPolicy)):
+        which = {"Record": 0, "HistStats": 1, "CdfReport": 2, "CdfPolicy": 3}[name]
+        if L.iph_struct_size(which) != C.sizeof(st):
+            raise IcePhaseError(-1, f"ABI mismatch for {name}: {L.iph_struct_size(which)} != {C.sizeof(st)}")
+    if L.iph_abi_version() != 1:
+        raise IcePhaseError(-1, "Unsupported icephase ABI version")
+    _lib = L
+    return L
+
+
+def _check(rc: int, ctx_ptr=None) -> None:
+    if rc != 0:
+        msg = lib().iph_last_error(ctx_ptr).decode(errors="replace")
+        if rc == -3:
+            raise AuditError(rc, msg)
+        raise IcePhaseError(rc, msg)
+
+
+EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_destroy", "iph_last_error",
+            "iph_u_edges", "iph_bin_directions", "iph_batch_create", "iph_batch_destroy", "iph_batch_run",
+            "iph_batch_set_schedule", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
+            "iph_build_cdf", "iph_write_record")
+
+
+def u_edges(nt: int) -> np.ndarray:
+    out = np.empty(nt + 1, np.float64)
+    _check(lib().iph_u_edges(nt, out.ctypes.data_as(P(f64))))
+    return out
+
+
+# ------------------------------------------------------------------ records
+def drop_incident(inclination_deg: float) -> tuple[float, float, float]:
+    """ki = (cos a, -sin a, 0): apps/generate_phase.cpp:217 convention."""
+    a = inclination_deg * math.pi / 180.0
+    return (math.cos(a), -math.sin(a), 0.0)
+
+
+def sphere_record(n_int: float, grid: int, theta: int = 900, phi: int = 1800, radius_mm: float = 1.0,
+                  inclination_deg: float = 0.0, max_internal_hits: int = 9, wavelength_nm: float = 550.0,
+                  tail: float = 0.0, grid_h: int | None = None) -> Record:
+    r = Record()
+    r.shape, r.max_internal_hits, r.theta_count, r.phi_count = SHAPE_SPHERE, max_internal_hits, theta, phi
+    r.grid_w, r.grid_h = grid, grid if grid_h is None else grid_h
+    r.radius_mm = radius_mm
+    r.incident = (f64 * 3)(*drop_incident(inclination_deg))
+    r.interior_index, r.exterior_index, r.wavelength_nm, r.tail_tolerance = n_int, 1.0, wavelength_nm, tail
+    return r
+
+
+def drop_record(coeffs: Sequence[float], n_int: float, grid: int, theta: int = 900, phi: int = 1800,
+                radius_mm: float = 2.0, inclination_deg: float = 0.0, max_internal_hits: int = 9,
+                wavelength_nm: float = 550.0, tail: float = 0.0) -> Record:
+    r = sphere_record(n_int, grid, theta, phi, radius_mm, inclination_deg, max_internal_hits, wavelength_nm, tail)
+    r.shape = SHAPE_DROP
+    c = list(coeffs) + [0.0] * (8 - len(coeffs))
+    r.shape_coefficients = (f64 * 8)(*c[:8])
+    return r
+
+
+def hex_record(radius_mm: float, length_mm: float, n_int: float, rays: int, incident=(1.0, -0.4, 0.3),
+               seed: int = 12345, theta: int = 900, phi: int = 1800, max_internal_hits: int = 12,
+               tail: float = 1e-12, wavelength_nm: float = 550.0, n_ext: float = 1.0) -> Record:
+    r = Record()
+    r.shape, r.max_internal_hits, r.theta_count, r.phi_count = SHAPE_HEX, max_internal_hits, theta, phi
+    r.rays, r.seed = rays, seed
+    r.radius_mm, r.length_mm = radius_mm, length_mm
+    r.incident = (f64 * 3)(*incident)
+    r.interior_index, r.exterior_index, r.wavelength_nm, r.tail_tolerance = n_int, n_ext, wavelength_nm, tail
+    return r
+
+
+def record_rays(r: Record) -> int:
+    return int(r.rays) if r.shape == SHAPE_HEX else int(r.grid_w) * int(r.grid_h)
+
+
+# ------------------------------------------------------------------ context / batch
+class Context:
+    """One HIP device, one stream (iph_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._p = C.c_void_p()
+        self._batches = weakref.WeakSet()
+        _check(lib().iph_ctx_create(device, C.byref(self._p)))
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def close(self) -> None:
+        """Destroy the context; live batches are closed first."""
+        if self._p:
+            for b in list(self._batches):
+                b.close()
+            lib().iph_ctx_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bin_directions(self, nt: int, np_: int, frame: np.ndarray, dirs: np.ndarray):
+        frame = np.ascontiguousarray(frame, np.float64).reshape(9)
+        dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+        n = dirs.shape[0]
+        index = np.zeros(n, np.uint32)
+        kind = np.zeros(n, np.int32)
+        _check(lib().iph_bin_directions(self._p, nt, np_, frame.ctypes.data_as(P(f64)), n, dirs.ctypes.data,
+                                        index.ctypes.data, kind.ctypes.data), self._p)
+        return index, kind
+
+    def build_cdf(self, mass: np.ndarray, nt: int, np_: int, out_theta: int = 0, out_phi: int = 0,
+                  max_l1: float = 1e-10, max_lost: float = 1e-12, max_tv: float = 1.0, check: bool = True):
+        mass = np.ascontiguousarray(mass, np.float64).reshape(-1)
+        pol = CdfPolicy(out_theta, out_phi, max_l1, max_lost, max_tv)
+        ont = out_theta or nt
+        onp = out_phi or np_
+        phi = np.zeros(onp + 1, np.float64)
+        theta = np.zeros(onp * (ont + 1), np.float64)
+        rep = CdfReport()
+        rc = lib().iph_build_cdf(self._p, mass.ctypes.data, nt, np_, C.byref(pol), phi.ctypes.data,
+                                 theta.ctypes.data, C.byref(rep))
+        if check:
+            _check(rc, self._p)
+        return phi, theta.reshape(onp, ont + 1), rep, rc
+
+
+@dataclass(eq=False)
+class Batch:
+    """Device state of several records (iph_batch)."""
+
+    ctx: Context
+    records: list
+    policy: CdfPolicy | None = None
+    _p: C.c_void_p = field(default_factory=C.c_void_p)
+
+    def __post_init__(self):
+        arr = (Record * len(self.records))(*self.records)
+        self._arr = arr
+        pol = C.byref(self.policy) if self.policy is not None else None
+        _check(lib().iph_batch_create(self.ctx.ptr, arr, len(self.records), pol, C.byref(self._p)), self.ctx.ptr)
+        self.ctx._batches.add(self)
+
+    def close(self):
+        if self._p and self.ctx.ptr:
+            lib().iph_batch_destroy(self._p)
+        self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, flags: int = RUN_TRACE | RUN_CDF) -> None:
+        _check(lib().iph_batch_run(self._p, flags), self.ctx.ptr)
+
+    def set_schedule(self, tile_rays: int = 0, blocks: int = 0) -> None:
+        _check(lib().iph_batch_set_schedule(self._p, tile_rays, blocks), self.ctx.ptr)
+
+    def timing(self) -> tuple[float, float, float]:
+        ms = (f64 * 3)()
+        _check(lib().iph_batch_timing(self._p, ms))
+        return ms[0], ms[1], ms[2]
+
+    def stats(self, rec: int = 0) -> HistStats:
+        st = HistStats()
+        _check(lib().iph_batch_stats(self._p, rec, C.byref(st)), self.ctx.ptr)
+        return st
+
+    def bins(self, rec: int = 0) -> np.ndarray:
+        r = self.records[rec]
+        out = np.zeros(int(r.theta_count) * int(r.phi_count), np.uint64)
+        _check(lib().iph_batch_bins(self._p, rec, out.ctypes.data), self.ctx.ptr)
+        return out
+
+    def cdf(self, rec: int = 0):
+        r = self.records[rec]
+        ont = (self.policy.out_theta if self.policy is not None and self.policy.out_theta else r.theta_count)
+        onp = (self.policy.out_phi if self.policy is not None and self.policy.out_phi else r.phi_count)
+        phi = np.zeros(onp + 1, np.float64)
+        theta = np.zeros(onp * (ont + 1), np.float64)
+        rep = CdfReport()
+        _check(lib().iph_batch_cdf(self._p, rec, phi.ctypes.data, theta.ctypes.data, C.byref(rep)), self.ctx.ptr)
+        return phi, theta.reshape(onp, ont + 1), rep
+
+
+def write_record(out_dir: str | Path, metadata_json: str, phi: np.ndarray, theta: np.ndarray) -> None:
+    theta = np.ascontiguousarray(theta, np.float64)
+    phi = np.ascontiguousarray(phi, np.float64)
+    np_, nt1 = theta.shape
+    _check(lib().iph_write_record(str(out_dir).encode(), metadata_json.encode(), phi.ctypes.data,
+                                  theta.ctypes.data, nt1 - 1, np_))

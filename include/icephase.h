@@ -1,0 +1,190 @@
+/*
+ * icephase — C-ABI of the MI355X phase-function generator (libicephase.so).
+ *
+ * This library replaces the reference's device-side path for one record or
+ * a batch of records: trace, histogram, FP64 CDF, record files. Every entry point
+ * below names the reference interface it stands in for (paths relative to the
+ * reference checkout).
+ *
+ * Conventions (the reference's, include/rainbow/cuda_driver.hpp and
+ * src/cuda_driver.cpp:148-213):
+ *   - one HIP device and one stream per context; the context is not
+ *     thread-safe; every call is synchronous (returns after the device work it
+ *     started has finished);
+ *   - no exceptions or C++ types cross the ABI: functions return IPH_OK (0) or
+ *     a negative IPH_E_* code, and iph_last_error() gives the message
+ *     (the reference throws std::runtime_error with the same texts);
+ *   - the caller owns every host array it passes; the library owns the device
+ *     memory of a batch until iph_batch_destroy.
+ * The library fails loudly when no HIP device is usable: there is no CPU
+ * fallback.
+ */
+#ifndef ICEPHASE_H
+#define ICEPHASE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPH_ABI_VERSION 1
+
+enum iph_status {
+    IPH_OK = 0,
+    IPH_E_INVALID = -1,   /* invalid argument / unsupported configuration */
+    IPH_E_DEVICE = -2,    /* HIP runtime error */
+    IPH_E_AUDIT = -3,     /* a trace or CDF quality audit rejected the record */
+    IPH_E_IO = -4,        /* record files could not be written */
+    IPH_E_EXISTS = -5,    /* output directory or its .part staging exists */
+    IPH_E_NOMEM = -6
+};
+
+enum iph_shape {
+    IPH_SHAPE_SPHERE = 0, /* unit sphere (equal-volume radius) */
+    IPH_SHAPE_DROP = 1,   /* axisymmetric Chebyshev drop r(mu)=1+c0+c1 mu+sum c_n T_n(mu), mu=-y/r */
+    IPH_SHAPE_HEX = 2     /* regular hexagonal prism, axis +y */
+};
+
+/* One phase-function record. Units: mm, nm, degrees.
+ * Drop records (sphere/drop): rays = grid_w*grid_h cell-centre rays over the
+ * square [-h,h]^2 of the plane normal to `incident`, h = 1.01*outer radius
+ * (include/rainbow/raindrop_settings.hpp:41-42 uses the same 1.01 factor), FP32
+ * tracing. Hex records: `rays` samples drawn by the reference's counter RNG
+ * (hex_trace/include/ice_crystal/math.hpp:136-146) with `seed`, FP64 tracing. */
+typedef struct iph_record {
+    uint32_t shape;                /* enum iph_shape */
+    uint32_t max_internal_hits;    /* internal encounters traced (each emits T, continues R) */
+    uint32_t theta_count, phi_count;  /* histogram grid (cells uniform in theta) */
+    uint32_t grid_w, grid_h;       /* drop incident grid; ignored for hex */
+    uint64_t rays;                 /* hex samples; drops: 0 or grid_w*grid_h */
+    uint64_t seed;                 /* hex RNG seed */
+    double radius_mm;              /* drop equal-volume radius / hex circumradius */
+    double length_mm;              /* hex full length (c = length/2, aspect c/a = length/(2 radius)) */
+    double shape_coefficients[8];  /* drop c0..c7 (must be convex); ignored for sphere/hex */
+    double incident[3];            /* incident propagation direction (normalized by the library) */
+    double interior_index, exterior_index;
+    double wavelength_nm;          /* recorded only; indices are given explicitly */
+    double tail_tolerance;         /* stop a ray when its remaining power <= this */
+} iph_record;
+
+/* Per-record trace/histogram audit. Power quanta are rint(p * 2^qbits) of a
+ * unit-power incident ray; the mass a quantum represents is
+ * ray_weight_mm2 * 2^-qbits. */
+typedef struct iph_hist_stats {
+    uint64_t q_forward, q_backward;  /* pole quanta (spread over the end rows) */
+    uint64_t samples, invalid, snapped;
+    uint64_t status[9];              /* trace statuses 0..7 (hex_trace_core.hpp:8-13), [8] = missed */
+    uint64_t q_escaped, q_unresolved;
+    uint64_t internal_hits;
+    uint64_t rays;
+    uint32_t qbits, pad;
+    double ray_weight_mm2;           /* cross-section carried by one ray */
+    double projected_area_mm2;       /* hex: exact face sum; drops: rays_hit * ray_weight */
+} iph_hist_stats;
+
+typedef struct iph_cdf_policy {
+    uint32_t out_theta, out_phi;     /* stored grid; 0 = histogram grid; must divide it */
+    double max_l1;                   /* default 1e-10 (phase_table.hpp CdfPolicy) */
+    double max_lost_mass;            /* default 1e-12 */
+    double max_coarsening_tv;        /* default 1 */
+} iph_cdf_policy;
+
+typedef struct iph_cdf_report {
+    double g_source, g_stored, coarsening_tv, cdf_mass;
+    double l1_error, lost_mass, max_cell_error, aggregation_error;
+    double fine_mass, stored_mass;
+    uint64_t lost_cells, malformed, weight_underflow, negative;
+    int32_t status;                  /* 0 accepted; >0 which audit failed */
+    int32_t pad;
+} iph_cdf_report;
+
+typedef struct iph_ctx iph_ctx;
+typedef struct iph_batch iph_batch;
+
+/* Replaces the ABI sentinel kernels and sizeof fingerprints
+ * (device/phase_cdf.cu:65,240; hex_trace/device/hex_trace.cu:6-11;
+ * hex_trace/src/hex_trace_cuda.cpp:39-44). Returns IPH_ABI_VERSION. */
+int iph_abi_version(void);
+/* sizeof(iph_record), sizeof(iph_hist_stats), sizeof(iph_cdf_report) for
+ * binding checks. */
+uint32_t iph_struct_size(int which);
+
+/* Replaces rainbow::CudaContext(device) (src/cuda_driver.cpp:148-213).
+ * iph_ctx_destroy also destroys every batch of the context that is still
+ * alive; do not use those batch handles afterwards. */
+int iph_ctx_create(int device, iph_ctx** out);
+void iph_ctx_destroy(iph_ctx* ctx);
+/* Replaces the exception texts of RAINBOW_CUDA_CHECK (include/rainbow/cuda_error.hpp:11-30).
+ * ctx may be NULL for the context-free functions (thread-local message). */
+const char* iph_last_error(const iph_ctx* ctx);
+
+/* Replaces rainbow::make_phase_u_edges (include/rainbow/phase_cdf_math.hpp:16-32). */
+int iph_u_edges(uint32_t theta_count, double* out /* theta_count+1 */);
+
+/* Bins n FP64 directions on the device with the histogram's own binning
+ * (hex_phase/include/ice_crystal/phase_math.hpp:37-81, called per sample by
+ * hex_phase/device/phase_accumulate.cu:10-28). frame: ki, e0, e1 (9 doubles).
+ * kind[k]: 0 cell (index[k] = i*phi_count+j), +1 forward pole, -1 backward pole,
+ * 2 invalid. */
+int iph_bin_directions(iph_ctx* ctx, uint32_t theta_count, uint32_t phi_count, const double* frame,
+                       uint64_t n, const double* dirs, uint32_t* index, int32_t* kind);
+
+/* Build the device state for nrec records: validates every record (convexity
+ * of drops, prism validity, index ranges), allocates the histogram slabs and
+ * CDF buffers. Replaces the per-record construction of
+ * RaindropTracer (include/rainbow/raindrop_tracer.hpp:27-39),
+ * HexPrismTracer (hex_trace/include/ice_crystal/hex_trace_cuda.hpp:20-28) and
+ * PhaseMassAccumulatorCuda (hex_phase/include/ice_crystal/phase_cuda.hpp:17).
+ * policy may be NULL (defaults). */
+int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const iph_cdf_policy* policy,
+                     iph_batch** out);
+void iph_batch_destroy(iph_batch* b);
+
+enum iph_run_flags {
+    IPH_RUN_TRACE = 1,   /* zero + trace + histogram (one persistent launch per record kind) */
+    IPH_RUN_CDF = 2,     /* FP64 CDFs of every record from the current histograms */
+    IPH_RUN_GRAPH = 4,   /* replay the steps as one captured hipGraph */
+    IPH_RUN_NO_AUDIT = 8 /* skip the host-side policy check (timing loops) */
+};
+/* Runs the batch on the device: replaces HexPrismTracer::trace + download_audits
+ * (hex_trace/src/hex_trace_cuda.cpp:47-69), PhaseMassAccumulatorCuda::add/finish
+ * (hex_phase/src/phase_cuda.cpp:37-66), RaindropTracer::trace_unpolarized
+ * (src/raindrop_tracer.cpp:144-177) and PhaseCdf::build (src/phase_cdf.cpp:275-472).
+ * Returns IPH_E_AUDIT if a CDF audit rejects a record (see iph_batch_cdf). */
+int iph_batch_run(iph_batch* b, uint32_t flags);
+/* Tile size in rays and workgroups of the trace launch (0 = library choice).
+ * The autotuner in the package sets these. */
+int iph_batch_set_schedule(iph_batch* b, uint64_t tile_rays, uint32_t blocks);
+/* Device time of the last run, from HIP events on the batch stream:
+ * ms[0] zero+trace launches, ms[1] CDF kernels, ms[2] whole run. */
+int iph_batch_timing(const iph_batch* b, double* ms /* 3 */);
+
+/* Replaces summarize()/TraceSummary (hex_trace/src/hex_trace.cpp:72-88) and
+ * PhaseMassAccumulator::statistics (hex_phase/src/phase_table.cpp:77-78). */
+int iph_batch_stats(iph_batch* b, uint32_t rec, iph_hist_stats* out);
+/* Copies record rec's integer histogram (theta-major, theta_count*phi_count). */
+int iph_batch_bins(iph_batch* b, uint32_t rec, uint64_t* out);
+/* Copies record rec's CDF arrays and audit: replaces PhaseCdf::write_arrays'
+ * sources and audit_statistics/storage_statistics (src/phase_cdf.cpp:482-493). */
+int iph_batch_cdf(iph_batch* b, uint32_t rec, double* phi /* onp+1 */, double* theta /* onp*(ont+1) */,
+                  iph_cdf_report* report);
+
+/* FP64 CDF of caller masses (theta-major, nt*np, nonnegative): replaces
+ * PhaseCdf::build with a Scalar density view (include/rainbow/phase_cdf.hpp:29-31,
+ * used by hex_phase/apps/generate_hex_phase.cpp:125-134) and its host twin
+ * build_host_cdf (hex_phase/src/phase_table.cpp:111-187). */
+int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
+                  double* phi, double* theta, iph_cdf_report* report);
+
+/* Writes <out_dir>.part/{phi_cdf,theta_given_phi_cdf,u_edges}.npy and
+ * metadata.json, then renames to out_dir: replaces DatasetDirectory +
+ * NpyFloat64Writer (src/npy_writer.cpp:18-121) and PhaseCdf::write_arrays
+ * (src/phase_cdf.cpp:482-493). Refuses an existing out_dir or .part. */
+int iph_write_record(const char* out_dir, const char* metadata_json, const double* phi, const double* theta,
+                     uint32_t nt, uint32_t np);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,146 @@
+// Test infrastructure only. A thin C API over the reference's own CPU sources,
+// compiled from where they lie in the reference checkout (see oracle/Makefile)
+// into oracle/_ref/libref.so. It lets the tests pin the oracle restatement
+// against the compiled reference and write golden vectors. Nothing in the
+// product links or loads it.
+#include <ice_crystal/hex_trace.hpp>
+#include <ice_crystal/phase_table.hpp>
+#include <rainbow/phase_cdf_math.hpp>
+#include <rainbow/water_refractive_index.hpp>
+#include <rainbow/projected_area.hpp>
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <vector>
+
+using namespace iceCrystal;
+
+extern "C" {
+
+struct ref_sample { double dir[3]; double power; double length_mm; uint64_t id;
+                    uint32_t entry_face, exit_face, reflections, kind; };
+struct ref_audit { double escaped, unresolved, balance, max_interface; uint64_t outputs;
+                   uint32_t internal_hits, tir_count, status, pad; };
+
+int ref_u_edges(uint32_t n, double* out)
+{
+    try { auto e = rainbow::make_phase_u_edges(n); std::memcpy(out, e.data(), e.size() * sizeof(double)); return 0; }
+    catch (const std::exception&) { return -1; }
+}
+
+int ref_make_frame(const double k[3], double ki[3], double e0[3], double e1[3])
+{
+    try {
+        const auto g = make_phase_grid(1, 1, {k[0], k[1], k[2]});
+        ki[0] = g.ki.x; ki[1] = g.ki.y; ki[2] = g.ki.z;
+        e0[0] = g.frame.e0.x; e0[1] = g.frame.e0.y; e0[2] = g.frame.e0.z;
+        e1[0] = g.frame.e1.x; e1[1] = g.frame.e1.y; e1[2] = g.frame.e1.z;
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+// Bins n directions on the grid of make_phase_grid(nt, np, ki).
+int ref_phase_bin(uint32_t nt, uint32_t np, const double k[3], uint64_t n, const double* dirs,
+                  uint64_t* index, int32_t* pole, int32_t* valid, int32_t* snapped)
+{
+    try {
+        const auto g = make_phase_grid(nt, np, {k[0], k[1], k[2]});
+        for (uint64_t i = 0; i < n; ++i) {
+            const auto b = phase_bin(g.view(), {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]});
+            index[i] = b.index; pole[i] = b.pole; valid[i] = b.valid; snapped[i] = b.boundary_snapped;
+        }
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+// trace_batch_cpu over [first, first+count). Returns the sample count or <0.
+int64_t ref_hex_trace(double R, double L, const double k[3], double ni, double nt, uint64_t seed,
+                      uint64_t total, uint32_t max_hits, double tail, uint64_t first, uint64_t count,
+                      ref_sample* out, uint64_t cap, ref_audit* audits)
+{
+    try {
+        const auto s = make_trace_settings({R, L}, {k[0], k[1], k[2]}, ni, nt, total, seed, max_hits, tail);
+        const auto b = trace_batch_cpu(s, first, count);
+        if (b.outgoing.size() > cap) return -1;
+        for (std::size_t i = 0; i < b.outgoing.size(); ++i) {
+            const auto& o = b.outgoing[i];
+            ref_sample& r = out[i];
+            r.dir[0] = o.direction.x; r.dir[1] = o.direction.y; r.dir[2] = o.direction.z;
+            r.power = o.power_fraction; r.length_mm = o.internal_length_mm; r.id = o.incident_sample_id;
+            r.entry_face = o.entry_face; r.exit_face = o.exit_face; r.reflections = o.internal_reflections;
+            r.kind = static_cast<uint32_t>(o.kind);
+        }
+        for (std::size_t i = 0; i < b.audits.size() && audits; ++i) {
+            const auto& a = b.audits[i];
+            audits[i] = {a.escaped_power, a.unresolved_power, a.balance_error, a.max_interface_balance_error,
+                         a.output_count, a.internal_hits, a.tir_count, static_cast<uint32_t>(a.status), 0};
+        }
+        return static_cast<int64_t>(b.outgoing.size());
+    } catch (const std::exception&) { return -2; }
+}
+
+// Single-orientation PhaseMassAccumulator over all rays of the record.
+int ref_hex_masses(double R, double L, const double k[3], double ni, double nt, uint64_t seed,
+                   uint64_t total, uint32_t max_hits, double tail, uint32_t gnt, uint32_t gnp,
+                   double* mass, double* forward, double* backward)
+{
+    try {
+        const auto s = make_trace_settings({R, L}, {k[0], k[1], k[2]}, ni, nt, total, seed, max_hits, tail);
+        PhaseMassAccumulator h(make_phase_grid(gnt, gnp, s.incident_direction));
+        const double scale = s.prism.projected_area_mm2(s.incident_direction) / double(total);
+        for (uint64_t first = 0; first < total;) {
+            const uint64_t count = std::min<uint64_t>(4096, total - first);
+            const auto b = trace_batch_cpu(s, first, count);
+            h.add(b.outgoing, Rotation{}, scale);
+            first += count;
+        }
+        const auto m = h.masses();
+        std::memcpy(mass, m.data(), m.size() * sizeof(double));
+        const auto st = h.statistics();
+        *forward = st.forward_mass; *backward = st.backward_mass;
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+struct ref_cdf_report { double g_source, g_stored, coarsening_tv, cdf_mass, l1_error, lost_mass,
+                        max_cell_error, aggregation_error; uint64_t lost_cells; };
+
+int ref_build_host_cdf(const double* mass, uint32_t nt, uint32_t np, const double k[3], uint32_t ont,
+                       uint32_t onp, double max_l1, double max_lost, double max_tv,
+                       double* phi, double* theta, ref_cdf_report* rep)
+{
+    try {
+        const auto g = make_phase_grid(nt, np, {k[0], k[1], k[2]});
+        CdfPolicy p; p.max_l1_error = max_l1; p.max_lost_mass = max_lost; p.max_coarsening_tv = max_tv;
+        const auto c = build_host_cdf(g, {mass, std::size_t(nt) * np}, ont, onp, p);
+        std::memcpy(phi, c.phi.data(), c.phi.size() * sizeof(double));
+        std::memcpy(theta, c.theta.data(), c.theta.size() * sizeof(double));
+        const auto& r = c.report;
+        *rep = {r.g_source, r.g_stored, r.coarsening_tv, r.cdf_mass, r.l1_error, r.lost_mass,
+                r.maximum_cell_error, r.aggregation_relative_error, r.lost_cells};
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+int ref_water_index(double lambda_nm, double celsius, double pressure, double* n, double* rho)
+{
+    try {
+        const auto s = rainbow::evaluate_water_optics(lambda_nm, celsius, pressure);
+        *n = s.refractive_index; *rho = s.density_kg_m3;
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+int ref_projected_area(const double c[8], double radius_mm, const double k[3], double* area, int* sphere)
+{
+    try {
+        std::array<double, 8> coeffs{};
+        for (int i = 0; i < 8; ++i) coeffs[i] = c[i];
+        const auto r = rainbow::compute_projected_area(coeffs, radius_mm, {k[0], k[1], k[2]});
+        *area = r.area_mm2; *sphere = r.analytic_sphere;
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+} // extern "C"

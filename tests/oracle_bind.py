@@ -1,0 +1,332 @@
+"""ctypes bindings to the CPU oracle (oracle/liboracle.so) and, when it was
+built, the compiled-reference shim (oracle/_ref/libref.so).
+
+Test infrastructure only: the product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libref.so"
+
+u32, u64, i32, i64, f64, f32 = C.c_uint32, C.c_uint64, C.c_int32, C.c_int64, C.c_double, C.c_float
+P = C.POINTER
+
+
+class Frame(C.Structure):
+    _fields_ = [("ki", f64 * 3), ("e0", f64 * 3), ("e1", f64 * 3)]
+
+
+class Bin(C.Structure):
+    _fields_ = [("index", u64), ("mu", f64), ("pole", i32), ("valid", i32), ("snapped", i32), ("pad", i32)]
+
+
+class HexSettings(C.Structure):
+    _fields_ = [("radius_mm", f64), ("length_mm", f64), ("ki", f64 * 3), ("n_ext", f64), ("n_int", f64),
+                ("tail", f64), ("seed", u64), ("total", u64), ("max_hits", u32), ("pad", u32)]
+
+
+class RayAudit(C.Structure):
+    _fields_ = [("escaped", f64), ("unresolved", f64), ("balance", f64), ("max_interface", f64),
+                ("outputs", u64), ("internal_hits", u32), ("tir_count", u32), ("status", u32), ("pad", u32)]
+
+
+class Sample(C.Structure):
+    _fields_ = [("dir", f64 * 3), ("power", f64), ("length_mm", f64), ("id", u64), ("entry_face", u32),
+                ("exit_face", u32), ("reflections", u32), ("kind", u32)]
+
+
+class HistStats(C.Structure):
+    _fields_ = [("q_forward", u64), ("q_backward", u64), ("samples", u64), ("invalid", u64), ("snapped", u64),
+                ("status", u64 * 8), ("q_escaped", u64), ("q_unresolved", u64), ("rays_hit", u64),
+                ("internal_hits", u64)]
+
+
+class CdfReport(C.Structure):
+    _fields_ = [("g_source", f64), ("g_stored", f64), ("coarsening_tv", f64), ("cdf_mass", f64),
+                ("l1_error", f64), ("lost_mass", f64), ("max_cell_error", f64), ("aggregation_error", f64),
+                ("lost_cells", u64), ("status", i32), ("pad", i32)]
+
+
+class RefCdfReport(C.Structure):
+    _fields_ = [("g_source", f64), ("g_stored", f64), ("coarsening_tv", f64), ("cdf_mass", f64),
+                ("l1_error", f64), ("lost_mass", f64), ("max_cell_error", f64), ("aggregation_error", f64),
+                ("lost_cells", u64)]
+
+
+class DropRecord(C.Structure):
+    _fields_ = [("coeffs", f32 * 8), ("inner", f32), ("outer", f32), ("ki", f32 * 3), ("e0", f32 * 3),
+                ("e1", f32 * 3), ("n_int", f32), ("n_ext", f32), ("half_extent", f32), ("grid_w", u32),
+                ("grid_h", u32), ("max_hits", u32), ("sphere", u32), ("tail", f32)]
+
+
+SAMPLE_DTYPE = np.dtype([("dir", "<f8", 3), ("power", "<f8"), ("length_mm", "<f8"), ("id", "<u8"),
+                         ("entry_face", "<u4"), ("exit_face", "<u4"), ("reflections", "<u4"), ("kind", "<u4")])
+AUDIT_DTYPE = np.dtype([("escaped", "<f8"), ("unresolved", "<f8"), ("balance", "<f8"), ("max_interface", "<f8"),
+                        ("outputs", "<u8"), ("internal_hits", "<u4"), ("tir_count", "<u4"), ("status", "<u4"),
+                        ("pad", "<u4")])
+assert SAMPLE_DTYPE.itemsize == C.sizeof(Sample)
+assert AUDIT_DTYPE.itemsize == C.sizeof(RayAudit)
+
+
+def _ptr(a: np.ndarray, ctype=C.c_void_p):
+    return a.ctypes.data_as(ctype)
+
+
+def build_oracle() -> None:
+    """Build the oracle (and _ref when the reference checkout exists)."""
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            build_oracle()
+        L = C.CDLL(str(ORACLE_SO))
+        L.orc_u_edges.argtypes = [u32, P(f64)]
+        L.orc_make_frame.argtypes = [P(f64), P(Frame)]
+        L.orc_drop_frame.argtypes = [P(f64), P(Frame)]
+        L.orc_phase_bin.argtypes = [P(f64), u32, u32, P(Frame), P(f64)]
+        L.orc_phase_bin.restype = Bin
+        L.orc_hex_trace.argtypes = [P(HexSettings), u64, u64, C.c_void_p, u64, C.c_void_p]
+        L.orc_hex_trace.restype = i64
+        L.orc_hex_projected_area.argtypes = [f64, f64, P(f64)]
+        L.orc_hex_projected_area.restype = f64
+        L.orc_quantum_bits.argtypes = [u64]
+        L.orc_quantum_bits.restype = u32
+        L.orc_hex_hist.argtypes = [P(HexSettings), u32, u32, u32, u64, u64, C.c_void_p, P(HistStats)]
+        L.orc_hex_masses.argtypes = [P(HexSettings), u32, u32, C.c_void_p, P(f64), P(f64)]
+        L.orc_bins_to_mass.argtypes = [C.c_void_p, u64, u64, u32, u32, C.c_void_p]
+        L.orc_bins_to_mass.restype = None
+        L.orc_build_cdf.argtypes = [C.c_void_p, u32, u32, u32, u32, f64, f64, f64, C.c_void_p, C.c_void_p,
+                                    P(CdfReport)]
+        L.orc_drop_make.argtypes = [P(f64), C.c_int, f64, f64, u32, u32, u32, f64, P(DropRecord)]
+        L.orc_drop_trace.argtypes = [P(DropRecord), C.c_int, u64, u64, C.c_void_p, u64, C.c_void_p]
+        L.orc_drop_trace.restype = i64
+        L.orc_drop_hist.argtypes = [P(DropRecord), C.c_int, u32, u32, u32, u64, u64, C.c_void_p, P(HistStats)]
+        L.orc_sphere_order.argtypes = [f64, f64, u32, P(f64), P(f64), P(f64)]
+        L.orc_sphere_order.restype = None
+        L.orc_normalize3.argtypes = [P(f64), P(f64)]
+        L.orc_sphere_theta_hist.argtypes = [u32, f32, f64, u32, u32, C.c_void_p]
+        L.orc_sphere_theta_hist.restype = None
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def ref() -> C.CDLL:
+    global _ref
+    if _ref is None:
+        R = C.CDLL(str(REF_SO))
+        R.ref_u_edges.argtypes = [u32, P(f64)]
+        R.ref_make_frame.argtypes = [P(f64), P(f64), P(f64), P(f64)]
+        R.ref_phase_bin.argtypes = [u32, u32, P(f64), u64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]
+        R.ref_hex_trace.argtypes = [f64, f64, P(f64), f64, f64, u64, u64, u32, f64, u64, u64, C.c_void_p, u64,
+                                    C.c_void_p]
+        R.ref_hex_trace.restype = i64
+        R.ref_hex_masses.argtypes = [f64, f64, P(f64), f64, f64, u64, u64, u32, f64, u32, u32, C.c_void_p,
+                                     P(f64), P(f64)]
+        R.ref_build_host_cdf.argtypes = [C.c_void_p, u32, u32, P(f64), u32, u32, f64, f64, f64, C.c_void_p,
+                                         C.c_void_p, P(RefCdfReport)]
+        R.ref_water_index.argtypes = [f64, f64, f64, P(f64), P(f64)]
+        R.ref_projected_area.argtypes = [P(f64), f64, P(f64), P(f64), P(C.c_int)]
+        _ref = R
+    return _ref
+
+
+def vec3(x) -> "C.Array":
+    return (f64 * 3)(*[float(v) for v in x])
+
+
+# ---------------------------------------------------------------- oracle helpers
+def u_edges(n: int) -> np.ndarray:
+    out = np.empty(n + 1, np.float64)
+    assert lib().orc_u_edges(n, _ptr(out, P(f64))) == 0
+    return out
+
+
+def make_frame(k) -> Frame:
+    f = Frame()
+    assert lib().orc_make_frame(vec3(k), C.byref(f)) == 0
+    return f
+
+
+def drop_frame(k) -> Frame:
+    f = Frame()
+    assert lib().orc_drop_frame(vec3(k), C.byref(f)) == 0
+    return f
+
+
+def phase_bin(edges: np.ndarray, nt: int, np_: int, frame: Frame, d) -> Bin:
+    return lib().orc_phase_bin(_ptr(edges, P(f64)), nt, np_, C.byref(frame), vec3(d))
+
+
+def hex_settings(radius_mm=0.05, length_mm=0.2, ki=(1, -0.4, 0.3), n_int=1.31, n_ext=1.0, tail=1e-12,
+                 seed=12345, total=1024, max_hits=12) -> HexSettings:
+    s = HexSettings()
+    s.radius_mm, s.length_mm = radius_mm, length_mm
+    s.ki = (f64 * 3)(*ki)
+    s.n_ext, s.n_int, s.tail, s.seed, s.total, s.max_hits = n_ext, n_int, tail, seed, total, max_hits
+    return s
+
+
+def hex_trace(s: HexSettings, first: int, count: int, cap: int = 1 << 20):
+    samples = np.zeros(cap, SAMPLE_DTYPE)
+    audits = np.zeros(count, AUDIT_DTYPE)
+    n = lib().orc_hex_trace(C.byref(s), first, count, samples.ctypes.data, cap, audits.ctypes.data)
+    assert n >= 0, n
+    return samples[:n], audits
+
+
+def hex_hist(s: HexSettings, nt: int, np_: int, qbits: int | None = None, first: int = 0,
+             count: int | None = None):
+    count = s.total if count is None else count
+    qbits = lib().orc_quantum_bits(s.total) if qbits is None else qbits
+    bins = np.zeros(nt * np_, np.uint64)
+    st = HistStats()
+    assert lib().orc_hex_hist(C.byref(s), nt, np_, qbits, first, count, bins.ctypes.data, C.byref(st)) == 0
+    return bins, st
+
+
+def bins_to_mass(bins: np.ndarray, qf: int, qb: int, nt: int, np_: int) -> np.ndarray:
+    m = np.empty(nt * np_, np.float64)
+    lib().orc_bins_to_mass(np.ascontiguousarray(bins, np.uint64).ctypes.data, qf, qb, nt, np_, m.ctypes.data)
+    return m
+
+
+def build_cdf(mass: np.ndarray, nt: int, np_: int, ont: int | None = None, onp: int | None = None,
+              max_l1=1e-10, max_lost=1e-12, max_tv=1.0):
+    ont = nt if ont is None else ont
+    onp = np_ if onp is None else onp
+    mass = np.ascontiguousarray(mass, np.float64)
+    phi = np.zeros(onp + 1, np.float64)
+    theta = np.zeros(onp * (ont + 1), np.float64)
+    r = CdfReport()
+    rc = lib().orc_build_cdf(mass.ctypes.data, nt, np_, ont, onp, max_l1, max_lost, max_tv, phi.ctypes.data,
+                             theta.ctypes.data, C.byref(r))
+    return rc, phi, theta.reshape(onp, ont + 1), r
+
+
+def drop_record(coeffs=(0,) * 8, sphere=True, alpha_deg=0.0, n_int=1.3330, grid=64, max_hits=9,
+                tail=0.0) -> DropRecord:
+    r = DropRecord()
+    c = (f64 * 8)(*[float(v) for v in coeffs])
+    assert lib().orc_drop_make(c, 1 if sphere else 0, alpha_deg, n_int, grid, grid, max_hits, tail,
+                               C.byref(r)) == 0
+    return r
+
+
+def drop_trace(r: DropRecord, precision: int = 32, first: int = 0, count: int | None = None, cap=1 << 22):
+    count = r.grid_w * r.grid_h if count is None else count
+    samples = np.zeros(cap, SAMPLE_DTYPE)
+    audits = np.zeros(count, AUDIT_DTYPE)
+    n = lib().orc_drop_trace(C.byref(r), precision, first, count, samples.ctypes.data, cap, audits.ctypes.data)
+    assert n >= 0
+    return samples[:n], audits
+
+
+def drop_hist(r: DropRecord, nt: int, np_: int, precision: int = 32, qbits: int | None = None, first=0,
+              count=None):
+    total = r.grid_w * r.grid_h
+    count = total if count is None else count
+    qbits = lib().orc_quantum_bits(total) if qbits is None else qbits
+    bins = np.zeros(nt * np_, np.uint64)
+    st = HistStats()
+    assert lib().orc_drop_hist(C.byref(r), precision, nt, np_, qbits, first, count, bins.ctypes.data,
+                               C.byref(st)) == 0
+    return bins, st
+
+
+def sphere_theta_hist(grid: int, half_extent: float, n: float, max_hits: int, nt: int) -> np.ndarray:
+    out = np.zeros(nt, np.float64)
+    lib().orc_sphere_theta_hist(grid, half_extent, n, max_hits, nt, out.ctypes.data)
+    return out
+
+
+def quantum_bits(rays: int) -> int:
+    return int(lib().orc_quantum_bits(rays))
+
+
+def sphere_order(b: float, n: float, p: int):
+    psi, ps, pp = f64(), f64(), f64()
+    lib().orc_sphere_order(b, n, p, C.byref(psi), C.byref(ps), C.byref(pp))
+    return psi.value, ps.value, pp.value
+
+
+# ---------------------------------------------------------------- reference shim helpers
+def ref_hex_trace(s: HexSettings, first: int, count: int, cap: int = 1 << 20):
+    samples = np.zeros(cap, SAMPLE_DTYPE)
+    audits = np.zeros(count, AUDIT_DTYPE)
+    n = ref().ref_hex_trace(s.radius_mm, s.length_mm, s.ki, s.n_ext, s.n_int, s.seed, s.total, s.max_hits,
+                            s.tail, first, count, samples.ctypes.data, cap, audits.ctypes.data)
+    assert n >= 0, n
+    return samples[:n], audits
+
+
+def ref_phase_bin(nt: int, np_: int, k, dirs: np.ndarray):
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    n = dirs.shape[0]
+    index = np.zeros(n, np.uint64)
+    pole = np.zeros(n, np.int32)
+    valid = np.zeros(n, np.int32)
+    snapped = np.zeros(n, np.int32)
+    assert ref().ref_phase_bin(nt, np_, vec3(k), n, dirs.ctypes.data, index.ctypes.data, pole.ctypes.data,
+                               valid.ctypes.data, snapped.ctypes.data) == 0
+    return index, pole, valid, snapped
+
+
+def ref_build_cdf(mass: np.ndarray, nt: int, np_: int, k, ont=None, onp=None, max_l1=1e-10, max_lost=1e-12,
+                  max_tv=1.0):
+    ont = nt if ont is None else ont
+    onp = np_ if onp is None else onp
+    mass = np.ascontiguousarray(mass, np.float64)
+    phi = np.zeros(onp + 1, np.float64)
+    theta = np.zeros(onp * (ont + 1), np.float64)
+    r = RefCdfReport()
+    rc = ref().ref_build_host_cdf(mass.ctypes.data, nt, np_, vec3(k), ont, onp, max_l1, max_lost, max_tv,
+                                  phi.ctypes.data, theta.ctypes.data, C.byref(r))
+    return rc, phi, theta.reshape(onp, ont + 1), r
+
+
+def ref_hex_masses(s: HexSettings, nt: int, np_: int):
+    mass = np.zeros(nt * np_, np.float64)
+    fw, bw = f64(), f64()
+    assert ref().ref_hex_masses(s.radius_mm, s.length_mm, s.ki, s.n_ext, s.n_int, s.seed, s.total, s.max_hits,
+                                s.tail, nt, np_, mass.ctypes.data, C.byref(fw), C.byref(bw)) == 0
+    return mass, fw.value, bw.value
+
+
+def ref_water_index(lam_nm: float, celsius: float = 20.0, pressure: float = 101325.0):
+    n, rho = f64(), f64()
+    rc = ref().ref_water_index(lam_nm, celsius, pressure, C.byref(n), C.byref(rho))
+    return rc, n.value, rho.value
+
+
+def ref_projected_area(coeffs, radius_mm, k):
+    area = f64()
+    sph = C.c_int()
+    rc = ref().ref_projected_area((f64 * 8)(*coeffs), radius_mm, vec3(k), C.byref(area), C.byref(sph))
+    return rc, area.value, bool(sph.value)
+
+
+def oracle_env_ok() -> bool:
+    return ORACLE_SO.exists() or (ROOT / "oracle" / "Makefile").exists()
+
+
+os.environ.setdefault("ICEPHASE_ROOT", str(ROOT))
