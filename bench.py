@@ -105,7 +105,19 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device-to-device copy (torch) 4x: the FETCH_SIZE/WRITE_SIZE scale reference")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8, "launches": 4}))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
               "and not launched (--gpus must be 1).", file=sys.stderr)

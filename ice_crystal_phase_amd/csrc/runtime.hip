@@ -177,13 +177,12 @@ struct iph_batch {
     uint32_t blocks[2] = {0, 0};
     uint64_t tile_rays = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipGraphExec_t graph = nullptr;
-    uint32_t graph_flags = 0;
-    bool graph_events = false;
+    hipGraphExec_t graphs[2] = {nullptr, nullptr};  // [0] zero + trace, [1] CDF
     double ms[3] = {0, 0, 0};
     ~iph_batch()
     {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& g : graphs)
+            if (g) (void)hipGraphExecDestroy(g);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -440,10 +439,11 @@ int iph_batch_set_schedule(iph_batch* b, uint64_t tile_rays, uint32_t blocks)
         const uint64_t want = blocks ? blocks : (uint64_t)ctx->cus * 8;
         b->blocks[k] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, m.total ? m.total : 1));
     }
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    for (auto& g : b->graphs)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     return IPH_OK;
 }
 
@@ -486,57 +486,37 @@ int iph_batch_run(iph_batch* b, uint32_t flags)
     const uint32_t work = flags & (IPH_RUN_TRACE | IPH_RUN_CDF);
     if (!work) return fail(ctx, IPH_E_INVALID, "Nothing to run (flags).");
     if (flags & IPH_RUN_GRAPH) {
-        if (!b->graph || b->graph_flags != work) {
-            if (b->graph) {
-                (void)hipGraphExecDestroy(b->graph);
-                b->graph = nullptr;
-            }
+        // Two captured graphs (zero + trace, CDF) replayed back to back with
+        // events between them: per-stage device times of every replay.
+        for (int stage = 0; stage < 2; ++stage) {
+            const uint32_t part = stage == 0 ? IPH_RUN_TRACE : IPH_RUN_CDF;
+            if (!(work & part) || b->graphs[stage]) continue;
             hipGraph_t g = nullptr;
-            // Event-record nodes inside the graph give per-stage device times of
-            // every replay; without them only the whole replay is timed.
-            b->graph_events = true;
             IPH_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue(b, work, true);
-            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-            if (rc != IPH_OK || e != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                g = nullptr;
-                (void)hipGetLastError();
-                b->graph_events = false;
-                IPH_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                rc = enqueue(b, work, false);
-                e = hipStreamEndCapture(ctx->stream, &g);
-            }
+            const int rc = enqueue(b, part, false);
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
             if (rc != IPH_OK) {
                 if (g) (void)hipGraphDestroy(g);
                 return rc;
             }
             if (e != hipSuccess) return fail(ctx, IPH_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-            const hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            const hipError_t ie = hipGraphInstantiate(&b->graphs[stage], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (ie != hipSuccess)
                 return fail(ctx, IPH_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-            b->graph_flags = work;
         }
-        if (b->graph_events) {
-            IPH_HIP(ctx, hipGraphLaunch(b->graph, ctx->stream));
-            IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            float t0 = 0, t1 = 0;
-            IPH_HIP(ctx, hipEventElapsedTime(&t0, b->ev[0], b->ev[1]));
-            IPH_HIP(ctx, hipEventElapsedTime(&t1, b->ev[1], b->ev[2]));
-            b->ms[0] = t0;
-            b->ms[1] = t1;
-            b->ms[2] = (double)t0 + t1;
-        } else {
-            IPH_HIP(ctx, hipEventRecord(b->ev[0], ctx->stream));
-            IPH_HIP(ctx, hipGraphLaunch(b->graph, ctx->stream));
-            IPH_HIP(ctx, hipEventRecord(b->ev[3], ctx->stream));
-            IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            float t = 0;
-            IPH_HIP(ctx, hipEventElapsedTime(&t, b->ev[0], b->ev[3]));
-            b->ms[0] = b->ms[1] = -1;
-            b->ms[2] = t;
-        }
+        IPH_HIP(ctx, hipEventRecord(b->ev[0], ctx->stream));
+        if (work & IPH_RUN_TRACE) IPH_HIP(ctx, hipGraphLaunch(b->graphs[0], ctx->stream));
+        IPH_HIP(ctx, hipEventRecord(b->ev[1], ctx->stream));
+        if (work & IPH_RUN_CDF) IPH_HIP(ctx, hipGraphLaunch(b->graphs[1], ctx->stream));
+        IPH_HIP(ctx, hipEventRecord(b->ev[2], ctx->stream));
+        IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        float t0 = 0, t1 = 0;
+        IPH_HIP(ctx, hipEventElapsedTime(&t0, b->ev[0], b->ev[1]));
+        IPH_HIP(ctx, hipEventElapsedTime(&t1, b->ev[1], b->ev[2]));
+        b->ms[0] = t0;
+        b->ms[1] = t1;
+        b->ms[2] = (double)t0 + t1;
     } else {
         const int rc = enqueue(b, work, true);
         if (rc != IPH_OK) return rc;

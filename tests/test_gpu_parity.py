@@ -227,8 +227,11 @@ def test_full_size_sphere_record_properties(ctx, oracle):
     assert rep.status == 0 and abs(rep.cdf_mass - 1) < 1e-12
     assert phi[0] == 0 and phi[-1] == 1 and np.all(np.diff(phi) >= 0)
     assert np.all(theta[:, 0] == 0) and np.all(theta[:, -1] == 1) and np.all(np.diff(theta, axis=1) >= 0)
-    # axisymmetric at alpha=0: the phi marginal is uniform to sampling accuracy
-    assert np.max(np.abs(np.diff(phi) * np_ - 1)) < 0.05
+    # axisymmetric at alpha=0: the phi marginal is uniform once the square
+    # incident lattice's azimuthal moire (2048^2 rays vs 1800 columns) is averaged
+    # over 10-degree groups
+    dphi = np.diff(phi).reshape(36, 50).sum(1) * 36
+    assert np.max(np.abs(dphi - 1)) < 0.01, np.max(np.abs(dphi - 1))
     # closed-form theta distribution of the same 2^22 impact parameters (Snell,
     # Fresnel, deviation D_p per order; no ray-surface intersection involved):
     # only FP32 rounding at cell edges may move a sample to a neighbour row.
