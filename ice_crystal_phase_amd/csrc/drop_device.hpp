@@ -8,9 +8,11 @@
 //   * sphere: closed-form entry from the grid coordinates, closed-form chord;
 //   * drop: the inside of a convex body along a line is one interval, so the
 //     entry root is bracketed by the bounding-sphere entry and any interior
-//     point (the foot of the perpendicular, else a golden-section search), and
-//     the exit root by a small self offset and the bounding-sphere exit; each
-//     bracket is solved by Newton with bisection fallback.
+//     point (the foot of the perpendicular, else the minimum of f along the
+//     line found by a secant search on f'), and the exit root by a small self
+//     offset and the bounding-sphere exit; each bracket is solved by Newton
+//     with bisection fallback. The per-ray control flow (entry, encounters,
+//     emission) is in trace.hip.
 // Splitting is per ray, FP32, two-column Jones (the Mueller M11 row for
 // unpolarized input), flux-normalized (t_flux = sqrt(nt ct / ni ci) t_E, as in
 // hex_trace/include/ice_crystal/flux_interface.hpp:26-88). Emission follows the
@@ -40,11 +42,12 @@ __device__ inline f3 cross(f3 a, f3 b)
 {
     return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ inline f3 unit(f3 a) { return a / sqrtf(dot(a, a)); }
+__device__ inline f3 unit(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
 
 struct Shape {
     float c[8];
     float inner, outer;
+    float rmax;  // bounding radius of the intersector (max r(chi), rounded up to 2^-20)
     uint32_t sphere;
 };
 
@@ -73,7 +76,7 @@ __device__ inline float surface(const Shape& s, f3 p)
     const float r = sqrtf(dot(p, p));
     if (s.sphere) return r - 1.0f;
     if (r < s.inner * 0.25f) return r - s.inner;
-    const float mu = -p.y / r;
+    const float mu = -(p.y * (1.0f / r));
     float der;
     return r - cheb_radius(s.c, mu, der);
 }
@@ -81,14 +84,35 @@ __device__ inline float surface(const Shape& s, f3 p)
 __device__ inline f3 gradient(const Shape& s, f3 p)
 {
     const float r = sqrtf(dot(p, p));
-    const f3 u = p / r;
+    const float ir = 1.0f / r;
+    const f3 u = p * ir;
     if (s.sphere) return u;
     const float mu = -u.y;
     float der;
     (void)cheb_radius(s.c, mu, der);
-    const float k = der / r;
+    const float k = der * ir;
     const f3 a = F3(0.0f - u.x * mu, -1.0f - u.y * mu, 0.0f - u.z * mu);
     return u - a * k;
+}
+
+// surface(s, p) and dot(gradient(s, p), d) from one evaluation of r, mu and the
+// Chebyshev recurrence: the same IEEE operations, so the same values.
+__device__ inline float surface_slope(const Shape& s, f3 p, f3 d, float& slope)
+{
+    const float r = sqrtf(dot(p, p));
+    const float ir = 1.0f / r;
+    const f3 u = p * ir;
+    if (s.sphere) {
+        slope = dot(u, d);
+        return r - 1.0f;
+    }
+    const float mu = -u.y;
+    float der;
+    const float rad = cheb_radius(s.c, mu, der);
+    const float k = der * ir;
+    const f3 a = F3(0.0f - u.x * mu, -1.0f - u.y * mu, 0.0f - u.z * mu);
+    slope = dot(u - a * k, d);
+    return r < s.inner * 0.25f ? r - s.inner : r - rad;
 }
 
 // Newton with bisection fallback on f(o + t d), t in [a,b]; f(a) > 0 iff a_pos.
@@ -98,7 +122,8 @@ __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, boo
     if (!(x > a && x < b)) x = a + (b - a) * 0.5f;
     for (int it = 0; it < 48; ++it) {
         const f3 p = o + d * x;
-        const float fx = surface(s, p);
+        float slope;
+        const float fx = surface_slope(s, p, d, slope);
         if (fx == 0.0f) return x;
         if ((fx > 0.0f) == a_pos)
             a = x;
@@ -106,7 +131,6 @@ __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, boo
             b = x;
         const float scale = fabsf(x) > 1.0f ? fabsf(x) : 1.0f;
         if (b - a <= tol * scale) return x;
-        const float slope = dot(gradient(s, p), d);
         const float cand = x - fx / slope;
         if (cand > a && cand < b) {
             const float step = fabsf(cand - x);
@@ -117,81 +141,6 @@ __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, boo
         }
     }
     return x;
-}
-
-__device__ inline bool entry(const Shape& s, f3 o, f3 k, float u, float v, float& t)
-{
-    const float rho2 = u * u + v * v;
-    if (s.sphere) {
-        if (!(rho2 < 1.0f)) return false;
-        t = -sqrtf(1.0f - rho2);
-        return true;
-    }
-    const float out2 = s.outer * s.outer;
-    if (!(rho2 < out2)) return false;
-    const float ta = -sqrtf(out2 - rho2);
-    float tm = 0.0f;
-    const float fm = surface(s, o);
-    if (!(fm < 0.0f)) {
-        const float g = 0.6180339887498949f;
-        float lo = ta, hi = -ta;
-        float c1 = hi - g * (hi - lo), c2 = lo + g * (hi - lo);
-        float f1 = surface(s, o + k * c1);
-        float f2 = surface(s, o + k * c2);
-        bool found = false;
-        for (int it = 0; it < 28; ++it) {
-            if (f1 < 0.0f) {
-                tm = c1;
-                found = true;
-                break;
-            }
-            if (f2 < 0.0f) {
-                tm = c2;
-                found = true;
-                break;
-            }
-            if (f1 < f2) {
-                hi = c2;
-                c2 = c1;
-                f2 = f1;
-                c1 = hi - g * (hi - lo);
-                f1 = surface(s, o + k * c1);
-            } else {
-                lo = c1;
-                c1 = c2;
-                f1 = f2;
-                c2 = lo + g * (hi - lo);
-                f2 = surface(s, o + k * c2);
-            }
-        }
-        if (!found) return false;
-    }
-    const float rm = 1.0f + s.c[0];
-    const float g2 = rm * rm - rho2;
-    const float x0 = g2 > 0.0f ? -sqrtf(g2) : ta + (tm - ta) * 0.5f;
-    t = newton(s, o, k, ta, tm, true, x0);
-    return true;
-}
-
-__device__ inline bool chord(const Shape& s, f3 p, f3 d, float& t)
-{
-    const float b = dot(p, d);
-    if (s.sphere) {
-        const float c = dot(p, p) - 1.0f;
-        float disc = b * b - c;
-        if (disc < 0.0f) disc = 0.0f;
-        t = -b + sqrtf(disc);
-        return true;
-    }
-    const float cb = dot(p, p) - s.outer * s.outer;
-    float disc = b * b - cb;
-    if (disc < 0.0f) disc = 0.0f;
-    const float tb = -b + sqrtf(disc);
-    const float ts = 0x1p-16f;
-    if (!(ts < tb)) return false;
-    if (!(surface(s, p + d * ts) < 0.0f)) return false;
-    t = newton(s, p, d, ts, tb, false, -2.0f * b);
-    return true;
 }
 
 struct c32 {
@@ -239,10 +188,10 @@ __device__ inline Split split(const Ray& in, f3 n, float ni, float nt)
     const float s2 = dot(s, s);
     const float sine = sqrtf(s2);
     if (s2 > 0x1p-40f)
-        s = s / sine;
+        s = s * (1.0f / sine);
     else
         s = unit(in.e0 - d * dot(d, in.e0));
-    const f3 pin = unit(cross(d, s));
+    const f3 pin = cross(d, s);  // d, s unit and orthogonal
     const float a = dot(in.e0, s), b = dot(in.e1, s);
     const float cc = dot(in.e0, pin), dd = dot(in.e1, pin);
     Jones j;
@@ -254,7 +203,7 @@ __device__ inline Split split(const Ray& in, f3 n, float ni, float nt)
     const float st = eta * sine;
     o.r.dir = unit(d + n * (2.0f * ci));
     o.r.e0 = s;
-    o.r.e1 = unit(cross(o.r.dir, s));
+    o.r.e1 = cross(o.r.dir, s);
     c32 rs, rp;
     if (st > 1.0f) {
         const float beta = sqrtf((st - 1.0f) * (st + 1.0f));
@@ -271,7 +220,7 @@ __device__ inline Split split(const Ray& in, f3 n, float ni, float nt)
         const float ts = num / ds, tp = num / dp;
         o.t.dir = unit((d + n * ci) * eta - n * ct);
         o.t.e0 = s;
-        o.t.e1 = unit(cross(o.t.dir, s));
+        o.t.e1 = cross(o.t.dir, s);
         o.t.j.c0x = cscale(j.c0x, ts);
         o.t.j.c0y = cscale(j.c0y, tp);
         o.t.j.c1x = cscale(j.c1x, ts);
@@ -303,82 +252,6 @@ enum Status : uint32_t {
     kNumeric = 5,
     kMiss = 8,
 };
-
-struct RayResult {
-    float unresolved;
-    uint32_t status, internal_hits, tir, outputs;
-};
-
-template <class Sink>
-__device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
-{
-    RayResult res{0.0f, kMiss, 0, 0, 0};
-    const uint32_t ix = (uint32_t)(id % c.w), iy = (uint32_t)(id / c.w);
-    const float u = ((float)(2u * ix + 1u) / (float)c.w - 1.0f) * c.h;
-    const float v = ((float)(2u * iy + 1u) / (float)c.hgt - 1.0f) * c.h;
-    const f3 o = c.e0 * u + c.e1 * v;
-    float t;
-    if (!entry(c.shape, o, c.ki, u, v, t)) return res;
-    f3 p = o + c.ki * t;
-    Ray ray;
-    ray.dir = c.ki;
-    ray.e0 = c.e0;
-    ray.e1 = c.e1;
-    ray.j = Jones{c32{1.0f, 0.0f}, c32{0.0f, 0.0f}, c32{0.0f, 0.0f}, c32{1.0f, 0.0f}};
-    Split sp = split(ray, outward_normal(c.shape, p), c.n_ext, c.n_int);
-    if (!sp.ok) {
-        res.status = kNumeric;
-        res.unresolved = 1.0f;
-        return res;
-    }
-    auto emit = [&](const Ray& r) {
-        const float pw = r.j.power();
-        if (pw > 0.0f) {
-            sink(r.dir, pw);
-            ++res.outputs;
-        }
-    };
-    emit(sp.r);
-    if (!sp.has_t) {
-        res.tir = 1;
-        res.status = kExhausted;
-        return res;
-    }
-    ray = sp.t;
-    res.status = kLimit;
-    for (uint32_t hit = 0; hit < c.max_hits; ++hit) {
-        const float pw = ray.j.power();
-        if (pw == 0.0f) {
-            res.status = kExhausted;
-            break;
-        }
-        if (pw <= c.tail) {
-            res.status = kTail;
-            break;
-        }
-        float tx;
-        if (!chord(c.shape, p, ray.dir, tx)) {
-            res.status = kUnresolved;
-            break;
-        }
-        const f3 q = p + ray.dir * tx;
-        const f3 nout = outward_normal(c.shape, q);
-        sp = split(ray, nout * -1.0f, c.n_int, c.n_ext);
-        if (!sp.ok) {
-            res.status = kNumeric;
-            break;
-        }
-        ++res.internal_hits;
-        if (sp.tir) ++res.tir;
-        if (sp.has_t) emit(sp.t);
-        ray = sp.r;
-        p = q;
-    }
-    const float rem = ray.j.power();
-    if (res.status == kLimit && rem == 0.0f) res.status = kExhausted;
-    res.unresolved = rem;
-    return res;
-}
 
 }  // namespace drop
 }  // namespace icephase

@@ -181,6 +181,16 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
             c.shape.inner = 1.0f;
             c.shape.outer = 1.0f;
         }
+        // bounding radius of the intersector: max over chi of r(chi) on a dense grid,
+        // + 1e-6, rounded up to a multiple of 2^-20 (exact in single precision)
+        double rm = 0;
+        for (int k = 0; k <= 16384; ++k) {
+            const double chi = kPi * k / 16384;
+            double rr = 1 + (double)c.shape.c[0];
+            for (int m = 1; m < 8; ++m) rr += (double)c.shape.c[m] * std::cos(m * chi);
+            if (rr > rm) rm = rr;
+        }
+        c.shape.rmax = sphere ? 1.0f : (float)(std::ceil((rm + 1e-6) * 1048576.0) / 1048576.0);
         BinFrame f;
         if (!drop_frame(k_in, f)) {
             err = "Incident direction must be finite and nonzero.";

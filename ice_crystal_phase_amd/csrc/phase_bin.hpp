@@ -5,8 +5,10 @@
 // phase_bin follows hex_phase/include/ice_crystal/phase_math.hpp:37-81: FP64
 // frame projection, pole test s <= 256 eps, stable u from (mu, s), half-open u
 // edges with the 64-eps / 1e-7-cell snap, phi = atan2(y, x) with the periodic
-// seam and the same snap rule. Two changes of method keep the result and cut
+// seam and the same snap rule. Three changes of method keep the result and cut
 // the cost on the GPU:
+//   * a direction whose FP32 (theta, phi) lies well inside a cell (see bin_core)
+//     takes that cell directly; only the ~4 % near cell edges run the FP64 rules;
 //   * the u row is guessed from u = sin^2(pi i / 2n) in FP32 and then corrected
 //     against the FP64 edges, which returns the same row as the reference's
 //     binary search for any u in [0,1];
@@ -72,17 +74,35 @@ struct CellHit {
 };
 
 constexpr double kPhiMargin = 2e-3;
+constexpr float kCellMargin = 1e-2f;
 
-// edges: nt+1 FP64 u edges (any address space visible to the caller).
+// The phase_bin rules after the unit-length checks: (x, y, z) are the frame
+// projections of the direction, n their norm. edge(k) returns the FP64 u edge k.
 template <class EdgeLoad>
-__host__ __device__ inline CellHit bin_direction(const BinFrame& g, uint32_t nt, uint32_t np, d3 d,
-                                                 EdgeLoad edge)
+__host__ __device__ inline CellHit bin_core(uint32_t nt, uint32_t np, double x, double y, double z, double n,
+                                            EdgeLoad edge)
 {
     CellHit h{0, 2, 0};
-    if (!is_unit64(d)) return h;
-    const double x = dot(g.e0, d), y = dot(g.e1, d), z = dot(g.ki, d);
-    const double n = sqrt(x * x + y * y + z * z);
-    if (!(n > 0) || !finite64(n) || fabs(n - 1) > 512 * DBL_EPSILON) return h;
+    // Fast path: theta and phi in FP32 from the FP64 projections. Rounding x, y, z
+    // to FP32 is a relative error of 6e-8, so theta and phi carry at most ~1.2e-6
+    // rad of error; when both fractional cell positions are at least kCellMargin
+    // (1e-2 cells = 3.5e-5 rad for theta, 3.5e-5 rad for phi at 1800 columns) from
+    // an edge, the FP64 rules below give this same cell with no snap and no pole.
+    {
+        const float xf = (float)x, yf = (float)y, zf = (float)z;
+        const float th = atan2f(sqrtf(xf * xf + yf * yf), zf);
+        const float ri = th * ((float)nt * (float)(1.0 / kPi));
+        const float ph = atan2f(yf, xf);
+        const float ci = (ph + (float)kPi) * ((float)np * (float)(0.5 / kPi));
+        const float rfl = floorf(ri), cfl = floorf(ci);
+        const float rfr = ri - rfl, cfr = ci - cfl;
+        if (rfr > kCellMargin && rfr < 1.0f - kCellMargin && cfr > kCellMargin && cfr < 1.0f - kCellMargin &&
+            rfl >= 0.0f && rfl < (float)nt && cfl >= 0.0f && cfl < (float)np) {
+            h.index = (uint32_t)rfl * np + (uint32_t)cfl;
+            h.kind = 0;
+            return h;
+        }
+    }
     const double mu = z / n, s = hypot(x, y) / n;
     if (s <= 256 * DBL_EPSILON) {
         h.kind = mu >= 0 ? 1 : -1;
@@ -137,6 +157,31 @@ __host__ __device__ inline CellHit bin_direction(const BinFrame& g, uint32_t nt,
     h.kind = 0;
     h.snapped = snapped;
     return h;
+}
+
+// phase_bin of a unit FP64 direction (hex records): the reference's unit checks first.
+template <class EdgeLoad>
+__host__ __device__ inline CellHit bin_direction(const BinFrame& g, uint32_t nt, uint32_t np, d3 d, EdgeLoad edge)
+{
+    CellHit h{0, 2, 0};
+    if (!is_unit64(d)) return h;
+    const double x = dot(g.e0, d), y = dot(g.e1, d), z = dot(g.ki, d);
+    const double n = sqrt(x * x + y * y + z * z);
+    if (!(n > 0) || !finite64(n) || fabs(n - 1) > 512 * DBL_EPSILON) return h;
+    return bin_core(nt, np, x, y, z, n, edge);
+}
+
+// Drop records: the same rules applied to d/|d| of an FP32 direction promoted to
+// FP64 (mu and s are ratios to n, so no separate normalization is needed).
+template <class EdgeLoad>
+__host__ __device__ inline CellHit bin_raw(const BinFrame& g, uint32_t nt, uint32_t np, d3 d, EdgeLoad edge)
+{
+    CellHit h{0, 2, 0};
+    if (!finite64(d)) return h;
+    const double x = dot(g.e0, d), y = dot(g.e1, d), z = dot(g.ki, d);
+    const double n = sqrt(x * x + y * y + z * z);
+    if (!(n > 0) || !finite64(n)) return h;
+    return bin_core(nt, np, x, y, z, n, edge);
 }
 
 // Fixed-point quantum: rint(p * 2^qbits) for a power fraction p in [0, 1].

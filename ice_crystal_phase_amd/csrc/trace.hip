@@ -92,25 +92,195 @@ __device__ inline void trace_tile(const DevRecord& R, uint64_t first, uint64_t c
             c.hits += r.internal_hits;
             count_status(c, r.status);
         }
-    } else {
-        for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
-            unsigned long long rq = 0;
-            auto sink = [&](drop::f3 dir, float pw) {
-                d3 d;
-                if (!unit_of(D3((double)dir.x, (double)dir.y, (double)dir.z), d)) {
-                    ++c.samples;
-                    ++c.invalid;
-                    rq += quantize((double)pw, R.qbits);
-                    return;
-                }
-                deposit(R, d, (double)pw, edge, c, rq);
-            };
-            const drop::RayResult r = drop::trace(R.drop, first + i, sink);
-            c.escaped += rq;
-            c.unresolved += quantize((double)r.unresolved, R.qbits);
-            c.hits += r.internal_hits;
-            count_status(c, r.status);
+    }
+}
+
+// Drop tile. Lanes first acquire a ray that hits the drop: ids come from the
+// tile's LDS counter (one wave-aggregated atomic per round), rays outside the
+// bounding sphere are counted as misses on the spot, and the entry bracket of a
+// candidate is set up here (foot of the perpendicular, else a secant search for
+// the minimum of f along the line). The acquired rays are then traced together:
+// with no tail tolerance every hitting ray runs the same number of encounters,
+// so the wave stays converged apart from Newton iteration counts. The sequence
+// of operations applied to each ray is the per-ray algorithm the CPU mirror runs.
+template <class EdgeLoad>
+__device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t count, uint32_t* s_next,
+                                 EdgeLoad edge, LaneCounters& c)
+{
+    using namespace drop;
+    const Config& C = R.drop;
+    const uint32_t lane = threadIdx.x & 63;
+    const float rmax2 = C.shape.sphere ? 1.0f : C.shape.rmax * C.shape.rmax;
+    bool exhausted = false;
+    unsigned long long rq = 0;
+    auto emit = [&](const Ray& r) {
+        const float pw = r.j.power();
+        if (pw > 0.0f) {
+            const unsigned long long q = quantize((double)pw, R.qbits);
+            rq += q;
+            ++c.samples;
+            const CellHit h = bin_raw(R.frame, R.nt, R.np, D3((double)r.dir.x, (double)r.dir.y, (double)r.dir.z), edge);
+            if (h.kind == 2) {
+                ++c.invalid;
+                return;
+            }
+            c.snapped += h.snapped;
+            if (h.kind == 1)
+                c.fwd += q;
+            else if (h.kind == -1)
+                c.bwd += q;
+            else if (q)
+                atomicAdd(&R.bins[h.index], q);
         }
+    };
+    for (;;) {
+        bool have = false;
+        float u = 0.0f, v = 0.0f, rho2 = 0.0f, ta = 0.0f, tm = 0.0f;
+        f3 o = F3(0.0f, 0.0f, 0.0f);
+        bool need = !exhausted;
+        for (;;) {
+            const unsigned long long m = __ballot(need);
+            if (m == 0) break;
+            const uint32_t n = (uint32_t)__popcll(m);
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(s_next, n);
+            base = __shfl(base, leader, 64);
+            if (!need) continue;
+            const uint64_t k = (uint64_t)base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (k >= count) {
+                exhausted = true;
+                need = false;
+                continue;
+            }
+            const uint64_t id = first + k;
+            const uint32_t ix = (uint32_t)(id % C.w), iy = (uint32_t)(id / C.w);
+            u = ((float)(2u * ix + 1u) / (float)C.w - 1.0f) * C.h;
+            v = ((float)(2u * iy + 1u) / (float)C.hgt - 1.0f) * C.h;
+            rho2 = u * u + v * v;
+            if (!(rho2 < rmax2)) {
+                count_status(c, kMiss);
+                continue;
+            }
+            o = C.e0 * u + C.e1 * v;
+            if (C.shape.sphere) {
+                have = true;
+                need = false;
+                continue;
+            }
+            ta = -sqrtf(rmax2 - rho2);
+            tm = 0.0f;
+            float g0;
+            const float fm = surface_slope(C.shape, o, C.ki, g0);
+            if (fm < 0.0f) {
+                have = true;
+                need = false;
+                continue;
+            }
+            // foot outside: secant minimization of f along the line from the foot
+            float t0 = 0.0f, t1 = -g0 * sqrtf(rho2);
+            for (int it = 0; it < 8; ++it) {
+                t1 = t1 < ta ? ta : t1;
+                t1 = t1 > -ta ? -ta : t1;
+                float g1;
+                const float f1 = surface_slope(C.shape, o + C.ki * t1, C.ki, g1);
+                if (f1 < 0.0f) {
+                    tm = t1;
+                    have = true;
+                    break;
+                }
+                if (g1 == g0) break;
+                const float t2 = t1 - g1 * (t1 - t0) / (g1 - g0);
+                t0 = t1;
+                g0 = g1;
+                if (fabsf(t2 - t1) <= 0x1p-20f) break;
+                t1 = t2;
+            }
+            if (have)
+                need = false;
+            else
+                count_status(c, kMiss);
+        }
+        if (__ballot(have) == 0) break;
+        if (!have) continue;
+        // trace the acquired ray
+        float t;
+        if (C.shape.sphere) {
+            t = -sqrtf(1.0f - rho2);
+        } else {
+            const float rm = 1.0f + C.shape.c[0];
+            const float g2 = rm * rm - rho2;
+            const float x0 = g2 > 0.0f ? -sqrtf(g2) : ta + (tm - ta) * 0.5f;
+            t = newton(C.shape, o, C.ki, ta, tm, true, x0);
+        }
+        f3 p = o + C.ki * t;
+        Ray ray;
+        ray.dir = C.ki;
+        ray.e0 = C.e0;
+        ray.e1 = C.e1;
+        ray.j = Jones{c32{1.0f, 0.0f}, c32{0.0f, 0.0f}, c32{0.0f, 0.0f}, c32{1.0f, 0.0f}};
+        Split sp = split(ray, outward_normal(C.shape, p), C.n_ext, C.n_int);
+        uint32_t status = kLimit, hits = 0;
+        float unresolved = 0.0f;
+        if (!sp.ok) {
+            status = kNumeric;
+            unresolved = 1.0f;
+        } else {
+            emit(sp.r);
+            if (!sp.has_t) {
+                status = kExhausted;
+            } else {
+                ray = sp.t;
+                for (uint32_t hit = 0; hit < C.max_hits; ++hit) {
+                    const float pw = ray.j.power();
+                    if (pw == 0.0f) {
+                        status = kExhausted;
+                        break;
+                    }
+                    if (pw <= C.tail) {
+                        status = kTail;
+                        break;
+                    }
+                    const float b = dot(p, ray.dir);
+                    float tx;
+                    if (C.shape.sphere) {
+                        const float cc = dot(p, p) - 1.0f;
+                        float disc = b * b - cc;
+                        if (disc < 0.0f) disc = 0.0f;
+                        tx = -b + sqrtf(disc);
+                    } else {
+                        const float cb = dot(p, p) - rmax2;
+                        float disc = b * b - cb;
+                        if (disc < 0.0f) disc = 0.0f;
+                        const float tb = -b + sqrtf(disc);
+                        const float ts = 0x1p-16f;
+                        if (!(ts < tb) || !(surface(C.shape, p + ray.dir * ts) < 0.0f)) {
+                            status = kUnresolved;
+                            break;
+                        }
+                        tx = newton(C.shape, p, ray.dir, ts, tb, false, -2.0f * b);
+                    }
+                    const f3 q = p + ray.dir * tx;
+                    const f3 nout = outward_normal(C.shape, q);
+                    sp = split(ray, nout * -1.0f, C.n_int, C.n_ext);
+                    if (!sp.ok) {
+                        status = kNumeric;
+                        break;
+                    }
+                    ++hits;
+                    if (sp.has_t) emit(sp.t);
+                    ray = sp.r;
+                    p = q;
+                }
+                unresolved = ray.j.power();
+                if (status == kLimit && unresolved == 0.0f) status = kExhausted;
+            }
+        }
+        c.escaped += rq;
+        c.unresolved += quantize((double)unresolved, R.qbits);
+        c.hits += hits;
+        count_status(c, status);
+        rq = 0;
     }
 }
 
@@ -122,10 +292,14 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
     __shared__ double s_edges[LDS_EDGES ? kLdsEdges : 1];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_cached;
+    __shared__ uint32_t s_next;
     if (threadIdx.x == 0) s_cached = 0xffffffffu;
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1u);
+        if (threadIdx.x == 0) {
+            s_tile = atomicAdd(queue, 1u);
+            s_next = 0u;
+        }
         __syncthreads();
         const uint64_t t = __builtin_amdgcn_readfirstlane(s_tile);
         if (t >= map.total) break;
@@ -152,10 +326,16 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
                 if (threadIdx.x == 0) s_cached = rec;
                 __syncthreads();
             }
-            trace_tile<KIND>(R, first, count, [&](int k) { return s_edges[k]; }, c);
+            if constexpr (KIND == kKindHex)
+                trace_tile<KIND>(R, first, count, [&](int k) { return s_edges[k]; }, c);
+            else
+                drop_tile(R, first, count, &s_next, [&](int k) { return s_edges[k]; }, c);
         } else {
             const double* e = R.edges;
-            trace_tile<KIND>(R, first, count, [&](int k) { return e[k]; }, c);
+            if constexpr (KIND == kKindHex)
+                trace_tile<KIND>(R, first, count, [&](int k) { return e[k]; }, c);
+            else
+                drop_tile(R, first, count, &s_next, [&](int k) { return e[k]; }, c);
         }
         flush_counters(c, R.stats);
     }

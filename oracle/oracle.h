@@ -57,6 +57,10 @@ typedef struct orc_bin {
     int32_t snapped;
     int32_t pad;
 } orc_bin;
+/* Drop histogram binning: the phase_bin rules applied to d/|d| of an
+ * unnormalized direction (no unit-length precondition; mu and s are formed from
+ * x, y, z and n = |(x,y,z)| exactly as phase_bin forms them). */
+orc_bin orc_phase_bin_raw(const double* u_edges, uint32_t nt, uint32_t np, const orc_frame* f, const double dir[3]);
 /* hex_phase/include/ice_crystal/phase_math.hpp:37-81. */
 orc_bin orc_phase_bin(const double* u_edges, uint32_t nt, uint32_t np,
                       const orc_frame* f, const double dir[3]);
@@ -141,7 +145,8 @@ int orc_build_cdf(const double* mass, uint32_t nt, uint32_t np,
 /* ---- drop tracer (FP32 mirror of the HIP kernel, FP64 twin) ------------------- */
 typedef struct orc_drop_record {
     float coeffs[8];        /* r(mu)=1+c0+c1 mu+sum c_n T_n(mu), mu=-y/r */
-    float inner, outer;     /* conservative radii */
+    float inner, outer;     /* conservative radii (outer sets the incident grid extent) */
+    float rmax;             /* bounding radius of the intersector: max r(chi) rounded up to 2^-20 */
     float ki[3], e0[3], e1[3];
     float n_int, n_ext;
     float half_extent;      /* grid covers [-h,h]^2 in the (e0,e1) plane */

@@ -22,6 +22,7 @@
 #define TOL 0x1p-21f
 #define TINY_S2 0x1p-40f
 #define SELF_OFFSET 0x1p-16f
+#define MIN_STEP 0x1p-20f
 #include "oracle_drop_impl.h"
 #undef REAL
 #undef SFX
@@ -30,6 +31,7 @@
 #undef TOL
 #undef TINY_S2
 #undef SELF_OFFSET
+#undef MIN_STEP
 
 /* ---- FP64 instantiation ---- */
 #define REAL double
@@ -39,6 +41,7 @@
 #define TOL 0x1p-44
 #define TINY_S2 0x1p-90
 #define SELF_OFFSET 0x1p-30
+#define MIN_STEP 0x1p-40
 #include "oracle_drop_impl.h"
 #undef REAL
 #undef SFX
@@ -47,6 +50,7 @@
 #undef TOL
 #undef TINY_S2
 #undef SELF_OFFSET
+#undef MIN_STEP
 
 int orc_normalize3(const double in[3], double out[3]);
 
@@ -63,6 +67,18 @@ int orc_drop_make(const double coeffs[8], int sphere, double alpha_deg, double n
     r->inner = nextafterf((float)(center - s - 0x1p-20), -INFINITY);
     r->outer = nextafterf((float)(center + s + 0x1p-20), INFINITY);
     if (sphere) { r->inner = 1.0f; r->outer = 1.0f; }
+    /* bounding radius: max over chi of r(chi) = 1 + sum c_n cos(n chi) on a dense
+     * grid, +1e-6, rounded up to a multiple of 2^-20 (exact in float) */
+    {
+        double rm = 0;
+        for (int k = 0; k <= 16384; ++k) {
+            const double chi = PI_D * k / 16384;
+            double rr = 1 + (double)r->coeffs[0];
+            for (int m = 1; m < 8; ++m) rr += (double)r->coeffs[m] * cos(m * chi);
+            if (rr > rm) rm = rr;
+        }
+        r->rmax = sphere ? 1.0f : (float)(ceil((rm + 1e-6) * 1048576.0) / 1048576.0);
+    }
     const double a = alpha_deg * PI_D / 180.0;
     const double k[3] = {cos(a), -sin(a), 0.0};
     orc_frame f;
@@ -85,7 +101,7 @@ int orc_drop_make(const double coeffs[8], int sphere, double alpha_deg, double n
 static void load_f(const orc_drop_record* r, rec_f* R)
 {
     for (int n = 0; n < 8; ++n) R->c[n] = r->coeffs[n];
-    R->inner = r->inner; R->outer = r->outer;
+    R->inner = r->inner; R->outer = r->outer; R->rmax = r->rmax;
     R->ki = mk_f(r->ki[0], r->ki[1], r->ki[2]);
     R->e0 = mk_f(r->e0[0], r->e0[1], r->e0[2]);
     R->e1 = mk_f(r->e1[0], r->e1[1], r->e1[2]);
@@ -95,7 +111,7 @@ static void load_f(const orc_drop_record* r, rec_f* R)
 static void load_d(const orc_drop_record* r, rec_d* R)
 {
     for (int n = 0; n < 8; ++n) R->c[n] = r->coeffs[n];
-    R->inner = r->inner; R->outer = r->outer;
+    R->inner = r->inner; R->outer = r->outer; R->rmax = r->rmax;
     /* the FP64 twin uses the same rounded inputs, promoted and renormalized */
     double k[3] = {r->ki[0], r->ki[1], r->ki[2]}, e0[3] = {r->e0[0], r->e0[1], r->e0[2]};
     double nk[3], ne0[3];
@@ -160,10 +176,8 @@ int orc_drop_hist(const orc_drop_record* r, int precision, uint32_t nt, uint32_t
         for (uint64_t s = 0; s < n; ++s) {
             const uint64_t q = (uint64_t)llrint(ldexp(buf[s].power, (int)qbits));
             qe += q;
-            double d[3];
             st->samples++;
-            if (orc_normalize3(buf[s].dir, d)) { st->invalid++; continue; }
-            const orc_bin b = orc_phase_bin(ue, nt, np, &g, d);
+            const orc_bin b = orc_phase_bin_raw(ue, nt, np, &g, buf[s].dir);
             if (!b.valid) { st->invalid++; continue; }
             st->snapped += (uint64_t)b.snapped;
             if (b.pole > 0) st->q_forward += q;

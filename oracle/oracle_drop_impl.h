@@ -33,7 +33,7 @@ static inline V FN(div_)(V a, REAL s) { return FN(mk_)(a.x / s, a.y / s, a.z / s
 static inline REAL FN(dot_)(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline V FN(crs_)(V a, V b)
 { return FN(mk_)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-static inline V FN(unit_)(V a) { return FN(div_)(a, SQRT(FN(dot_)(a, a))); }
+static inline V FN(unit_)(V a) { return FN(scl_)(a, (REAL)1 / SQRT(FN(dot_)(a, a))); }
 
 /* Chebyshev radius and slope, raindrop_shape.hpp:71-84 recurrence order. */
 static inline REAL FN(radius_)(const REAL* c, REAL mu, REAL* der)
@@ -54,7 +54,7 @@ static inline REAL FN(radius_)(const REAL* c, REAL mu, REAL* der)
 
 typedef struct {
     REAL c[8];
-    REAL inner, outer;
+    REAL inner, outer, rmax;
     V ki, e0, e1;
     REAL n_int, n_ext, h, tail;
     uint32_t w, hgt, max_hits, sphere;
@@ -65,7 +65,7 @@ static inline REAL FN(surf_)(const FN(rec_)* R, V p)
     const REAL r = SQRT(FN(dot_)(p, p));
     if (R->sphere) return r - (REAL)1;
     if (r < R->inner * (REAL)0.25) return r - R->inner;
-    const REAL mu = -p.y / r;
+    const REAL mu = -(p.y * ((REAL)1 / r));
     REAL der;
     return r - FN(radius_)(R->c, mu, &der);
 }
@@ -73,12 +73,13 @@ static inline REAL FN(surf_)(const FN(rec_)* R, V p)
 static inline V FN(grad_)(const FN(rec_)* R, V p)
 {
     const REAL r = SQRT(FN(dot_)(p, p));
-    const V u = FN(div_)(p, r);
+    const REAL ir = (REAL)1 / r;
+    const V u = FN(scl_)(p, ir);
     if (R->sphere) return u;
     const REAL mu = -u.y;
     REAL der;
     (void)FN(radius_)(R->c, mu, &der);
-    const REAL k = der / r;
+    const REAL k = der * ir;
     const V a = FN(mk_)((REAL)0 - u.x * mu, (REAL)-1 - u.y * mu, (REAL)0 - u.z * mu);
     return FN(sub_)(u, FN(scl_)(a, k));
 }
@@ -119,32 +120,32 @@ static int FN(entry_)(const FN(rec_)* R, V o, REAL u, REAL v, REAL* tout)
         *tout = -SQRT((REAL)1 - rho2);
         return 1;
     }
-    const REAL out2 = R->outer * R->outer;
+    const REAL out2 = R->rmax * R->rmax;
     if (!(rho2 < out2)) return 0;
     const REAL ta = -SQRT(out2 - rho2);
     REAL tm = (REAL)0;
     REAL fm = FN(surf_)(R, o);
     if (!(fm < (REAL)0)) {
-        /* foot of the perpendicular is outside: golden-section search for an
-         * interior point along the bounding chord */
-        const REAL g = (REAL)0.6180339887498949;
-        REAL lo = ta, hi = -ta;
-        REAL c1 = hi - g * (hi - lo), c2 = lo + g * (hi - lo);
-        REAL f1 = FN(surf_)(R, FN(add_)(o, FN(scl_)(R->ki, c1)));
-        REAL f2 = FN(surf_)(R, FN(add_)(o, FN(scl_)(R->ki, c2)));
+        /* foot of the perpendicular is outside: minimize f along the line by the
+         * secant method on f'(t) = grad f . ki from the foot (the inside of a convex
+         * body along a line is the single interval around that minimum); any
+         * iterate with f < 0 is an interior point, convergence with f >= 0 is a miss */
+        REAL g0 = FN(dot_)(FN(grad_)(R, o), R->ki);
+        REAL t0 = (REAL)0;
+        REAL t1 = -g0 * SQRT(rho2);
         int found = 0;
-        for (int it = 0; it < 28 && !found; ++it) {
-            if (f1 < (REAL)0) { tm = c1; found = 1; break; }
-            if (f2 < (REAL)0) { tm = c2; found = 1; break; }
-            if (f1 < f2) {
-                hi = c2; c2 = c1; f2 = f1;
-                c1 = hi - g * (hi - lo);
-                f1 = FN(surf_)(R, FN(add_)(o, FN(scl_)(R->ki, c1)));
-            } else {
-                lo = c1; c1 = c2; f1 = f2;
-                c2 = lo + g * (hi - lo);
-                f2 = FN(surf_)(R, FN(add_)(o, FN(scl_)(R->ki, c2)));
-            }
+        for (int it = 0; it < 8; ++it) {
+            if (t1 < ta) t1 = ta;
+            if (t1 > -ta) t1 = -ta;
+            const V p1 = FN(add_)(o, FN(scl_)(R->ki, t1));
+            const REAL f1 = FN(surf_)(R, p1);
+            if (f1 < (REAL)0) { tm = t1; found = 1; break; }
+            const REAL g1 = FN(dot_)(FN(grad_)(R, p1), R->ki);
+            if (g1 == g0) break;
+            const REAL t2 = t1 - g1 * (t1 - t0) / (g1 - g0);
+            t0 = t1; g0 = g1;
+            if (FABS(t2 - t1) <= MIN_STEP) break;
+            t1 = t2;
         }
         if (!found) return 0;
     }
@@ -167,7 +168,7 @@ static int FN(exit_)(const FN(rec_)* R, V p, V d, REAL* tout)
         *tout = -b + SQRT(disc);
         return 1;
     }
-    const REAL cb = FN(dot_)(p, p) - R->outer * R->outer;
+    const REAL cb = FN(dot_)(p, p) - R->rmax * R->rmax;
     REAL disc = b * b - cb;
     if (disc < (REAL)0) disc = (REAL)0;
     const REAL tb = -b + SQRT(disc);
@@ -205,9 +206,9 @@ static FN(split_) FN(split_fn_)(const FN(ray_)* in, V n, REAL ni, REAL nt)
     V s = FN(crs_)(d, n);
     const REAL s2 = FN(dot_)(s, s);
     const REAL sine = SQRT(s2);
-    if (s2 > TINY_S2) s = FN(div_)(s, sine);
+    if (s2 > TINY_S2) s = FN(scl_)(s, (REAL)1 / sine);
     else s = FN(unit_)(FN(sub_)(in->e0, FN(scl_)(d, FN(dot_)(d, in->e0))));
-    const V pin = FN(unit_)(FN(crs_)(d, s));
+    const V pin = FN(crs_)(d, s); /* d, s unit and orthogonal */
     const REAL a = FN(dot_)(in->e0, s), b = FN(dot_)(in->e1, s);
     const REAL cc = FN(dot_)(in->e0, pin), dd = FN(dot_)(in->e1, pin);
     J j;
@@ -219,7 +220,7 @@ static FN(split_) FN(split_fn_)(const FN(ray_)* in, V n, REAL ni, REAL nt)
     const REAL st = eta * sine;
     o.r.dir = FN(unit_)(FN(add_)(d, FN(scl_)(n, (REAL)2 * ci)));
     o.r.e0 = s;
-    o.r.e1 = FN(unit_)(FN(crs_)(o.r.dir, s));
+    o.r.e1 = FN(crs_)(o.r.dir, s);
     C rs, rp;
     if (st > (REAL)1) {
         const REAL beta = SQRT((st - (REAL)1) * (st + (REAL)1));
@@ -236,7 +237,7 @@ static FN(split_) FN(split_fn_)(const FN(ray_)* in, V n, REAL ni, REAL nt)
         const REAL ts = num / ds, tp = num / dp;
         o.t.dir = FN(unit_)(FN(sub_)(FN(scl_)(FN(add_)(d, FN(scl_)(n, ci)), eta), FN(scl_)(n, ct)));
         o.t.e0 = s;
-        o.t.e1 = FN(unit_)(FN(crs_)(o.t.dir, s));
+        o.t.e1 = FN(crs_)(o.t.dir, s);
         o.t.j.c0x = FN(cr_)(j.c0x, ts); o.t.j.c0y = FN(cr_)(j.c0y, tp);
         o.t.j.c1x = FN(cr_)(j.c1x, ts); o.t.j.c1y = FN(cr_)(j.c1y, tp);
         o.has_t = 1;

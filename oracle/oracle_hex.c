@@ -476,17 +476,10 @@ int orc_drop_frame(const double k_in[3], orc_frame* f)
     return 0;
 }
 
-/* phase_math.hpp:37-81 */
-orc_bin orc_phase_bin(const double* ue, uint32_t nt, uint32_t np, const orc_frame* g, const double dir[3])
+/* phase_math.hpp:37-81: everything after the unit-length checks. */
+static orc_bin bin_core(const double* ue, uint32_t nt, uint32_t np, double x, double y, double z, double n)
 {
     orc_bin b; memset(&b, 0, sizeof b);
-    const v3 d = mk(dir[0], dir[1], dir[2]);
-    if (!ue || !nt || !np || !is_unit(d)) return b;
-    const v3 e0 = mk(g->e0[0], g->e0[1], g->e0[2]), e1 = mk(g->e1[0], g->e1[1], g->e1[2]);
-    const v3 ki = mk(g->ki[0], g->ki[1], g->ki[2]);
-    const double x = dot(e0, d), y = dot(e1, d), z = dot(ki, d);
-    const double n = sqrt(x * x + y * y + z * z);
-    if (!(n > 0) || !fin(n) || fabs(n - 1) > 512 * DBL_EPSILON) return b;
     const double mu = z / n, s = hypot(x, y) / n;
     b.mu = mu;
     if (s <= 256 * DBL_EPSILON) { b.pole = mu >= 0 ? 1 : -1; b.valid = 1; return b; }
@@ -519,6 +512,32 @@ orc_bin orc_phase_bin(const double* ue, uint32_t nt, uint32_t np, const orc_fram
     b.index = (uint64_t)lo * np + j;
     b.valid = 1;
     return b;
+}
+
+orc_bin orc_phase_bin(const double* ue, uint32_t nt, uint32_t np, const orc_frame* g, const double dir[3])
+{
+    orc_bin b; memset(&b, 0, sizeof b);
+    const v3 d = mk(dir[0], dir[1], dir[2]);
+    if (!ue || !nt || !np || !is_unit(d)) return b;
+    const v3 e0 = mk(g->e0[0], g->e0[1], g->e0[2]), e1 = mk(g->e1[0], g->e1[1], g->e1[2]);
+    const v3 ki = mk(g->ki[0], g->ki[1], g->ki[2]);
+    const double x = dot(e0, d), y = dot(e1, d), z = dot(ki, d);
+    const double n = sqrt(x * x + y * y + z * z);
+    if (!(n > 0) || !fin(n) || fabs(n - 1) > 512 * DBL_EPSILON) return b;
+    return bin_core(ue, nt, np, x, y, z, n);
+}
+
+orc_bin orc_phase_bin_raw(const double* ue, uint32_t nt, uint32_t np, const orc_frame* g, const double dir[3])
+{
+    orc_bin b; memset(&b, 0, sizeof b);
+    const v3 d = mk(dir[0], dir[1], dir[2]);
+    if (!ue || !nt || !np || !vfin(d)) return b;
+    const v3 e0 = mk(g->e0[0], g->e0[1], g->e0[2]), e1 = mk(g->e1[0], g->e1[1], g->e1[2]);
+    const v3 ki = mk(g->ki[0], g->ki[1], g->ki[2]);
+    const double x = dot(e0, d), y = dot(e1, d), z = dot(ki, d);
+    const double n = sqrt(x * x + y * y + z * z);
+    if (!(n > 0) || !fin(n)) return b;
+    return bin_core(ue, nt, np, x, y, z, n);
 }
 
 uint32_t orc_quantum_bits(uint64_t rays)

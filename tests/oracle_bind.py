@@ -62,7 +62,7 @@ class RefCdfReport(C.Structure):
 
 
 class DropRecord(C.Structure):
-    _fields_ = [("coeffs", f32 * 8), ("inner", f32), ("outer", f32), ("ki", f32 * 3), ("e0", f32 * 3),
+    _fields_ = [("coeffs", f32 * 8), ("inner", f32), ("outer", f32), ("rmax", f32), ("ki", f32 * 3), ("e0", f32 * 3),
                 ("e1", f32 * 3), ("n_int", f32), ("n_ext", f32), ("half_extent", f32), ("grid_w", u32),
                 ("grid_h", u32), ("max_hits", u32), ("sphere", u32), ("tail", f32)]
 
