@@ -341,7 +341,7 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
     }
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
     icephase_trace_drop(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindDrop, true>(recs, map, queue);
@@ -351,7 +351,7 @@ extern "C" __global__ void __launch_bounds__(256)
 {
     trace_loop<kKindHex, true>(recs, map, queue);
 }
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
     icephase_trace_drop_wide(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindDrop, false>(recs, map, queue);
