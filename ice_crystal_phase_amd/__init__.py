@@ -6,11 +6,13 @@ with FP32/FP64 Fresnel-Jones algebra, bin the outgoing flux into fixed-point
 kernels behind the C-ABI in include/icephase.h (libicephase.so).
 """
 from .icephase import (AuditError, Batch, CdfPolicy, CdfReport, Context, HistStats, IcePhaseError, Record,
-                       RUN_CDF, RUN_GRAPH, RUN_NO_AUDIT, RUN_TRACE, SHAPE_DROP, SHAPE_HEX, SHAPE_SPHERE,
-                       STATUS_NAMES, drop_incident, drop_record, hex_record, lib, record_rays, sphere_record,
-                       u_edges, write_record)
+                       REC_DIFFRACTION, RUN_CDF, RUN_GRAPH, RUN_NO_AUDIT, RUN_TRACE, SHAPE_DROP, SHAPE_HEX,
+                       SHAPE_SPHERE, STATUS_NAMES, drop_incident, drop_projected_area, drop_record, drop_shape,
+                       hex_projected_area, hex_record, lib, quaternion_rotation, record_rays, sphere_record,
+                       u_edges, water_index, water_optics, write_record)
 
 __all__ = ["AuditError", "Batch", "CdfPolicy", "CdfReport", "Context", "HistStats", "IcePhaseError", "Record",
-           "RUN_CDF", "RUN_GRAPH", "RUN_NO_AUDIT", "RUN_TRACE", "SHAPE_DROP", "SHAPE_HEX", "SHAPE_SPHERE",
-           "STATUS_NAMES", "drop_incident", "drop_record", "hex_record", "lib", "record_rays", "sphere_record",
-           "u_edges", "write_record"]
+           "REC_DIFFRACTION", "RUN_CDF", "RUN_GRAPH", "RUN_NO_AUDIT", "RUN_TRACE", "SHAPE_DROP", "SHAPE_HEX",
+           "SHAPE_SPHERE", "STATUS_NAMES", "drop_incident", "drop_projected_area", "drop_record", "drop_shape",
+           "hex_projected_area", "hex_record", "lib", "quaternion_rotation", "record_rays", "sphere_record",
+           "u_edges", "water_index", "water_optics", "write_record"]

@@ -7,6 +7,9 @@
 #include <filesystem>
 #include <fstream>
 #include <sstream>
+#include <algorithm>
+#include <cfloat>
+#include <utility>
 
 namespace icephase {
 
@@ -48,28 +51,95 @@ bool drop_frame(d3 k_in, BinFrame& f)
 
 bool drop_convex(const double c[8], std::string& err)
 {
-    const int n = 16384;
-    double min_r = 1e300, min_k = 1e300;
-    for (int s = 0; s <= n; ++s) {
-        const double chi = kPi * s / n;
-        double r = 1 + c[0], r1 = 0, r2 = 0;
-        for (int m = 1; m < 8; ++m) {
-            r += c[m] * std::cos(m * chi);
-            r1 -= m * c[m] * std::sin(m * chi);
-            r2 -= m * m * c[m] * std::cos(m * chi);
+    uint32_t cells = 0;
+    return drop_convexity(c, cells, err);
+}
+
+bool rotation_valid(const double m[9])
+{
+    // columns are the images of the body axes (Rotation::valid,
+    // hex_phase/include/ice_crystal/phase_math.hpp:12-18)
+    const d3 x = D3(m[0], m[3], m[6]), y = D3(m[1], m[4], m[7]), z = D3(m[2], m[5], m[8]);
+    return is_unit64(x) && is_unit64(y) && is_unit64(z) && std::fabs(dot(x, y)) <= 128 * DBL_EPSILON &&
+           max_abs(cross(x, y) - z) <= 256 * DBL_EPSILON;
+}
+
+// Aperture polygon of the prism seen along c.k, in the trace basis (e0, e1) of
+// c.k, and the lobe's sampling parameters.
+static bool setup_lobe(const hex::Config& c, DevRecord& R, double wavelength_nm, std::string& err)
+{
+    if (!(std::isfinite(wavelength_nm) && wavelength_nm > 0)) {
+        err = "Diffraction needs a finite positive wavelength.";
+        return false;
+    }
+    Lobe& L = R.lobe;
+    std::vector<std::pair<double, double>> pts;
+    for (int f = 0; f < 6; ++f)
+        for (int s = -1; s <= 1; s += 2) {
+            const d3 v = hex::Prism::vertex(f);
+            const d3 p = D3(c.prism.R * v.x, 0.5 * s * c.prism.L, c.prism.R * v.z);
+            pts.emplace_back(dot(p, c.f.e0), dot(p, c.f.e1));
         }
-        min_r = std::fmin(min_r, r);
-        min_k = std::fmin(min_k, r * r + 2 * r1 * r1 - r * r2);
+    // convex hull (monotone chain), counter-clockwise, collinear points dropped
+    std::sort(pts.begin(), pts.end());
+    const double scale = c.prism.R > c.prism.L ? c.prism.R : c.prism.L;
+    auto turn = [&](const std::pair<double, double>& o, const std::pair<double, double>& a,
+                    const std::pair<double, double>& b) {
+        return (a.first - o.first) * (b.second - o.second) - (a.second - o.second) * (b.first - o.first);
+    };
+    std::vector<std::pair<double, double>> hull(2 * pts.size());
+    size_t k = 0;
+    for (size_t i = 0; i < pts.size(); ++i) {
+        while (k >= 2 && turn(hull[k - 2], hull[k - 1], pts[i]) <= 1e-12 * scale * scale) --k;
+        hull[k++] = pts[i];
     }
-    if (!(min_r > 0)) {
-        err = "Drop radius r(mu) must stay positive.";
+    for (size_t i = pts.size() - 1, t = k + 1; i > 0; --i) {
+        while (k >= t && turn(hull[k - 2], hull[k - 1], pts[i - 1]) <= 1e-12 * scale * scale) --k;
+        hull[k++] = pts[i - 1];
+    }
+    hull.resize(k > 1 ? k - 1 : k);
+    if (hull.size() < 3 || hull.size() > 8) {
+        err = "Projected aperture polygon is degenerate.";
         return false;
     }
-    if (!(min_k > 1e-9)) {
-        err = "Drop shape is not convex (meridian curvature changes sign); the analytic convex intersection "
-              "requires a convex particle.";
+    L.nv = (uint32_t)hull.size();
+    double a2 = 0, dmax = 0;
+    for (uint32_t i = 0; i < L.nv; ++i) {
+        L.vx[i] = hull[i].first;
+        L.vy[i] = hull[i].second;
+        const auto& q = hull[(i + 1) % L.nv];
+        a2 += hull[i].first * q.second - q.first * hull[i].second;
+        for (uint32_t j = 0; j < L.nv; ++j)
+            dmax = std::fmax(dmax, std::hypot(hull[i].first - hull[j].first, hull[i].second - hull[j].second));
+    }
+    L.area = 0.5 * a2;
+    L.dmax = dmax;
+    if (!(L.area > 0) || std::fabs(L.area / c.area - 1) > 1e-9) {
+        err = "Projected aperture polygon disagrees with the face-sum projected area.";
         return false;
     }
+    const double lambda = wavelength_nm * 1e-6;  // mm
+    L.k0 = 2 * kPi / lambda;
+    L.inv_lambda2 = 1 / (lambda * lambda);
+    // aperture basis in binning-frame (world) coordinates
+    if (R.rotated) {
+        const double* m = R.rot;
+        auto rot = [&](d3 v) {
+            return D3((m[0] * v.x + m[1] * v.y) + m[2] * v.z, (m[3] * v.x + m[4] * v.y) + m[5] * v.z,
+                      (m[6] * v.x + m[7] * v.y) + m[8] * v.z);
+        };
+        L.p1 = rot(c.f.e0);
+        L.p2 = rot(c.f.e1);
+    } else {
+        L.p1 = c.f.e0;
+        L.p2 = c.f.e1;
+    }
+    // rows within min(0.5 rad, 200 fringes) of the forward direction
+    const double cut = std::fmin(0.5, 200 * lambda / dmax);
+    uint32_t rows = (uint32_t)std::ceil(cut / (kPi / R.nt));
+    rows = rows < 1 ? 1 : (rows > R.nt ? R.nt : rows);
+    L.rows = rows;
+    R.lobe_on = 1;
     return true;
 }
 
@@ -82,6 +152,38 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
                     std::string& err)
 {
     std::memset(&R, 0, sizeof R);
+    R.qfactor = 1.0;
+    R.owns = in.histogram_of == 0 ? 1u : 0u;
+    if (in.flags & ~(uint32_t)IPH_REC_DIFFRACTION) {
+        err = "Unknown record flags.";
+        return false;
+    }
+    {
+        bool any = false;
+        for (int k = 0; k < 9; ++k) {
+            if (!std::isfinite(in.rotation[k])) {
+                err = "Rotation must be finite.";
+                return false;
+            }
+            any = any || in.rotation[k] != 0;
+        }
+        if (any) {
+            if (in.shape != IPH_SHAPE_HEX) {
+                err = "Orientation rotations apply to hex records only.";
+                return false;
+            }
+            if (!rotation_valid(in.rotation)) {
+                err = "Rotation is not orthonormal and right-handed.";
+                return false;
+            }
+            std::memcpy(R.rot, in.rotation, sizeof R.rot);
+            R.rotated = 1;
+        }
+    }
+    if (in.shape != IPH_SHAPE_HEX && (in.histogram_of != 0 || in.weight != 0 || in.flags != 0)) {
+        err = "Orientation nodes, node weights and diffraction apply to hex records only.";
+        return false;
+    }
     if (in.theta_count == 0 || in.phi_count == 0 || (uint64_t)in.theta_count * in.phi_count > 0xffffffffull ||
         in.phi_count > 0x7fffffffu) {
         err = "Unsupported phase grid dimensions.";
@@ -121,9 +223,28 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
             err = "Invalid hex prism geometry.";
             return false;
         }
-        if (!unit_of(k_in, c.k) || !hex::frame_of(c.k, c.f)) {
+        d3 k_ens;
+        if (!unit_of(k_in, k_ens)) {
             err = "Incident direction must be finite and nonzero.";
             return false;
+        }
+        if (R.rotated) {
+            // trace in the body frame: normalize(R^T k)
+            // (hex_phase/src/phase_generate.cpp:44-49, phase_math.hpp:11)
+            const double* m = R.rot;
+            const d3 kb = D3((m[0] * k_ens.x + m[3] * k_ens.y) + m[6] * k_ens.z,
+                             (m[1] * k_ens.x + m[4] * k_ens.y) + m[7] * k_ens.z,
+                             (m[2] * k_ens.x + m[5] * k_ens.y) + m[8] * k_ens.z);
+            if (!unit_of(kb, c.k) || !hex::frame_of(c.k, c.f)) {
+                err = "Incident direction must be finite and nonzero.";
+                return false;
+            }
+        } else {
+            c.k = k_ens;
+            if (!hex::frame_of(c.k, c.f)) {
+                err = "Incident direction must be finite and nonzero.";
+                return false;
+            }
         }
         c.ni = in.exterior_index;
         c.nt = in.interior_index;
@@ -135,11 +256,35 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
             err = "Per-incident-sample area underflows FP64.";
             return false;
         }
-        R.frame.ki = c.k;
-        R.frame.e0 = c.f.e0;
-        R.frame.e1 = c.f.e1;
-        weight = c.area / double(in.rays);
-        area = c.area;
+        if (R.rotated) {
+            // bin in the ensemble frame (make_frame of the ensemble direction)
+            hex::Basis fe;
+            if (!hex::frame_of(k_ens, fe)) {
+                err = "Incident direction must be finite and nonzero.";
+                return false;
+            }
+            R.frame.ki = k_ens;
+            R.frame.e0 = fe.e0;
+            R.frame.e1 = fe.e1;
+        } else {
+            R.frame.ki = c.k;
+            R.frame.e0 = c.f.e0;
+            R.frame.e1 = c.f.e1;
+        }
+        const double node_weight = in.weight == 0 ? 1.0 : in.weight;
+        if (!(std::isfinite(node_weight) && node_weight > 0)) {
+            err = "Orientation weight must be finite and positive.";
+            return false;
+        }
+        weight = node_weight * c.area / double(in.rays);
+        area = node_weight * c.area;
+        if (!(weight > 0) || !std::isfinite(weight)) {
+            err = "Orientation area weight over/underflow.";
+            return false;
+        }
+        if (in.flags & IPH_REC_DIFFRACTION) {
+            if (!setup_lobe(c, R, in.wavelength_nm, err)) return false;
+        }
     } else if (in.shape == IPH_SHAPE_SPHERE || in.shape == IPH_SHAPE_DROP) {
         R.kind = kKindDrop;
         if (in.grid_w == 0 || in.grid_h == 0 || (uint64_t)in.grid_w * in.grid_h > (1ull << 40) ||
@@ -213,7 +358,15 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
         }
         const double cell = (2.0 * (double)c.h / in.grid_w) * (2.0 * (double)c.h / in.grid_h);
         weight = cell * in.radius_mm * in.radius_mm;
-        area = sphere ? kPi * in.radius_mm * in.radius_mm : 0.0;
+        {
+            // projected area of the traced shape along the traced direction
+            double cd[8];
+            for (int m = 0; m < 8; ++m) cd[m] = (double)c.shape.c[m];
+            const double kd[3] = {(double)c.ki.x, (double)c.ki.y, (double)c.ki.z};
+            AreaResult ar;
+            if (!drop_projected_area(cd, in.radius_mm, kd, ar, err)) return false;
+            area = ar.area_mm2;
+        }
     } else {
         err = "Unknown shape.";
         return false;

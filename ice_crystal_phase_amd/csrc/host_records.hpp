@@ -3,6 +3,7 @@
 #pragma once
 
 #include "kernels.hpp"
+#include "materials.hpp"
 
 #include <cstdint>
 #include <string>
@@ -20,11 +21,12 @@ bool make_u_edges(uint32_t n, std::vector<double>& out, std::string& err);
 // |k.z| >= 0.9 (then +x), e1 = k x e0 (include/rainbow/vec3.hpp:56-65, FP64).
 bool drop_frame(d3 k, BinFrame& f);
 
-// Convexity of the drop meridian r(chi) = 1 + sum c_n cos(n chi): the polar
-// curvature numerator r^2 + 2 r'^2 - r r'' must stay positive on [0, pi]
-// (checked on a dense grid; the reference's interval check is
-// src/projected_area.cpp:50-87).
+// Certified strict convexity of the drop (materials.hpp drop_convexity; the
+// reference's interval check is src/projected_area.cpp:50-87).
 bool drop_convex(const double c[8], std::string& err);
+
+// Proper rotation check of a row-major body -> ensemble matrix.
+bool rotation_valid(const double m[9]);
 
 // Fills everything but the device pointers. tile_rays: requested tile length.
 bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& out, double& ray_weight_mm2,

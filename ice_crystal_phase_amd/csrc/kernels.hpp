@@ -18,7 +18,24 @@ enum StatSlot : uint32_t {
     kStatEscaped = 14,
     kStatUnresolved = 15,
     kStatInternalHits = 16,
+    kStatDiffraction = 17,   // quanta of the Fraunhofer lobe
     kStatCount = 24,
+};
+
+// Fraunhofer diffraction lobe of a hex record's projected aperture. The
+// aperture is the convex polygon (counter-clockwise, up to 8 vertices) the
+// prism projects onto the plane normal to the incident direction, in the
+// in-plane basis (p1, p2) expressed in the binning frame's coordinates.
+struct Lobe {
+    double vx[8], vy[8];  // polygon vertices, mm
+    d3 p1, p2;            // aperture basis (world coordinates of the binning frame)
+    double k0;            // 2 pi / lambda, 1/mm
+    double inv_lambda2;   // 1 / lambda^2, 1/mm^2
+    double area;          // polygon area, mm^2
+    double dmax;          // largest vertex distance, mm (sub-panel sizing)
+    uint32_t nv, rows;    // vertex count; leading theta rows computed
+    double* mass;         // rows*np FP64 cell masses (mm^2 before scaling)
+    double* partial;      // per-workgroup partial sums of mass, then total and captured fraction
 };
 
 enum RecordKind : uint32_t { kKindDrop = 0, kKindHex = 1 };
@@ -32,8 +49,16 @@ struct DevRecord {
     unsigned long long* stats;  // kStatCount counters
     const double* edges;        // nt+1 u edges
     BinFrame frame;             // FP64 binning frame (e0, e1, ki)
+    // Quanta of an emission: rint(power * qfactor * 2^qbits). qfactor = 1 for a
+    // record that owns its histogram alone; orientation nodes sharing one
+    // histogram carry their sample weight relative to the largest node's.
+    double qfactor;
+    double rot[9];              // hex: body -> ensemble rotation (row-major) when rotated
+    uint32_t rotated, lobe_on;
+    uint32_t owns;              // 1: this record's histogram (zeroed with it); 0: a node adding into another's
     drop::Config drop;
     hex::Config hex;
+    Lobe lobe;
 };
 
 // Tiles are implicit: the records of one kind are listed in `recs_of_kind`,

@@ -23,6 +23,8 @@ hipError_t launch_bin_dirs(const BinFrame& g, uint32_t nt, uint32_t np, const do
 hipError_t launch_cdf(const CdfJob* d_jobs, uint32_t njobs, uint32_t max_onp, bool any_coarse, uint32_t reduce_blocks,
                       hipStream_t s);
 uint32_t cdf_partial_blocks(uint32_t reduce_blocks, uint32_t max_onp);
+hipError_t launch_lobe(const DevRecord* d_recs, uint32_t rec, uint32_t rows, uint32_t np, hipStream_t s);
+uint32_t lobe_blocks(uint32_t rows, uint32_t np);
 }  // namespace icephase
 
 using namespace icephase;
@@ -169,7 +171,13 @@ struct iph_batch {
     DevBuf<double> d_cdf;  // phi, theta, cols, scalars, partials, coarse
     std::vector<size_t> off_phi, off_theta;
     std::vector<size_t> off_bins;
-    std::vector<CdfJob> jobs;
+    std::vector<CdfJob> jobs;           // one per histogram owner
+    std::vector<int32_t> job_of;        // record -> job index, -1 for orientation nodes
+    std::vector<uint32_t> owner;        // record -> owner record
+    std::vector<uint64_t> group_rays;   // owner: rays of every record adding into it
+    std::vector<double> group_area;     // owner: sum of node weight * projected area
+    DevBuf<double> d_lobe;              // lobe cell masses and partial sums
+    std::vector<uint32_t> lobe_recs;    // records with a diffraction lobe
     TileMap map[2]{};
     bool wide[2] = {false, false};
     uint32_t max_onp = 1, reduce_blocks = 512;
@@ -199,6 +207,8 @@ uint32_t iph_struct_size(int which)
     case 1: return (uint32_t)sizeof(iph_hist_stats);
     case 2: return (uint32_t)sizeof(iph_cdf_report);
     case 3: return (uint32_t)sizeof(iph_cdf_policy);
+    case 4: return (uint32_t)sizeof(iph_water_state);
+    case 5: return (uint32_t)sizeof(iph_area);
     default: return 0;
     }
 }
@@ -304,17 +314,51 @@ int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const 
     size_t bins_total = 0, edges_total = 0;
     b->off_bins.resize(nrec);
     std::vector<size_t> off_edges(nrec), off_cedges(nrec);
+    b->owner.resize(nrec);
+    b->group_rays.assign(nrec, 0);
+    b->group_area.assign(nrec, 0.0);
     for (uint32_t r = 0; r < nrec; ++r) {
         if (!prepare_record(recs[r], 0, b->host[r], b->weight[r], b->area[r], err))
             return fail(ctx, IPH_E_INVALID, "record " + std::to_string(r) + ": " + err);
+        b->owner[r] = r;
+        if (recs[r].histogram_of) {
+            const uint32_t o = recs[r].histogram_of - 1;
+            if (o >= r || recs[o].histogram_of != 0 || recs[o].shape != IPH_SHAPE_HEX || recs[r].shape != IPH_SHAPE_HEX ||
+                recs[o].theta_count != recs[r].theta_count || recs[o].phi_count != recs[r].phi_count ||
+                std::memcmp(recs[o].incident, recs[r].incident, sizeof recs[r].incident) != 0 ||
+                !b->host[o].rotated || !b->host[r].rotated)
+                return fail(ctx, IPH_E_INVALID,
+                            "record " + std::to_string(r) +
+                                ": an orientation node must follow its owner (a rotated hex record with the same grid "
+                                "and ensemble incident direction).");
+            b->owner[r] = o;
+        }
+        b->group_rays[b->owner[r]] += b->host[r].rays;
+        b->group_area[b->owner[r]] += b->area[r];
+    }
+    // composition: every member quantizes with the group's exponent, weighted
+    // relative to the group's largest sample weight
+    {
+        std::vector<double> max_scale(nrec, 0.0);
+        for (uint32_t r = 0; r < nrec; ++r) max_scale[b->owner[r]] = std::max(max_scale[b->owner[r]], b->weight[r]);
+        for (uint32_t r = 0; r < nrec; ++r) {
+            const uint32_t o = b->owner[r];
+            if (o == r && b->group_rays[r] == b->host[r].rays) continue;  // alone: unchanged
+            b->host[r].qfactor = b->weight[r] / max_scale[o];
+            b->host[r].qbits = quantum_bits(b->group_rays[o]);
+        }
+        for (uint32_t r = 0; r < nrec; ++r)
+            if (b->owner[r] == r) b->weight[r] = max_scale[r];
+    }
+    for (uint32_t r = 0; r < nrec; ++r) {
         if (!resolve_policy(policy, recs[r].theta_count, recs[r].phi_count, b->cdf[r], err))
             return fail(ctx, IPH_E_INVALID, "record " + std::to_string(r) + ": " + err);
         if (!make_u_edges(recs[r].theta_count, b->edges_host[r], err)) return fail(ctx, IPH_E_INVALID, err);
         const uint32_t dt = recs[r].theta_count / b->cdf[r].ont;
         b->cedges_host[r].resize(b->cdf[r].ont + 1);
         for (uint32_t i = 0; i <= b->cdf[r].ont; ++i) b->cedges_host[r][i] = b->edges_host[r][(size_t)i * dt];
-        b->off_bins[r] = bins_total;
-        bins_total += (size_t)recs[r].theta_count * recs[r].phi_count;
+        b->off_bins[r] = b->owner[r] == r ? bins_total : b->off_bins[b->owner[r]];
+        if (b->owner[r] == r) bins_total += (size_t)recs[r].theta_count * recs[r].phi_count;
         off_edges[r] = edges_total;
         edges_total += b->edges_host[r].size();
         off_cedges[r] = edges_total;
@@ -336,8 +380,26 @@ int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const 
     }
     for (uint32_t r = 0; r < nrec; ++r) {
         b->host[r].bins = b->d_bins.p + b->off_bins[r];
-        b->host[r].stats = b->d_stats.p + (size_t)r * kStatCount;
+        b->host[r].stats = b->d_stats.p + (size_t)b->owner[r] * kStatCount;
         b->host[r].edges = b->d_edges.p + off_edges[r];
+    }
+    // diffraction lobes: cell masses + partial sums (+ total, captured fraction)
+    {
+        size_t lobe_total = 0;
+        std::vector<size_t> off_lobe(nrec, 0);
+        for (uint32_t r = 0; r < nrec; ++r)
+            if (b->host[r].lobe_on) {
+                b->lobe_recs.push_back(r);
+                off_lobe[r] = lobe_total;
+                lobe_total += (size_t)b->host[r].lobe.rows * b->host[r].np +
+                              lobe_blocks(b->host[r].lobe.rows, b->host[r].np) + 2;
+            }
+        IPH_HIP(ctx, b->d_lobe.alloc(lobe_total));
+        for (uint32_t r : b->lobe_recs) {
+            Lobe& L = b->host[r].lobe;
+            L.mass = b->d_lobe.p + off_lobe[r];
+            L.partial = L.mass + (size_t)L.rows * b->host[r].np;
+        }
     }
     // CDF buffers
     size_t cdf_total = 0;
@@ -347,6 +409,7 @@ int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const 
     for (uint32_t r = 0; r < nrec; ++r) b->max_onp = std::max(b->max_onp, b->cdf[r].onp);
     const uint32_t pblocks = cdf_partial_blocks(b->reduce_blocks, b->max_onp);
     for (uint32_t r = 0; r < nrec; ++r) {
+        if (b->owner[r] != r) continue;
         const CdfSetup& s = b->cdf[r];
         const bool coarse = s.ont != recs[r].theta_count || s.onp != recs[r].phi_count;
         b->any_coarse = b->any_coarse || coarse;
@@ -358,11 +421,14 @@ int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const 
         off_coarse[r] = cdf_total; cdf_total += coarse ? (size_t)s.ont * s.onp : 0;
     }
     IPH_HIP(ctx, b->d_cdf.alloc(cdf_total));
-    b->jobs.resize(nrec);
+    b->job_of.assign(nrec, -1);
     for (uint32_t r = 0; r < nrec; ++r) {
+        if (b->owner[r] != r) continue;
         const CdfSetup& s = b->cdf[r];
         const bool coarse = s.ont != recs[r].theta_count || s.onp != recs[r].phi_count;
-        CdfJob& J = b->jobs[r];
+        b->job_of[r] = (int32_t)b->jobs.size();
+        b->jobs.emplace_back();
+        CdfJob& J = b->jobs.back();
         J.nt = recs[r].theta_count; J.np = recs[r].phi_count; J.ont = s.ont; J.onp = s.onp;
         J.bins = b->host[r].bins; J.mass = nullptr; J.stats = b->host[r].stats;
         J.edges = b->d_edges.p + off_edges[r]; J.cedges = b->d_edges.p + off_cedges[r];
@@ -371,8 +437,8 @@ int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const 
         J.cols = b->d_cdf.p + off_cols[r]; J.scalars = b->d_cdf.p + off_sc[r];
         J.partials = b->d_cdf.p + off_part[r];
     }
-    IPH_HIP(ctx, b->d_jobs.alloc(nrec));
-    IPH_HIP(ctx, hipMemcpy(b->d_jobs.p, b->jobs.data(), nrec * sizeof(CdfJob), hipMemcpyHostToDevice));
+    IPH_HIP(ctx, b->d_jobs.alloc(b->jobs.size()));
+    IPH_HIP(ctx, hipMemcpy(b->d_jobs.p, b->jobs.data(), b->jobs.size() * sizeof(CdfJob), hipMemcpyHostToDevice));
     for (auto& e : b->ev) IPH_HIP(ctx, hipEventCreate(&e));
     iph_batch* raw = b.release();
     const int rc = iph_batch_set_schedule(raw, 0, 0);
@@ -457,9 +523,12 @@ static int enqueue(iph_batch* b, uint32_t flags, bool timed)
         for (int k = 0; k < 2; ++k)
             if (b->map[k].total)
                 IPH_HIP(ctx, launch_trace(k, b->wide[k], b->d_recs.p, b->map[k], b->d_queue.p + k, b->blocks[k], 256, s));
+        for (uint32_t r : b->lobe_recs)
+            IPH_HIP(ctx, launch_lobe(b->d_recs.p, r, b->host[r].lobe.rows, b->host[r].np, s));
     }
     if (timed) IPH_HIP(ctx, hipEventRecord(b->ev[1], s));
-    if (flags & IPH_RUN_CDF) IPH_HIP(ctx, launch_cdf(b->d_jobs.p, b->nrec, b->max_onp, b->any_coarse, b->reduce_blocks, s));
+    if (flags & IPH_RUN_CDF)
+        IPH_HIP(ctx, launch_cdf(b->d_jobs.p, (uint32_t)b->jobs.size(), b->max_onp, b->any_coarse, b->reduce_blocks, s));
     if (timed) IPH_HIP(ctx, hipEventRecord(b->ev[2], s));
     return IPH_OK;
 }
@@ -468,9 +537,11 @@ static int check_cdf(iph_batch* b)
 {
     iph_ctx* ctx = b->ctx;
     for (uint32_t r = 0; r < b->nrec; ++r) {
+        if (b->job_of[r] < 0) continue;
+        const CdfJob& J = b->jobs[b->job_of[r]];
         double sc[kCdfScalars];
-        IPH_HIP(ctx, hipMemcpy(sc, b->jobs[r].scalars, sizeof sc, hipMemcpyDeviceToHost));
-        const bool coarse = b->jobs[r].coarse != nullptr;
+        IPH_HIP(ctx, hipMemcpy(sc, J.scalars, sizeof sc, hipMemcpyDeviceToHost));
+        const bool coarse = J.coarse != nullptr;
         const iph_cdf_report rep = make_report(sc, b->cdf[r], coarse);
         if (rep.status)
             return fail(ctx, IPH_E_AUDIT, "record " + std::to_string(r) + ": " + report_reason(rep.status));
@@ -539,9 +610,17 @@ int iph_batch_timing(const iph_batch* b, double* ms)
     return IPH_OK;
 }
 
+static int node_error(iph_batch* b, uint32_t rec)
+{
+    return fail(b->ctx, IPH_E_INVALID,
+                "record " + std::to_string(rec) + " is an orientation node of record " + std::to_string(b->owner[rec]) +
+                    "; read the owner's histogram.");
+}
+
 int iph_batch_stats(iph_batch* b, uint32_t rec, iph_hist_stats* out)
 {
     if (!b || !out || rec >= b->nrec) return fail(b ? b->ctx : nullptr, IPH_E_INVALID, "Invalid record index.");
+    if (b->owner[rec] != rec) return node_error(b, rec);
     iph_ctx* ctx = b->ctx;
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     unsigned long long st[kStatCount];
@@ -556,16 +635,29 @@ int iph_batch_stats(iph_batch* b, uint32_t rec, iph_hist_stats* out)
     out->q_escaped = st[kStatEscaped];
     out->q_unresolved = st[kStatUnresolved];
     out->internal_hits = st[kStatInternalHits];
-    out->rays = b->host[rec].rays;
+    out->rays = b->group_rays[rec];
     out->qbits = b->host[rec].qbits;
     out->ray_weight_mm2 = b->weight[rec];
-    out->projected_area_mm2 = b->area[rec] > 0 ? b->area[rec] : (double)(out->rays - out->status[8]) * b->weight[rec];
+    out->projected_area_mm2 = b->group_area[rec];
+    out->q_diffraction = st[kStatDiffraction];
+    out->diffraction_captured = 0;
+    bool first = true;
+    for (uint32_t r : b->lobe_recs) {
+        if (b->owner[r] != rec) continue;
+        const Lobe& L = b->host[r].lobe;
+        double cap = 0;
+        IPH_HIP(ctx, hipMemcpy(&cap, L.partial + lobe_blocks(L.rows, b->host[r].np) + 1, sizeof cap,
+                               hipMemcpyDeviceToHost));
+        out->diffraction_captured = first ? cap : std::min(out->diffraction_captured, cap);
+        first = false;
+    }
     return IPH_OK;
 }
 
 int iph_batch_bins(iph_batch* b, uint32_t rec, uint64_t* out)
 {
     if (!b || !out || rec >= b->nrec) return fail(b ? b->ctx : nullptr, IPH_E_INVALID, "Invalid record index.");
+    if (b->owner[rec] != rec) return node_error(b, rec);
     iph_ctx* ctx = b->ctx;
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     const size_t n = (size_t)b->host[rec].nt * b->host[rec].np;
@@ -576,9 +668,10 @@ int iph_batch_bins(iph_batch* b, uint32_t rec, uint64_t* out)
 int iph_batch_cdf(iph_batch* b, uint32_t rec, double* phi, double* theta, iph_cdf_report* report)
 {
     if (!b || rec >= b->nrec) return fail(b ? b->ctx : nullptr, IPH_E_INVALID, "Invalid record index.");
+    if (b->owner[rec] != rec) return node_error(b, rec);
     iph_ctx* ctx = b->ctx;
     IPH_HIP(ctx, hipSetDevice(ctx->device));
-    const CdfJob& J = b->jobs[rec];
+    const CdfJob& J = b->jobs[b->job_of[rec]];
     if (phi) IPH_HIP(ctx, hipMemcpy(phi, J.phi, (J.onp + 1) * sizeof(double), hipMemcpyDeviceToHost));
     if (theta)
         IPH_HIP(ctx, hipMemcpy(theta, J.theta, (size_t)J.onp * (J.ont + 1) * sizeof(double), hipMemcpyDeviceToHost));
@@ -656,6 +749,67 @@ int iph_write_record(const char* out_dir, const char* metadata_json, const doubl
     if (!make_u_edges(nt, e, err)) return fail(nullptr, IPH_E_INVALID, err);
     const int rc = write_record_dir(out_dir, metadata_json, phi, theta, e, nt, np, err);
     if (rc != IPH_OK) return fail(nullptr, rc, err);
+    return IPH_OK;
+}
+
+int iph_water_optics(double wavelength_nm, double celsius, double pascal, iph_water_state* out)
+{
+    if (!out) return fail(nullptr, IPH_E_INVALID, "out must not be NULL.");
+    WaterState w;
+    std::string err;
+    if (!water_optics(wavelength_nm, celsius, pascal, w, err)) return fail(nullptr, IPH_E_INVALID, err);
+    out->temperature_kelvin = w.temperature_kelvin;
+    out->pressure_pascal = w.pressure_pascal;
+    out->density_kg_m3 = w.density_kg_m3;
+    out->density_pressure_residual_pascal = w.density_pressure_residual_pascal;
+    out->density_bracket_width_kg_m3 = w.density_bracket_width_kg_m3;
+    out->refractive_index = w.refractive_index;
+    return IPH_OK;
+}
+
+int iph_water_index(double wavelength_nm, double kelvin, double density_kg_m3, double* n)
+{
+    if (!n) return fail(nullptr, IPH_E_INVALID, "n must not be NULL.");
+    std::string err;
+    if (!water_index_r9_97(wavelength_nm, kelvin, density_kg_m3, *n, err)) return fail(nullptr, IPH_E_INVALID, err);
+    return IPH_OK;
+}
+
+int iph_drop_shape(float radius_mm, int sphere, float coefficients[8])
+{
+    if (!coefficients) return fail(nullptr, IPH_E_INVALID, "coefficients must not be NULL.");
+    std::string err;
+    if (!drop_shape_table(radius_mm, sphere != 0, coefficients, err)) return fail(nullptr, IPH_E_INVALID, err);
+    return IPH_OK;
+}
+
+int iph_drop_projected_area(const double coefficients[8], double radius_mm, const double incident[3], iph_area* out)
+{
+    if (!coefficients || !incident || !out) return fail(nullptr, IPH_E_INVALID, "Invalid arguments.");
+    AreaResult a;
+    std::string err;
+    if (!drop_projected_area(coefficients, radius_mm, incident, a, err)) return fail(nullptr, IPH_E_INVALID, err);
+    std::memset(out, 0, sizeof *out);
+    out->area_mm2 = a.area_mm2;
+    out->abs_error_mm2 = a.abs_error_mm2;
+    out->rel_error = a.rel_error;
+    out->evaluations = a.evaluations;
+    out->convexity_cells = a.convexity_cells;
+    out->analytic_sphere = a.analytic_sphere ? 1 : 0;
+    return IPH_OK;
+}
+
+int iph_hex_projected_area(double radius_mm, double length_mm, const double incident[3], double* area_mm2)
+{
+    if (!incident || !area_mm2) return fail(nullptr, IPH_E_INVALID, "Invalid arguments.");
+    hex::Prism p;
+    p.R = radius_mm;
+    p.L = length_mm;
+    if (!p.valid()) return fail(nullptr, IPH_E_INVALID, "Invalid hex prism geometry.");
+    d3 k;
+    if (!unit_of(D3(incident[0], incident[1], incident[2]), k))
+        return fail(nullptr, IPH_E_INVALID, "Incident direction must be finite and nonzero.");
+    *area_mm2 = p.projected_area(k);
     return IPH_OK;
 }
 

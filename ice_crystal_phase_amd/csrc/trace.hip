@@ -60,9 +60,16 @@ template <class EdgeLoad>
 __device__ inline void deposit(const DevRecord& R, d3 dir, double power, EdgeLoad edge, LaneCounters& c,
                                unsigned long long& ray_quanta)
 {
-    const unsigned long long q = quantize(power, R.qbits);
+    const unsigned long long q = quantize(power * R.qfactor, R.qbits);
     ray_quanta += q;
     ++c.samples;
+    if (R.rotated) {
+        // body -> ensemble (hex_phase/include/ice_crystal/phase_math.hpp:10, same
+        // operation order: column images summed left to right)
+        const double* m = R.rot;
+        dir = D3((m[0] * dir.x + m[1] * dir.y) + m[2] * dir.z, (m[3] * dir.x + m[4] * dir.y) + m[5] * dir.z,
+                 (m[6] * dir.x + m[7] * dir.y) + m[8] * dir.z);
+    }
     const CellHit h = bin_direction(R.frame, R.nt, R.np, dir, edge);
     if (h.kind == 2) {
         ++c.invalid;
@@ -88,7 +95,7 @@ __device__ inline void trace_tile(const DevRecord& R, uint64_t first, uint64_t c
             auto sink = [&](d3 dir, double pw) { deposit(R, dir, pw, edge, c, rq); };
             const hex::RayResult r = hex::trace(R.hex, first + i, sink);
             c.escaped += rq;
-            c.unresolved += quantize(r.unresolved, R.qbits);
+            c.unresolved += quantize(r.unresolved * R.qfactor, R.qbits);
             c.hits += r.internal_hits;
             count_status(c, r.status);
         }
@@ -383,6 +390,7 @@ extern "C" __global__ void __launch_bounds__(256)
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) queue[threadIdx.x] = 0u;
     for (uint32_t r = blockIdx.y; r < nrec; r += gridDim.y) {
         const DevRecord& R = recs[r];
+        if (!R.owns) continue;
         const uint64_t n = (uint64_t)R.nt * R.np;
         for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
             R.bins[k] = 0ull;

@@ -22,6 +22,8 @@ u32, u64, i32, f64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
 P = C.POINTER
 
 SHAPE_SPHERE, SHAPE_DROP, SHAPE_HEX = 0, 1, 2
+REC_DIFFRACTION = 1
+ABI_VERSION = 2
 RUN_TRACE, RUN_CDF, RUN_GRAPH, RUN_NO_AUDIT = 1, 2, 4, 8
 
 STATUS_NAMES = ("exhausted", "tail_tolerance_reached", "interaction_limit", "ambiguous_boundary",
@@ -43,14 +45,26 @@ class Record(C.Structure):
                 ("grid_w", u32), ("grid_h", u32), ("rays", u64), ("seed", u64), ("radius_mm", f64),
                 ("length_mm", f64), ("shape_coefficients", f64 * 8), ("incident", f64 * 3),
                 ("interior_index", f64), ("exterior_index", f64), ("wavelength_nm", f64),
-                ("tail_tolerance", f64)]
+                ("tail_tolerance", f64), ("rotation", f64 * 9), ("weight", f64), ("histogram_of", u32),
+                ("flags", u32)]
 
 
 class HistStats(C.Structure):
     _fields_ = [("q_forward", u64), ("q_backward", u64), ("samples", u64), ("invalid", u64), ("snapped", u64),
                 ("status", u64 * 9), ("q_escaped", u64), ("q_unresolved", u64), ("internal_hits", u64),
                 ("rays", u64), ("qbits", u32), ("pad", u32), ("ray_weight_mm2", f64),
-                ("projected_area_mm2", f64)]
+                ("projected_area_mm2", f64), ("q_diffraction", u64), ("diffraction_captured", f64)]
+
+
+class WaterState(C.Structure):
+    _fields_ = [("temperature_kelvin", f64), ("pressure_pascal", f64), ("density_kg_m3", f64),
+                ("density_pressure_residual_pascal", f64), ("density_bracket_width_kg_m3", f64),
+                ("refractive_index", f64)]
+
+
+class Area(C.Structure):
+    _fields_ = [("area_mm2", f64), ("abs_error_mm2", f64), ("rel_error", f64), ("evaluations", u32),
+                ("convexity_cells", u32), ("analytic_sphere", i32), ("pad", i32)]
 
 
 class CdfPolicy(C.Structure):
@@ -99,12 +113,16 @@ def lib() -> C.CDLL:
     L.iph_build_cdf.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(CdfPolicy), C.c_void_p, C.c_void_p,
                                 P(CdfReport)]
     L.iph_write_record.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, u32, u32]
-    for name, st in (("Record", Record), ("HistStats", HistStats), ("CdfReport", CdfReport),
-                     ("CdfPolicy", CdfPolicy)):
-        which = {"Record": 0, "HistStats": 1, "CdfReport": 2, "CdfPolicy": 3}[name]
+    L.iph_water_optics.argtypes = [f64, f64, f64, P(WaterState)]
+    L.iph_water_index.argtypes = [f64, f64, f64, P(f64)]
+    L.iph_drop_shape.argtypes = [C.c_float, C.c_int, P(C.c_float)]
+    L.iph_drop_projected_area.argtypes = [P(f64), f64, P(f64), P(Area)]
+    L.iph_hex_projected_area.argtypes = [f64, f64, P(f64), P(f64)]
+    for which, (name, st) in enumerate((("Record", Record), ("HistStats", HistStats), ("CdfReport", CdfReport),
+                                        ("CdfPolicy", CdfPolicy), ("WaterState", WaterState), ("Area", Area))):
         if L.iph_struct_size(which) != C.sizeof(st):
             raise IcePhaseError(-1, f"ABI mismatch for {name}: {L.iph_struct_size(which)} != {C.sizeof(st)}")
-    if L.iph_abi_version() != 1:
+    if L.iph_abi_version() != ABI_VERSION:
         raise IcePhaseError(-1, "Unsupported icephase ABI version")
     _lib = L
     return L
@@ -121,13 +139,50 @@ def _check(rc: int, ctx_ptr=None) -> None:
 EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_destroy", "iph_last_error",
             "iph_u_edges", "iph_bin_directions", "iph_batch_create", "iph_batch_destroy", "iph_batch_run",
             "iph_batch_set_schedule", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
-            "iph_build_cdf", "iph_write_record")
+            "iph_build_cdf", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
+            "iph_drop_projected_area", "iph_hex_projected_area")
 
 
 def u_edges(nt: int) -> np.ndarray:
     out = np.empty(nt + 1, np.float64)
     _check(lib().iph_u_edges(nt, out.ctypes.data_as(P(f64))))
     return out
+
+
+# ------------------------------------------------------------------ host services
+def water_optics(wavelength_nm: float, celsius: float = 20.0, pascal: float = 101325.0) -> WaterState:
+    """IAPWS-95 liquid density + IAPWS R9-97 index (iph_water_optics)."""
+    w = WaterState()
+    _check(lib().iph_water_optics(wavelength_nm, celsius, pascal, C.byref(w)))
+    return w
+
+
+def water_index(wavelength_nm: float, kelvin: float, density: float) -> float:
+    n = f64()
+    _check(lib().iph_water_index(wavelength_nm, kelvin, density, C.byref(n)))
+    return n.value
+
+
+def drop_shape(radius_mm: float, sphere: bool = False) -> list[float]:
+    """Table I coefficients (single precision values, as floats)."""
+    c = (C.c_float * 8)()
+    _check(lib().iph_drop_shape(radius_mm, 1 if sphere else 0, c))
+    return [float(v) for v in c]
+
+
+def drop_projected_area(coeffs: Sequence[float], radius_mm: float, incident) -> Area:
+    c = (f64 * 8)(*(list(coeffs) + [0.0] * (8 - len(coeffs)))[:8])
+    k = (f64 * 3)(*incident)
+    a = Area()
+    _check(lib().iph_drop_projected_area(c, radius_mm, k, C.byref(a)))
+    return a
+
+
+def hex_projected_area(radius_mm: float, length_mm: float, incident) -> float:
+    k = (f64 * 3)(*incident)
+    a = f64()
+    _check(lib().iph_hex_projected_area(radius_mm, length_mm, k, C.byref(a)))
+    return a.value
 
 
 # ------------------------------------------------------------------ records
@@ -161,14 +216,36 @@ def drop_record(coeffs: Sequence[float], n_int: float, grid: int, theta: int = 9
 
 def hex_record(radius_mm: float, length_mm: float, n_int: float, rays: int, incident=(1.0, -0.4, 0.3),
                seed: int = 12345, theta: int = 900, phi: int = 1800, max_internal_hits: int = 12,
-               tail: float = 1e-12, wavelength_nm: float = 550.0, n_ext: float = 1.0) -> Record:
+               tail: float = 1e-12, wavelength_nm: float = 550.0, n_ext: float = 1.0, rotation=None,
+               weight: float = 0.0, histogram_of: int = 0, diffraction: bool = False) -> Record:
+    """Hex record; rotation (3x3 body -> ensemble, row-major) makes it an
+    orientation node traced in the body frame and binned in the ensemble frame;
+    histogram_of = k > 0 adds it into record k-1's histogram."""
     r = Record()
     r.shape, r.max_internal_hits, r.theta_count, r.phi_count = SHAPE_HEX, max_internal_hits, theta, phi
     r.rays, r.seed = rays, seed
     r.radius_mm, r.length_mm = radius_mm, length_mm
     r.incident = (f64 * 3)(*incident)
     r.interior_index, r.exterior_index, r.wavelength_nm, r.tail_tolerance = n_int, n_ext, wavelength_nm, tail
+    if rotation is not None:
+        r.rotation = (f64 * 9)(*np.asarray(rotation, np.float64).reshape(9))
+    r.weight, r.histogram_of = weight, histogram_of
+    r.flags = REC_DIFFRACTION if diffraction else 0
     return r
+
+
+def quaternion_rotation(w: float, x: float, y: float, z: float) -> np.ndarray:
+    """Active Hamilton quaternion (w,x,y,z) -> row-major body -> ensemble matrix,
+    columns the images of the body axes (hex_phase/src/phase_table.cpp:11-21)."""
+    n = w * w + x * x + y * y + z * z
+    if not math.isfinite(n) or abs(n - 1) > 2e-12:
+        raise ValueError("Quaternion must be unit length (w,x,y,z).")
+    k = 1 / math.sqrt(n)
+    w, x, y, z = w * k, x * k, y * k, z * k
+    cx = (1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y))
+    cy = (2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x))
+    cz = (2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y))
+    return np.array([[cx[r], cy[r], cz[r]] for r in range(3)], np.float64)
 
 
 def record_rays(r: Record) -> int:

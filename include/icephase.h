@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define IPH_ABI_VERSION 1
+#define IPH_ABI_VERSION 2
 
 enum iph_status {
     IPH_OK = 0,
@@ -64,9 +64,32 @@ typedef struct iph_record {
     double shape_coefficients[8];  /* drop c0..c7 (must be convex); ignored for sphere/hex */
     double incident[3];            /* incident propagation direction (normalized by the library) */
     double interior_index, exterior_index;
-    double wavelength_nm;          /* recorded only; indices are given explicitly */
+    double wavelength_nm;          /* hex diffraction lobe wavelength; otherwise recorded only */
     double tail_tolerance;         /* stop a ray when its remaining power <= this */
+    /* Orientation node (hex only; hex_phase/src/phase_generate.cpp:44-53 and
+     * hex_phase/include/ice_crystal/phase_math.hpp:6-20): body -> ensemble
+     * rotation, row-major, ko_ensemble = R ko_body. All zero = no rotation:
+     * the record is traced and binned in its own frame. With a rotation, the
+     * ensemble direction `incident` is normalized, traced in the body frame as
+     * normalize(R^T k) and every outgoing direction is binned as R ko in the
+     * ensemble frame (make_frame of k), as ice_generate_phase does. */
+    double rotation[9];
+    /* Node cross-section weight (number probability of the orientation); the
+     * sample weight is weight * projected_area / rays (0 = 1). */
+    double weight;
+    /* 0: the record owns its histogram. k > 0: the record adds its samples to
+     * the histogram of record k-1 (an orientation node of that record; same
+     * grid, same ensemble incident direction, k-1 < this record's index). Only
+     * owners get CDFs. */
+    uint32_t histogram_of;
+    uint32_t flags;                /* IPH_REC_* */
 } iph_record;
+
+enum iph_record_flags {
+    /* hex: add the Fraunhofer diffraction lobe of the projected aperture
+     * (closed-form polygon transform, total mass = projected area). */
+    IPH_REC_DIFFRACTION = 1
+};
 
 /* Per-record trace/histogram audit. Power quanta are rint(p * 2^qbits) of a
  * unit-power incident ray; the mass a quantum represents is
@@ -80,7 +103,10 @@ typedef struct iph_hist_stats {
     uint64_t rays;
     uint32_t qbits, pad;
     double ray_weight_mm2;           /* cross-section carried by one ray */
-    double projected_area_mm2;       /* hex: exact face sum; drops: rays_hit * ray_weight */
+    double projected_area_mm2;       /* hex: exact face sum (weighted over nodes); drops: GK15 */
+    uint64_t q_diffraction;          /* quanta of the Fraunhofer lobe (IPH_REC_DIFFRACTION) */
+    double diffraction_captured;     /* lobe power inside the computed rows / projected area,
+                                        before the lobe is scaled to carry exactly the area */
 } iph_hist_stats;
 
 typedef struct iph_cdf_policy {
@@ -183,6 +209,37 @@ int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, co
  * (src/phase_cdf.cpp:482-493). Refuses an existing out_dir or .part. */
 int iph_write_record(const char* out_dir, const char* metadata_json, const double* phi, const double* theta,
                      uint32_t nt, uint32_t np);
+
+/* ---- host material and geometry services (no device work) ---------------- */
+
+typedef struct iph_water_state {
+    double temperature_kelvin, pressure_pascal, density_kg_m3;
+    double density_pressure_residual_pascal, density_bracket_width_kg_m3;
+    double refractive_index;
+} iph_water_state;
+/* Replaces rainbow::evaluate_water_optics (src/water_refractive_index.cpp:157-186):
+ * IAPWS-95 liquid density at (celsius, pascal), then the IAPWS R9-97 index at
+ * the vacuum wavelength. Rejects states outside 0..60 C, 50 kPa..100 MPa. */
+int iph_water_optics(double wavelength_nm, double celsius, double pascal, iph_water_state* out);
+/* Replaces rainbow::water_refractive_index_r9_97 (src/water_refractive_index.cpp:139-155). */
+int iph_water_index(double wavelength_nm, double kelvin, double density_kg_m3, double* n);
+
+/* Replaces RaindropShape::try_make (include/rainbow/raindrop_shape.hpp:22-69):
+ * Table I coefficients (single precision) for an equal-volume radius in (0,3] mm. */
+int iph_drop_shape(float radius_mm, int sphere, float coefficients[8]);
+
+typedef struct iph_area {
+    double area_mm2, abs_error_mm2, rel_error;
+    uint32_t evaluations, convexity_cells;
+    int32_t analytic_sphere, pad;
+} iph_area;
+/* Replaces rainbow::compute_projected_area (src/projected_area.cpp:163-228):
+ * certified strict convexity, then the projected area of the drop of
+ * equal-volume radius radius_mm along incident (particle frame). */
+int iph_drop_projected_area(const double coefficients[8], double radius_mm, const double incident[3],
+                            iph_area* out);
+/* Replaces HexPrism::projected_area_mm2 (hex_trace/include/ice_crystal/hex_prism.hpp:45-51). */
+int iph_hex_projected_area(double radius_mm, double length_mm, const double incident[3], double* area_mm2);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,28 @@ uint32_t orc_quantum_bits(uint64_t rays);
 int orc_hex_hist(const orc_hex_settings* s, uint32_t nt, uint32_t np, uint32_t qbits,
                  uint64_t first, uint64_t count, uint64_t* bins, orc_hist_stats* st);
 
+/* Orientation node of a composed record: trace normalize(R^T k) in the body
+ * frame, bin R d in make_frame(k), quanta rint(power*qfactor*2^qbits); bins and
+ * st are accumulated (hex_phase/src/phase_generate.cpp:44-53,
+ * hex_phase/src/phase_table.cpp:52-62). rot is row-major body -> ensemble. */
+int orc_hex_hist_node(const orc_hex_settings* s, const double rot[9], double qfactor, uint32_t nt, uint32_t np,
+                      uint32_t qbits, uint64_t* bins, orc_hist_stats* st);
+
+/* Projected area along the node's body direction normalize(R^T k). */
+double orc_hex_node_area(const orc_hex_settings* s, const double rot[9]);
+
+/* ---- Fraunhofer lobe (no reference counterpart: hex_trace/README.md:20-21) -- */
+/* Closed-form transform of a counter-clockwise polygon aperture. */
+void orc_aperture_transform(const double* vx, const double* vy, uint32_t nv, double q1, double q2, double* re,
+                            double* im);
+/* Convex aperture of the prism seen along k, in make_frame(k)'s (e0, e1). */
+uint32_t orc_hex_aperture(double radius_mm, double length_mm, const double k[3], double* vx, double* vy,
+                          double p1[3], double p2[3]);
+/* Lobe mass of one cell (mm^2), |F|^2/lambda^2 integrated over the cell. */
+double orc_lobe_cell(const double* vx, const double* vy, uint32_t nv, double area, double dmax, const double p1[3],
+                     const double p2[3], double lambda_mm, const orc_frame* g, uint32_t nt, uint32_t np, uint32_t i,
+                     uint32_t j);
+
 /* Reference PhaseMassAccumulator semantics: per-bin Kahan FP64 of w=scale*power,
  * poles spread by /np into the end rows. hex_phase/src/phase_table.cpp:39-76. */
 int orc_hex_masses(const orc_hex_settings* s, uint32_t nt, uint32_t np,

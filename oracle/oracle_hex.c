@@ -593,6 +593,207 @@ int orc_hex_hist(const orc_hex_settings* s, uint32_t nt, uint32_t np, uint32_t q
     return 0;
 }
 
+/* One orientation node of a composed record (hex_phase/src/phase_generate.cpp:44-53,
+ * hex_phase/src/phase_table.cpp:52-62): the ensemble direction s->ki is
+ * normalized, the node is traced in the body frame as normalize(R^T k), every
+ * outgoing direction is binned as R d in make_frame(k), and the emission quanta
+ * are rint(power * qfactor * 2^qbits). bins/st are accumulated, not zeroed. */
+int orc_hex_hist_node(const orc_hex_settings* s, const double rot[9], double qfactor, uint32_t nt, uint32_t np,
+                      uint32_t qbits, uint64_t* bins, orc_hist_stats* st)
+{
+    v3 ke;
+    if (!nrm(mk(s->ki[0], s->ki[1], s->ki[2]), &ke)) return -2;
+    const double* m = rot;
+    orc_hex_settings body = *s;
+    body.ki[0] = (m[0] * ke.x + m[3] * ke.y) + m[6] * ke.z;
+    body.ki[1] = (m[1] * ke.x + m[4] * ke.y) + m[7] * ke.z;
+    body.ki[2] = (m[2] * ke.x + m[5] * ke.y) + m[8] * ke.z;
+    hexcfg c;
+    if (hex_setup(&body, &c)) return -2;
+    frm fe;
+    if (!make_frame(ke, &fe)) return -2;
+    orc_frame g;
+    g.ki[0] = ke.x; g.ki[1] = ke.y; g.ki[2] = ke.z;
+    g.e0[0] = fe.e0.x; g.e0[1] = fe.e0.y; g.e0[2] = fe.e0.z;
+    g.e1[0] = fe.e1.x; g.e1[1] = fe.e1.y; g.e1[2] = fe.e1.z;
+    double* ue = (double*)malloc(sizeof(double) * (nt + 1));
+    if (orc_u_edges(nt, ue)) { free(ue); return -3; }
+    orc_sample buf[4096];
+    for (uint64_t i = 0; i < c.total; ++i) {
+        sink sk = {buf, 4096, 0, 0};
+        const orc_ray_audit a = trace_one(&c, i, &sk);
+        if (sk.overflow) { free(ue); return -1; }
+        st->status[a.status & 7]++;
+        st->internal_hits += a.internal_hits;
+        if (a.status != ST_INVALID && a.status != ST_AMBIG) st->rays_hit++;
+        uint64_t qe = 0;
+        for (uint64_t k = 0; k < sk.n; ++k) {
+            const uint64_t q = quantize(buf[k].power * qfactor, qbits);
+            qe += q;
+            const double* d = buf[k].dir;
+            const double r[3] = {(m[0] * d[0] + m[1] * d[1]) + m[2] * d[2], (m[3] * d[0] + m[4] * d[1]) + m[5] * d[2],
+                                 (m[6] * d[0] + m[7] * d[1]) + m[8] * d[2]};
+            const orc_bin b = orc_phase_bin(ue, nt, np, &g, r);
+            st->samples++;
+            if (!b.valid) { st->invalid++; continue; }
+            st->snapped += (uint64_t)b.snapped;
+            if (b.pole > 0) st->q_forward += q;
+            else if (b.pole < 0) st->q_backward += q;
+            else bins[b.index] += q;
+        }
+        st->q_escaped += qe;
+        st->q_unresolved += quantize(a.unresolved * qfactor, qbits);
+    }
+    free(ue);
+    return 0;
+}
+
+/* Projected area of the prism along the node's body direction normalize(R^T k)
+ * (the operations orc_hex_hist_node traces with). */
+double orc_hex_node_area(const orc_hex_settings* s, const double rot[9])
+{
+    v3 ke;
+    if (!nrm(mk(s->ki[0], s->ki[1], s->ki[2]), &ke)) return -1;
+    const double* m = rot;
+    orc_hex_settings body = *s;
+    body.ki[0] = (m[0] * ke.x + m[3] * ke.y) + m[6] * ke.z;
+    body.ki[1] = (m[1] * ke.x + m[4] * ke.y) + m[7] * ke.z;
+    body.ki[2] = (m[2] * ke.x + m[5] * ke.y) + m[8] * ke.z;
+    hexcfg c;
+    if (hex_setup(&body, &c)) return -1;
+    return proj_area(c.h, c.k);
+}
+
+/* ---- Fraunhofer lobe of the projected aperture ---- */
+/* F(q) of a counter-clockwise polygon by the divergence theorem over its edges:
+ * F = (i/|q|^2) sum (q1 e_y - q2 e_x) exp(-i q.m) sinc(q.e/2). */
+void orc_aperture_transform(const double* vx, const double* vy, uint32_t nv, double q1, double q2, double* re,
+                            double* im)
+{
+    double sr = 0, si = 0;
+    for (uint32_t k = 0; k < nv; ++k) {
+        const uint32_t k1 = k + 1 == nv ? 0 : k + 1;
+        const double ex = vx[k1] - vx[k], ey = vy[k1] - vy[k];
+        const double mx_ = 0.5 * (vx[k1] + vx[k]), my_ = 0.5 * (vy[k1] + vy[k]);
+        const double qn = q1 * ey - q2 * ex;
+        const double h = 0.5 * (q1 * ex + q2 * ey);
+        const double sinc = fabs(h) < 1e-8 ? 1.0 - h * h / 6.0 : sin(h) / h;
+        const double ph = -(q1 * mx_ + q2 * my_);
+        const double w = qn * sinc;
+        sr += w * cos(ph);
+        si += w * sin(ph);
+    }
+    const double qq = q1 * q1 + q2 * q2;
+    *re = -si / qq;
+    *im = sr / qq;
+}
+
+static int cmp_pt(const void* a, const void* b)
+{
+    const double* p = (const double*)a;
+    const double* q = (const double*)b;
+    if (p[0] != q[0]) return p[0] < q[0] ? -1 : 1;
+    if (p[1] != q[1]) return p[1] < q[1] ? -1 : 1;
+    return 0;
+}
+
+/* Aperture of the prism seen along k (normalized inside): convex hull of the 12
+ * projected vertices in make_frame(k)'s (e0, e1), counter-clockwise. Returns the
+ * vertex count (0 on failure); p1/p2 receive e0/e1. */
+uint32_t orc_hex_aperture(double radius_mm, double length_mm, const double k_in[3], double* vx, double* vy,
+                          double p1[3], double p2[3])
+{
+    v3 k;
+    frm f;
+    if (!nrm(mk(k_in[0], k_in[1], k_in[2]), &k) || !make_frame(k, &f)) return 0;
+    double pts[12][2];
+    int n = 0;
+    for (unsigned fc = 0; fc < 6; ++fc)
+        for (int sg = -1; sg <= 1; sg += 2) {
+            const v3 v = pvertex(fc);
+            const v3 p = mk(radius_mm * v.x, 0.5 * sg * length_mm, radius_mm * v.z);
+            pts[n][0] = dot(p, f.e0);
+            pts[n][1] = dot(p, f.e1);
+            ++n;
+        }
+    qsort(pts, 12, sizeof pts[0], cmp_pt);
+    const double sc = radius_mm > length_mm ? radius_mm : length_mm;
+    const double tol = 1e-12 * sc * sc;
+    double hull[24][2];
+    int h = 0;
+#define TURN(o, a, b) (((a)[0] - (o)[0]) * ((b)[1] - (o)[1]) - ((a)[1] - (o)[1]) * ((b)[0] - (o)[0]))
+    for (int i = 0; i < 12; ++i) {
+        while (h >= 2 && TURN(hull[h - 2], hull[h - 1], pts[i]) <= tol) --h;
+        hull[h][0] = pts[i][0]; hull[h][1] = pts[i][1]; ++h;
+    }
+    for (int i = 10, t = h + 1; i >= 0; --i) {
+        while (h >= t && TURN(hull[h - 2], hull[h - 1], pts[i]) <= tol) --h;
+        hull[h][0] = pts[i][0]; hull[h][1] = pts[i][1]; ++h;
+    }
+#undef TURN
+    const int nv = h - 1;
+    if (nv < 3 || nv > 8) return 0;
+    for (int i = 0; i < nv; ++i) { vx[i] = hull[i][0]; vy[i] = hull[i][1]; }
+    p1[0] = f.e0.x; p1[1] = f.e0.y; p1[2] = f.e0.z;
+    p2[0] = f.e1.x; p2[1] = f.e1.y; p2[2] = f.e1.z;
+    return (uint32_t)nv;
+}
+
+static const double LGX[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+static const double LGW[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+
+static int panels(double x)
+{
+    const double c = ceil(x);
+    return c < 1 ? 1 : (c > 64 ? 64 : (int)c);
+}
+
+/* Lobe mass (mm^2) in cell (i, j) of an nt x np grid with binning frame g:
+ * int |F(q)|^2 / lambda^2 dOmega over the cell, q = k0 (d.p1, d.p2), by 4-point
+ * Gauss-Legendre in theta and phi on sub-panels no wider than half a fringe. */
+double orc_lobe_cell(const double* vx, const double* vy, uint32_t nv, double area, double dmax, const double p1[3],
+                     const double p2[3], double lambda_mm, const orc_frame* g, uint32_t nt, uint32_t np, uint32_t i,
+                     uint32_t j)
+{
+    const double k0 = 2 * PI_D / lambda_mm;
+    const double t0 = PI_D * i / nt, t1 = PI_D * (i + 1) / nt;
+    const double f0 = -PI_D + 2 * PI_D * j / np, f1 = -PI_D + 2 * PI_D * (j + 1) / np;
+    const double fringe = lambda_mm / dmax;
+    const double thi = t1 < 0.5 * PI_D ? t1 : 0.5 * PI_D;
+    const int nts = panels(2 * (t1 - t0) / fringe), nps = panels(2 * sin(thi) * (f1 - f0) / fringe);
+    const double ht = (t1 - t0) / nts, hp = (f1 - f0) / nps;
+    double total = 0;
+    for (int a = 0; a < nts; ++a) {
+        double rowsum = 0;
+        for (int gt = 0; gt < 4; ++gt) {
+            const double th = t0 + ht * a + 0.5 * ht * (1 + LGX[gt]);
+            const double st = sin(th), ct = cos(th);
+            double colsum = 0;
+            for (int b = 0; b < nps; ++b)
+                for (int gp = 0; gp < 4; ++gp) {
+                    const double ph = f0 + hp * b + 0.5 * hp * (1 + LGX[gp]);
+                    const double cx = st * cos(ph), cy = st * sin(ph);
+                    double d[3];
+                    for (int c = 0; c < 3; ++c) d[c] = cx * g->e0[c] + cy * g->e1[c] + ct * g->ki[c];
+                    const double q1 = k0 * (d[0] * p1[0] + d[1] * p1[1] + d[2] * p1[2]);
+                    const double q2 = k0 * (d[0] * p2[0] + d[1] * p2[1] + d[2] * p2[2]);
+                    double dens;
+                    if (sqrt(q1 * q1 + q2 * q2) * dmax < 1e-4) {
+                        dens = area * area;
+                    } else {
+                        double re, im;
+                        orc_aperture_transform(vx, vy, nv, q1, q2, &re, &im);
+                        dens = re * re + im * im;
+                    }
+                    colsum += LGW[gp] * dens / (lambda_mm * lambda_mm);
+                }
+            rowsum += LGW[gt] * st * colsum;
+        }
+        total += rowsum;
+    }
+    return total * (0.5 * ht) * (0.5 * hp);
+}
+
 int orc_hex_masses(const orc_hex_settings* s, uint32_t nt, uint32_t np,
                    double* mass, double* forward, double* backward)
 {

@@ -8,6 +8,9 @@
 #include <rainbow/phase_cdf_math.hpp>
 #include <rainbow/water_refractive_index.hpp>
 #include <rainbow/projected_area.hpp>
+#include <rainbow/raindrop_shape.hpp>
+#include <ice_crystal/phase_generate.hpp>
+#include <ice_crystal/orientation_plan.hpp>
 
 #include <cstdint>
 #include <cstring>
@@ -139,6 +142,45 @@ int ref_projected_area(const double c[8], double radius_mm, const double k[3], d
         for (int i = 0; i < 8; ++i) coeffs[i] = c[i];
         const auto r = rainbow::compute_projected_area(coeffs, radius_mm, {k[0], k[1], k[2]});
         *area = r.area_mm2; *sphere = r.analytic_sphere;
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
+int ref_drop_shape(float radius_mm, int sphere, float c[8], float* inner, float* outer)
+{
+    rainbow::RaindropShape s{};
+    if (!rainbow::RaindropShape::try_make(radius_mm, sphere != 0, s)) return -1;
+    for (int i = 0; i < 8; ++i) c[i] = s.coefficients[i];
+    *inner = s.inner_radius; *outer = s.outer_radius;
+    return 0;
+}
+
+// The reference CLI's composition (hex_phase/apps/generate_hex_phase.cpp:73-118,
+// CPU branch): every node traced with node_trace_settings and accumulated with
+// PhaseMassAccumulator::add(outgoing, rotation, weight*area/samples).
+int ref_orientation_masses(double R, double L, const double k[3], double ni, double nti, uint32_t max_hits,
+                           double tail, const char* csv, uint32_t nt, uint32_t np, double* mass, double* area_out)
+{
+    try {
+        PhaseGenerationSettings s{};
+        s.prism.circumradius_mm = R; s.prism.length_mm = L;
+        s.grid = make_phase_grid(nt, np, Vec3{k[0], k[1], k[2]});
+        s.interior_index = nti; s.exterior_index = ni; s.wavelength_nm = 550;
+        s.max_internal_hits = max_hits; s.tail_tolerance = tail;
+        s.out_nt = nt; s.out_np = np;
+        const auto plan = parse_orientation_plan(csv);
+        PhaseMassAccumulator acc(s.grid);
+        double area = 0;
+        for (const auto& node : plan.nodes) {
+            const auto c = node_trace_settings(s, node);
+            const double scale = node.weight * node_area(s, node) / double(node.samples);
+            area += node.weight * node_area(s, node);
+            const auto batch = trace_batch_cpu(c, 0, node.samples, 64u * 1024 * 1024);
+            acc.add(batch.outgoing, node.rotation, scale);
+        }
+        const auto m = acc.masses();
+        std::memcpy(mass, m.data(), m.size() * sizeof(double));
+        *area_out = area;
         return 0;
     } catch (const std::exception&) { return -1; }
 }

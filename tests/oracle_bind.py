@@ -121,6 +121,16 @@ def lib() -> C.CDLL:
         L.orc_normalize3.argtypes = [P(f64), P(f64)]
         L.orc_sphere_theta_hist.argtypes = [u32, f32, f64, u32, u32, C.c_void_p]
         L.orc_sphere_theta_hist.restype = None
+        L.orc_hex_hist_node.argtypes = [P(HexSettings), P(f64), f64, u32, u32, u32, C.c_void_p, P(HistStats)]
+        L.orc_hex_node_area.argtypes = [P(HexSettings), P(f64)]
+        L.orc_hex_node_area.restype = f64
+        L.orc_aperture_transform.argtypes = [P(f64), P(f64), u32, f64, f64, P(f64), P(f64)]
+        L.orc_aperture_transform.restype = None
+        L.orc_hex_aperture.argtypes = [f64, f64, P(f64), P(f64), P(f64), P(f64), P(f64)]
+        L.orc_hex_aperture.restype = u32
+        L.orc_lobe_cell.argtypes = [P(f64), P(f64), u32, f64, f64, P(f64), P(f64), f64, P(Frame), u32, u32, u32,
+                                    u32]
+        L.orc_lobe_cell.restype = f64
         _lib = L
     return _lib
 
@@ -146,6 +156,9 @@ def ref() -> C.CDLL:
                                          C.c_void_p, P(RefCdfReport)]
         R.ref_water_index.argtypes = [f64, f64, f64, P(f64), P(f64)]
         R.ref_projected_area.argtypes = [P(f64), f64, P(f64), P(f64), P(C.c_int)]
+        R.ref_drop_shape.argtypes = [f32, C.c_int, P(f32), P(f32), P(f32)]
+        R.ref_orientation_masses.argtypes = [f64, f64, P(f64), f64, f64, u32, f64, C.c_char_p, u32, u32,
+                                             C.c_void_p, P(f64)]
         _ref = R
     return _ref
 
@@ -323,6 +336,72 @@ def ref_projected_area(coeffs, radius_mm, k):
     sph = C.c_int()
     rc = ref().ref_projected_area((f64 * 8)(*coeffs), radius_mm, vec3(k), C.byref(area), C.byref(sph))
     return rc, area.value, bool(sph.value)
+
+
+def hex_hist_node(s: HexSettings, rot: np.ndarray, qfactor: float, nt: int, np_: int, qbits: int,
+                  bins: np.ndarray | None = None, st: HistStats | None = None):
+    """Accumulate one orientation node (rot: 3x3 body -> ensemble) into bins/st."""
+    bins = np.zeros(nt * np_, np.uint64) if bins is None else bins
+    st = HistStats() if st is None else st
+    r = (f64 * 9)(*np.asarray(rot, np.float64).reshape(9))
+    assert lib().orc_hex_hist_node(C.byref(s), r, qfactor, nt, np_, qbits, bins.ctypes.data, C.byref(st)) == 0
+    return bins, st
+
+
+def hex_node_area(s: HexSettings, rot) -> float:
+    r = (f64 * 9)(*np.asarray(rot, np.float64).reshape(9))
+    return float(lib().orc_hex_node_area(C.byref(s), r))
+
+
+def aperture_transform(vx, vy, q1: float, q2: float) -> complex:
+    vx = np.ascontiguousarray(vx, np.float64)
+    vy = np.ascontiguousarray(vy, np.float64)
+    re, im = f64(), f64()
+    lib().orc_aperture_transform(_ptr(vx, P(f64)), _ptr(vy, P(f64)), len(vx), q1, q2, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def hex_aperture(radius_mm: float, length_mm: float, k):
+    vx = np.zeros(8, np.float64)
+    vy = np.zeros(8, np.float64)
+    p1, p2 = (f64 * 3)(), (f64 * 3)()
+    nv = lib().orc_hex_aperture(radius_mm, length_mm, vec3(k), _ptr(vx, P(f64)), _ptr(vy, P(f64)), p1, p2)
+    assert nv >= 3
+    return vx[:nv].copy(), vy[:nv].copy(), np.array(p1[:]), np.array(p2[:])
+
+
+def polygon_area(vx, vy) -> float:
+    return 0.5 * float(np.sum(vx * np.roll(vy, -1) - np.roll(vx, -1) * vy))
+
+
+def lobe_cells(radius_mm, length_mm, k, lambda_nm, nt, np_, rows, frame: Frame | None = None):
+    """Oracle lobe masses (mm^2) of the leading `rows` rows, frame = make_frame(k) by default."""
+    vx, vy, p1, p2 = hex_aperture(radius_mm, length_mm, k)
+    area = polygon_area(vx, vy)
+    dmax = max(np.hypot(vx[i] - vx[j], vy[i] - vy[j]) for i in range(len(vx)) for j in range(len(vx)))
+    g = make_frame(k) if frame is None else frame
+    vxa, vya = np.ascontiguousarray(vx), np.ascontiguousarray(vy)
+    out = np.zeros((rows, np_), np.float64)
+    for i in range(rows):
+        for j in range(np_):
+            out[i, j] = lib().orc_lobe_cell(_ptr(vxa, P(f64)), _ptr(vya, P(f64)), len(vx), area, dmax, vec3(p1),
+                                            vec3(p2), lambda_nm * 1e-6, C.byref(g), nt, np_, i, j)
+    return out, area, dmax
+
+
+def ref_drop_shape(radius_mm: float, sphere: bool = False):
+    c = (f32 * 8)()
+    inner, outer = f32(), f32()
+    rc = ref().ref_drop_shape(radius_mm, 1 if sphere else 0, c, C.byref(inner), C.byref(outer))
+    return rc, np.array(c[:], np.float32), inner.value, outer.value
+
+
+def ref_orientation_masses(radius_mm, length_mm, k, n_ext, n_int, max_hits, tail, csv: str, nt, np_):
+    mass = np.zeros(nt * np_, np.float64)
+    area = f64()
+    rc = ref().ref_orientation_masses(radius_mm, length_mm, vec3(k), n_ext, n_int, max_hits, tail, csv.encode(), nt,
+                                      np_, mass.ctypes.data, C.byref(area))
+    return rc, mass, area.value
 
 
 def oracle_env_ok() -> bool:

@@ -32,12 +32,14 @@ def test_header_symbols_are_exported():
 
 def test_abi_version_and_struct_sizes():
     lib = ip.lib()
-    assert lib.iph_abi_version() == 1
+    assert lib.iph_abi_version() == 2
     import ctypes
-    assert lib.iph_struct_size(0) == ctypes.sizeof(icephase.Record) == 176
+    assert lib.iph_struct_size(0) == ctypes.sizeof(icephase.Record) == 264
     assert lib.iph_struct_size(1) == ctypes.sizeof(icephase.HistStats)
     assert lib.iph_struct_size(2) == ctypes.sizeof(icephase.CdfReport)
     assert lib.iph_struct_size(3) == ctypes.sizeof(icephase.CdfPolicy)
+    assert lib.iph_struct_size(4) == ctypes.sizeof(icephase.WaterState)
+    assert lib.iph_struct_size(5) == ctypes.sizeof(icephase.Area)
 
 
 def test_u_edges_equal_oracle(oracle):

@@ -35,7 +35,7 @@ def run_one(ctx, rec, flags=ip.RUN_TRACE | ip.RUN_CDF, policy=None):
 
 
 def test_abi_version(ctx):
-    assert ip.lib().iph_abi_version() == 1
+    assert ip.lib().iph_abi_version() == ip.icephase.ABI_VERSION
 
 
 def test_bin_directions_match_oracle(ctx, oracle):

@@ -1,0 +1,214 @@
+"""GPU tests of the widened path: orientation-node composition (bit-exact
+against the oracle), the Fraunhofer lobe (against the oracle restatement), and
+the two command-line tools end to end (against the reference's own CPU tool
+where it was built, and against the on-disk contract).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import ice_crystal_phase_amd as ip
+import record_check
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+BIN = ROOT / "ice_crystal_phase_amd" / "bin"
+REF_CLI = ROOT / "oracle" / "_ref" / "ice_generate_phase_cpu"
+S2 = math.sqrt(0.5)
+PLAN = [(1, 0, 0, 0, 0.5, 6000, 1), (S2, 0, S2, 0, 0.25, 4000, 2),
+        (0.9238795325112867, 0.3826834323650898, 0, 0, 0.25, 4000, 3)]
+
+
+def _csv(rows):
+    return "qw,qx,qy,qz,weight,samples,seed\n" + "".join(",".join(repr(v) for v in r) + "\n" for r in rows)
+
+
+def _node_records(k, nt, np_, plan, max_hits=64, diffraction=False):
+    recs = []
+    for i, (w, x, y, z, wt, n, seed) in enumerate(plan):
+        recs.append(ip.hex_record(0.05, 0.2, 1.31, n, incident=k, seed=seed, theta=nt, phi=np_,
+                                  max_internal_hits=max_hits, rotation=ip.quaternion_rotation(w, x, y, z), weight=wt,
+                                  histogram_of=0 if i == 0 else 1, diffraction=diffraction))
+    return recs
+
+
+def test_orientation_nodes_bitexact(ctx, oracle):
+    print("case: orientation nodes 3 x hex, 48x96", flush=True)
+    k = (1.0, -0.4, 0.3)
+    nt, np_ = 48, 96
+    with ip.Batch(ctx, _node_records(k, nt, np_, PLAN)) as b:
+        b.run(ip.RUN_TRACE | ip.RUN_CDF)
+        gpu = b.bins(0)
+        st = b.stats(0)
+        with pytest.raises(ip.IcePhaseError, match="orientation node"):
+            b.bins(1)
+    total = sum(r[5] for r in PLAN)
+    qbits = oracle.quantum_bits(total)
+    scales = []
+    for w, x, y, z, wt, n, seed in PLAN:
+        s = oracle.hex_settings(ki=k, seed=seed, total=n, max_hits=64)
+        scales.append(wt * oracle.hex_node_area(s, ip.quaternion_rotation(w, x, y, z)) / n)
+    smax = max(scales)
+    bins = np.zeros(nt * np_, np.uint64)
+    ost = oracle.HistStats()
+    for (w, x, y, z, wt, n, seed), sc in zip(PLAN, scales):
+        s = oracle.hex_settings(ki=k, seed=seed, total=n, max_hits=64)
+        oracle.hex_hist_node(s, ip.quaternion_rotation(w, x, y, z), sc / smax, nt, np_, qbits, bins, ost)
+    assert st.qbits == qbits and st.rays == total
+    assert abs(st.ray_weight_mm2 / smax - 1) == 0
+    assert np.array_equal(gpu, bins)
+    assert (st.q_forward, st.q_backward, st.q_escaped, st.q_unresolved) == (
+        ost.q_forward, ost.q_backward, ost.q_escaped, ost.q_unresolved)
+    assert list(st.status[:8]) == list(ost.status)
+
+
+def test_orientation_node_validation(ctx):
+    k = (1.0, -0.4, 0.3)
+    recs = _node_records(k, 24, 48, PLAN[:2])
+    bad = ip.hex_record(0.05, 0.2, 1.31, 100, incident=k, theta=24, phi=48, histogram_of=1)  # not rotated
+    with pytest.raises(ip.IcePhaseError, match="orientation node"):
+        ip.Batch(ctx, [recs[0], bad])
+    with pytest.raises(ip.IcePhaseError, match="orthonormal"):
+        ip.Batch(ctx, [ip.hex_record(0.05, 0.2, 1.31, 100, rotation=np.diag([1.0, 1.0, -1.0]))])
+    drop = ip.sphere_record(1.333, 16, 18, 36)
+    drop.weight = 0.5
+    with pytest.raises(ip.IcePhaseError, match="hex records only"):
+        ip.Batch(ctx, [drop])
+
+
+def _lobe_rows(nt, lam_nm, dmax):
+    cut = min(0.5, 200 * lam_nm * 1e-6 / dmax)
+    return max(1, min(nt, math.ceil(cut / (math.pi / nt))))
+
+
+@pytest.mark.parametrize("k", [(1.0, -0.4, 0.3), (0.0, 1.0, 0.0)])
+def test_diffraction_lobe_matches_oracle(ctx, oracle, k):
+    print(f"case: lobe k={k}", flush=True)
+    nt, np_, rays, lam = 180, 72, 4096, 550.0
+    plain = ip.hex_record(0.05, 0.2, 1.31, rays, incident=k, theta=nt, phi=np_, max_internal_hits=32)
+    lobe = ip.hex_record(0.05, 0.2, 1.31, rays, incident=k, theta=nt, phi=np_, max_internal_hits=32,
+                         wavelength_nm=lam, diffraction=True)
+    with ip.Batch(ctx, [plain, lobe]) as b:
+        b.run(ip.RUN_TRACE | ip.RUN_CDF)
+        q_lobe = b.bins(1).astype(np.int64) - b.bins(0).astype(np.int64)
+        st = b.stats(1)
+        b.run(ip.RUN_TRACE)
+        again = b.bins(1).astype(np.int64) - b.bins(0).astype(np.int64)
+    assert np.array_equal(q_lobe, again), "lobe must be bitwise reproducible"
+    vx, vy, _, _ = oracle.hex_aperture(0.05, 0.2, k)
+    dmax = max(np.hypot(vx[i] - vx[j], vy[i] - vy[j]) for i in range(len(vx)) for j in range(len(vx)))
+    rows = _lobe_rows(nt, lam, dmax)
+    m, area, _ = oracle.lobe_cells(0.05, 0.2, k, lam, nt, np_, rows)
+    total = m.sum()
+    qbits = oracle.quantum_bits(rays)
+    want = np.rint(m / total * rays * 2.0 ** qbits).reshape(-1)
+    got = q_lobe.reshape(nt, np_)[:rows].reshape(-1).astype(np.float64)
+    assert np.all(q_lobe.reshape(nt, np_)[rows:] == 0)
+    assert np.all(np.abs(got - want) <= 1e-9 * want + 2)
+    assert st.q_diffraction == int(q_lobe.sum())
+    assert abs(st.diffraction_captured - total / area) < 1e-12
+    assert 0.95 < st.diffraction_captured <= 1.0 + 1e-9
+    # the lobe carries the record's incident power: rays * 2^Q quanta
+    assert abs(st.q_diffraction / (rays * 2.0 ** qbits) - 1) < 1e-9
+
+
+# ---------------------------------------------------------------- command-line tools
+def _run(args, cwd, timeout=300):
+    print("run:", " ".join(str(a) for a in args), flush=True)
+    return subprocess.run([str(a) for a in args], cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+HEX_ARGS = ["--radius-mm", "0.05", "--length-mm", "0.2", "--ki", "1", "-0.4", "0.3", "--ior", "1.31",
+            "--wavelength-nm", "550", "--theta", "90", "--phi", "180"]
+
+
+def test_ice_generate_phase_matches_reference_cli(tmp_path):
+    (tmp_path / "plan.csv").write_text(_csv(PLAN))
+    ours = _run([BIN / "ice_generate_phase", "--out", "ours", "--orientations", "plan.csv"] + HEX_ARGS, tmp_path)
+    assert ours.returncode == 0, ours.stderr
+    m = record_check.check_record(tmp_path / "ours")
+    assert m["orientation"]["csv_sha256"] == hashlib.sha256((tmp_path / "plan.csv").read_bytes()).hexdigest()
+    assert m["sampling"]["processed_rays"] == sum(r[5] for r in PLAN)
+    assert m["histogram"]["relative_mass_audit_error"] == 0
+    if not REF_CLI.exists():
+        pytest.skip("reference CPU tool not built (no reference checkout at build time)")
+    ref = _run([REF_CLI, "--out", "ref", "--orientations", "plan.csv"] + HEX_ARGS, tmp_path)
+    assert ref.returncode == 0, ref.stderr
+    r = record_check.check_record(tmp_path / "ref")
+    missing = record_check.key_paths(r) - record_check.key_paths(m)
+    assert not missing, missing
+    assert m["transport_audit"]["status_counts"] == r["transport_audit"]["status_counts"]
+    assert m["sampling"]["outgoing_samples"] == r["sampling"]["outgoing_samples"]
+    assert abs(m["geometry"]["projected_area_mm2"] / r["geometry"]["projected_area_mm2"] - 1) < 1e-14
+    assert np.allclose(m["sampling_frame_columns"], r["sampling_frame_columns"], rtol=0, atol=1e-15)
+    for name in ("phi_cdf.npy", "theta_given_phi_cdf.npy", "u_edges.npy"):
+        a = np.load(tmp_path / "ours" / name, allow_pickle=False)
+        b = np.load(tmp_path / "ref" / name, allow_pickle=False)
+        assert np.max(np.abs(a - b)) < 1e-11, name
+    assert abs(m["hg"]["g"] - r["hg"]["g"]) < 1e-11
+
+
+def test_ice_generate_phase_rejects_interaction_limit(tmp_path):
+    """Rays that stop at the interaction limit are not accepted (the reference
+    rule): exit 2, failure.json kept in <out>.part."""
+    (tmp_path / "plan.csv").write_text(_csv([(1, 0, 0, 0, 1.0, 3000, 7)]))
+    res = _run([BIN / "ice_generate_phase", "--out", "x", "--orientations", "plan.csv", "--max-internal-hits", "2"]
+               + HEX_ARGS, tmp_path)
+    assert res.returncode == 2
+    f = json.loads((tmp_path / "x.part" / "failure.json").read_text())
+    assert f["complete"] is False and f["status_counts"]["interaction_limit"] > 0
+    assert not (tmp_path / "x").exists()
+
+
+DROP_KEYS = {
+    "coordinate_contract_id", "direction_convention", "particle_frame_handedness", "particle_up_axis",
+    "shape_polar_axis", "sampling_frame_layout", "sampling_frame_map", "theta_zero", "theta_pi", "azimuth_periodic",
+    "inclination_definition", "geometry", "cross_sections", "material", "diffraction_policy", "schema", "complete",
+    "dtype", "order", "theta_count", "phi_count", "density_measure", "cell_model", "coordinates", "theta_definition",
+    "cdf_axis_order", "phi_range_rad", "normalization", "input_polarization", "incident_total_intensity",
+    "single_scattering_albedo", "collision_model", "projected_area_storage", "stage", "radius_mm", "wavelength_nm",
+    "exterior_index", "interior_index", "force_sphere", "incident_grid", "incident_grid_half_extent",
+    "reference_distances", "query_frame_axis", "query_frame_e0", "sampling_frame_columns", "shape_coefficients",
+    "focal_quarter_turn_offsets", "diffraction_primary_sigma_degrees_setting",
+    "diffraction_zero_setting_selects_Table_II", "diffraction_transition_contrast", "folded_branch_model",
+    "folded_relative_tolerance", "source_commit", "source_dirty_at_configure", "generator_version", "quality",
+    "incident_inclination_degrees", "query_grid", "storage_processing", "hg",
+}
+
+
+@pytest.mark.parametrize("extra", [[], ["--sphere"], ["--shape-coefficients", "-0.044708,-0.032825,-0.124905,-0.021774"]])
+def test_rainbow_generate_record(tmp_path, extra):
+    args = [BIN / "rainbow_generate", "--out", "rec", "--radius-mm", "2.0", "--wavelength-nm", "550",
+            "--inclination-deg", "-30", "--grid", "256", "--query-theta", "180", "--query-phi", "360",
+            "--cdf-theta", "90", "--cdf-phi", "180"] + extra
+    res = _run(args, tmp_path)
+    assert res.returncode == 0, res.stderr
+    m = record_check.check_record(tmp_path / "rec")
+    assert DROP_KEYS <= set(m), DROP_KEYS - set(m)
+    w = ip.water_optics(550.0)
+    assert m["interior_index"] == float(np.float32(w.refractive_index))
+    assert m["material"]["density_kg_m3"] == w.density_kg_m3
+    assert m["stage"] == "incoherent"
+    coeffs = [0.0] * 8 if "--sphere" in extra else (
+        [-0.044708, -0.032825, -0.124905, -0.021774] if extra else ip.drop_shape(2.0))
+    cf = [float(np.float32(c)) for c in coeffs] + [0.0] * (8 - len(coeffs))
+    assert m["shape_coefficients"] == cf[:8]
+    a = ip.drop_projected_area(cf, 2.0, [float(np.float32(v)) for v in ip.drop_incident(-30.0)])
+    assert abs(m["geometry"]["projected_area_mm2"] / a.area_mm2 - 1) < 1e-15
+    # the histogram's hit area estimates the projected area to the grid resolution
+    hist = m["histogram"]
+    hit_area = (hist["rays"] - hist["rays_missed"]) * hist["ray_weight_mm2"]
+    assert abs(hit_area / a.area_mm2 - 1) < 5e-3
+    assert "Saved NumPy CDF dataset" in res.stdout
+    # an existing destination is refused
+    again = _run(args, tmp_path)
+    assert again.returncode == 1 and "already exists" in again.stderr
