@@ -19,6 +19,7 @@ enum StatSlot : uint32_t {
     kStatUnresolved = 15,
     kStatInternalHits = 16,
     kStatDiffraction = 17,   // quanta of the Fraunhofer lobe
+    kStatScratch = 23,       // scratch word of host-requested reductions (probability export)
     kStatCount = 24,
 };
 

@@ -3,6 +3,8 @@
 // context; every entry point is synchronous; errors are returned as codes with
 // the message kept per context (or per thread for context-free calls).
 #include "host_records.hpp"
+#include "runtime_internal.hpp"
+#include "surrogate.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -29,53 +31,7 @@ uint32_t lobe_blocks(uint32_t rows, uint32_t np);
 
 using namespace icephase;
 
-struct iph_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int cus = 256;
-    std::string err;
-    std::vector<iph_batch*> batches;  // live batches; destroyed with the context
-};
-
 namespace {
-
-thread_local std::string g_err;
-
-int fail(iph_ctx* c, int code, const std::string& msg)
-{
-    if (c)
-        c->err = msg;
-    else
-        g_err = msg;
-    return code;
-}
-
-#define IPH_HIP(ctx, call)                                                                                  \
-    do {                                                                                                    \
-        const hipError_t e_ = (call);                                                                       \
-        if (e_ != hipSuccess)                                                                               \
-            return fail((ctx), IPH_E_DEVICE,                                                                \
-                        std::string(#call) + " failed: " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
-                            std::to_string(__LINE__) + ")");                                                \
-    } while (0)
-
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t count)
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = count;
-        if (!count) return hipSuccess;
-        return hipMalloc(&p, count * sizeof(T));
-    }
-};
 
 struct CdfSetup {
     uint32_t ont, onp;
@@ -213,7 +169,7 @@ uint32_t iph_struct_size(int which)
     }
 }
 
-const char* iph_last_error(const iph_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+const char* iph_last_error(const iph_ctx* ctx) { return ctx ? ctx->err.c_str() : thread_error().c_str(); }
 
 int iph_ctx_create(int device, iph_ctx** out)
 {
@@ -245,6 +201,8 @@ void iph_ctx_destroy(iph_ctx* ctx)
     // batches still alive are owned by the context: release them first
     const std::vector<iph_batch*> live = ctx->batches;
     for (iph_batch* b : live) iph_batch_destroy(b);
+    const std::vector<iph_sur*> sur_live = ctx->surrogates;
+    for (iph_sur* s : sur_live) iph_sur_destroy(s);
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
@@ -749,6 +707,20 @@ int iph_write_record(const char* out_dir, const char* metadata_json, const doubl
     if (!make_u_edges(nt, e, err)) return fail(nullptr, IPH_E_INVALID, err);
     const int rc = write_record_dir(out_dir, metadata_json, phi, theta, e, nt, np, err);
     if (rc != IPH_OK) return fail(nullptr, rc, err);
+    return IPH_OK;
+}
+
+int iph_batch_probabilities(iph_batch* b, uint32_t rec, float* d_out)
+{
+    if (!b || !d_out || rec >= b->nrec) return fail(b ? b->ctx : nullptr, IPH_E_INVALID, "Invalid arguments.");
+    if (b->owner[rec] != rec) return node_error(b, rec);
+    iph_ctx* ctx = b->ctx;
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned long long st[kStatCount];
+    IPH_HIP(ctx, hipMemcpy(st, b->host[rec].stats, sizeof st, hipMemcpyDeviceToHost));
+    IPH_HIP(ctx, sur::export_probs(b->host[rec].bins, st[kStatForward], st[kStatBackward], b->host[rec].nt,
+                                   b->host[rec].np, b->host[rec].stats + kStatScratch, d_out, ctx->stream));
+    IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return IPH_OK;
 }
 

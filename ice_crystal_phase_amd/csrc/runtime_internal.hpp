@@ -1,0 +1,69 @@
+// Internals shared by the C-ABI translation units (runtime.hip,
+// surrogate_api.hip): the context, error reporting and device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/icephase.h"
+
+struct iph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    std::string err;
+    std::vector<iph_batch*> batches;  // live batches; destroyed with the context
+    std::vector<iph_sur*> surrogates; // live surrogates; destroyed with the context
+};
+
+namespace icephase {
+
+// message of the last context-free failure on this thread
+inline std::string& thread_error()
+{
+    thread_local std::string s;
+    return s;
+}
+
+inline int fail(iph_ctx* c, int code, const std::string& msg)
+{
+    if (c)
+        c->err = msg;
+    else
+        thread_error() = msg;
+    return code;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t count)
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (!count) return hipSuccess;
+        return hipMalloc(&p, count * sizeof(T));
+    }
+};
+
+}  // namespace icephase
+
+#define IPH_HIP(ctx, call)                                                                                  \
+    do {                                                                                                    \
+        const hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                               \
+            return ::icephase::fail((ctx), IPH_E_DEVICE,                                                    \
+                                    std::string(#call) + " failed: " + hipGetErrorString(e_) + " (" +       \
+                                        __FILE__ + ":" + std::to_string(__LINE__) + ")");                   \
+    } while (0)

@@ -1,0 +1,83 @@
+// Learned surrogate: device-side layout shared by the kernels (surrogate.hip)
+// and the C-ABI host code (surrogate_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace icephase {
+namespace sur {
+
+constexpr int kHidden = 256;
+constexpr int kFourier = 16;
+constexpr int kInputs = 1 + 2 * kFourier + 4;  // 37
+constexpr int kStatic = 1 + 2 * kFourier;      // 33 record-independent inputs
+constexpr int kInPad = 64;                      // input columns padded for MFMA
+
+// fp32 parameters in device layout: W1 [256][64] (columns >= 37 stay zero),
+// b1, then (W [256][256], b [256]) for layers 2..4, w5 [256], b5.
+constexpr uint32_t kOffW1 = 0;
+constexpr uint32_t kOffB1 = 256 * 64;
+constexpr uint32_t kOffW2 = kOffB1 + 256;
+constexpr uint32_t kLayerStride = 65536 + 256;
+constexpr uint32_t kOffW5 = kOffW2 + 3 * kLayerStride;
+constexpr uint32_t kOffB5 = kOffW5 + 256;
+constexpr uint32_t kParams = kOffB5 + 1;
+constexpr uint32_t kParamsPad = (kParams + 255) / 256 * 256;
+// packed (C-ABI) parameters: W1 [256][37] without the padding
+constexpr uint32_t kPacked = kParams - 256 * (kInPad - kInputs);
+
+constexpr uint32_t kLseParts = 256;
+constexpr uint32_t kCompareBlocks = 512;
+constexpr uint32_t kReduceGroups = 32;
+constexpr uint32_t kTileRows = 64;  // rows of a forward/backward workgroup tile
+
+inline uint32_t off_w(int layer) { return layer == 0 ? kOffW1 : kOffW2 + (layer - 1) * kLayerStride; }
+inline uint32_t off_b(int layer) { return layer == 0 ? kOffB1 : kOffW2 + (layer - 1) * kLayerStride + 65536; }
+
+struct Net {
+    const __bf16* w1hi;  // [256][64]
+    const __bf16* w1lo;
+    const __bf16* w[3];   // layers 2..4, [out][in]
+    const __bf16* wt[3];  // transposes, [in][out]
+    const float* b[4];
+    const float* w5;
+    const float* b5;
+};
+
+struct Static {
+    uint32_t nt, np, M, Mpad;
+    const __bf16* xhi;  // [Mpad][64] static inputs, hi part
+    const __bf16* xlo;  //                          lo part
+    const float* logw;  // [Mpad] log du of the cell's row, -inf for padding rows
+    uint32_t rows_per_split, splits;
+};
+
+hipError_t forward(const Net& net, const Static& st, const float* d_conds, uint32_t nrec, bool train, float* logits,
+                   __bf16* Z, hipStream_t s);
+hipError_t lse(const Static& st, const float* logits, uint32_t nrec, float2* part, float* out, hipStream_t s);
+hipError_t dlogits(const Static& st, const float* logits, const float* lse, const float* target, float* dF,
+                   float* loss_part, float* loss, hipStream_t s);
+hipError_t probs(const Static& st, const float* logits, const float* lse, uint32_t nrec, double* mass, float* p,
+                 hipStream_t s);
+// part_db: [4][Mpad/kTileRows][256] fp32 column sums of dZ (bias gradients)
+hipError_t backward(const Net& net, const Static& st, const float* dF, const __bf16* Z, __bf16* dZ, float* part_w5,
+                    float* part_b5, float* part_db, hipStream_t s);
+hipError_t wgrad(const Static& st, int layer, const __bf16* dZl, const __bf16* Zprev, const float* d_cond, float* part,
+                 hipStream_t s);
+// part2: kReduceGroups * n floats of scratch
+hipError_t reduce(const float* part, uint32_t splits, uint32_t n, float* out, bool accumulate, float* part2,
+                  hipStream_t s);
+hipError_t adam(float* w, float* m, float* v, float* g, uint32_t n, float lr, float inv_batch, float c1, float c2,
+                hipStream_t s);
+hipError_t pack(const float* p, __bf16* w1hi, __bf16* w1lo, __bf16* w, __bf16* wt, hipStream_t s);
+// Partial sums of |p - q| and (p - q) * row_cos[row] over kCompareBlocks blocks.
+hipError_t compare(const float* p, const float* q, uint32_t M, uint32_t np, const double* row_cos, double* part,
+                   hipStream_t s);
+// FP32 cell probabilities of a traced record (u64 bins + pole quanta); scratch: one u64.
+hipError_t export_probs(const unsigned long long* bins, unsigned long long qf, unsigned long long qb, uint32_t nt,
+                        uint32_t np, unsigned long long* scratch, float* out, hipStream_t s);
+
+}  // namespace sur
+}  // namespace icephase

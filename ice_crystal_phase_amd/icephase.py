@@ -67,6 +67,10 @@ class Area(C.Structure):
                 ("convexity_cells", u32), ("analytic_sphere", i32), ("pad", i32)]
 
 
+class SurConfig(C.Structure):
+    _fields_ = [("theta_count", u32), ("phi_count", u32), ("max_records", u32), ("dataset", u32), ("seed", u64)]
+
+
 class CdfPolicy(C.Structure):
     _fields_ = [("out_theta", u32), ("out_phi", u32), ("max_l1", f64), ("max_lost_mass", f64),
                 ("max_coarsening_tv", f64)]
@@ -118,6 +122,21 @@ def lib() -> C.CDLL:
     L.iph_drop_shape.argtypes = [C.c_float, C.c_int, P(C.c_float)]
     L.iph_drop_projected_area.argtypes = [P(f64), f64, P(f64), P(Area)]
     L.iph_hex_projected_area.argtypes = [f64, f64, P(f64), P(f64)]
+    L.iph_batch_probabilities.argtypes = [C.c_void_p, u32, C.c_void_p]
+    L.iph_sur_create.argtypes = [C.c_void_p, P(SurConfig), P(C.c_void_p)]
+    L.iph_sur_destroy.argtypes = [C.c_void_p]
+    L.iph_sur_destroy.restype = None
+    L.iph_sur_param_count.restype = u32
+    L.iph_sur_get_params.argtypes = [C.c_void_p, C.c_void_p]
+    L.iph_sur_set_params.argtypes = [C.c_void_p, C.c_void_p]
+    L.iph_sur_set_record.argtypes = [C.c_void_p, u32, C.c_void_p, u32, P(C.c_float)]
+    L.iph_sur_set_target.argtypes = [C.c_void_p, u32, C.c_void_p, P(C.c_float)]
+    L.iph_sur_gradient.argtypes = [C.c_void_p, u32, P(C.c_float), C.c_void_p]
+    L.iph_sur_train_step.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_float, C.c_void_p]
+    L.iph_sur_predict.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_void_p]
+    L.iph_sur_compare.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.iph_sur_records.argtypes = [C.c_void_p, C.c_void_p, u32, P(CdfPolicy), C.c_void_p, C.c_void_p, C.c_void_p]
+    L.iph_sur_timing.argtypes = [C.c_void_p, P(f64)]
     for which, (name, st) in enumerate((("Record", Record), ("HistStats", HistStats), ("CdfReport", CdfReport),
                                         ("CdfPolicy", CdfPolicy), ("WaterState", WaterState), ("Area", Area))):
         if L.iph_struct_size(which) != C.sizeof(st):
@@ -140,7 +159,10 @@ EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_des
             "iph_u_edges", "iph_bin_directions", "iph_batch_create", "iph_batch_destroy", "iph_batch_run",
             "iph_batch_set_schedule", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
             "iph_build_cdf", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
-            "iph_drop_projected_area", "iph_hex_projected_area")
+            "iph_drop_projected_area", "iph_hex_projected_area", "iph_batch_probabilities", "iph_sur_create",
+            "iph_sur_destroy", "iph_sur_param_count", "iph_sur_get_params", "iph_sur_set_params",
+            "iph_sur_set_record", "iph_sur_set_target", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",
+            "iph_sur_compare", "iph_sur_records", "iph_sur_timing")
 
 
 def u_edges(nt: int) -> np.ndarray:
@@ -365,6 +387,10 @@ class Batch:
         out = np.zeros(int(r.theta_count) * int(r.phi_count), np.uint64)
         _check(lib().iph_batch_bins(self._p, rec, out.ctypes.data), self.ctx.ptr)
         return out
+
+    @property
+    def ptr(self):
+        return self._p
 
     def cdf(self, rec: int = 0):
         r = self.records[rec]

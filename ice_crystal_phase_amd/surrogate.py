@@ -1,0 +1,232 @@
+"""Learned surrogate of the phase-function records (host side of the
+iph_sur_* C-ABI in include/icephase.h).
+
+A conditional density MLP over the (u, phi) cells, conditioned on a record's
+(inclination deg, wavelength nm, equal-volume radius mm, aspect ratio), trained
+on the GPU by the library's BF16 MFMA kernels on the cell probabilities of
+traced records (copied device to device into the surrogate's training set),
+and served as FP64 CDF records by the same CDF kernels as traced records. The
+plain tracer stays the judge: `compare` measures total variation and |g| error
+against traced records on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import icephase as icp
+
+
+# ---------------------------------------------------------------- drop family
+def drop_aspect(coeffs) -> float:
+    """Axis ratio of the drop r(chi): vertical semi-extent (r(mu=1) + r(mu=-1))/2
+    over the horizontal semi-extent max r sin chi."""
+    c = list(coeffs) + [0.0] * (8 - len(coeffs))
+    chi = np.linspace(0.0, np.pi, 20001)
+    r = 1 + c[0] + sum(c[n] * np.cos(n * chi) for n in range(1, 8))
+    vert = 0.5 * (r[0] + r[-1])
+    return float(vert / np.max(r * np.sin(chi)))
+
+
+def drop_conditions(n: int, seed: int, alpha=(-90.0, 90.0), wavelength=(400.0, 700.0),
+                    radius=(0.25, 3.0)) -> np.ndarray:
+    """n seeded records of the water-drop family: (alpha, lambda, radius, aspect),
+    the aspect that of the Table I shape of the radius."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(*alpha, n)
+    lam = rng.uniform(*wavelength, n)
+    r = rng.uniform(*radius, n)
+    asp = np.array([drop_aspect(icp.drop_shape(float(np.float32(x)))) for x in r])
+    return np.stack([a, lam, r, asp], axis=1).astype(np.float32)
+
+
+def drop_records(conds: np.ndarray, grid: int, theta: int, phi: int, max_internal_hits: int = 9):
+    """Tracer records of the family (Table I shape, IAPWS water index at 20 C)."""
+    recs = []
+    for a, lam, r, _ in np.asarray(conds, np.float64):
+        n = icp.water_optics(float(np.float32(lam))).refractive_index
+        coeffs = icp.drop_shape(float(np.float32(r)))
+        if all(v == 0 for v in coeffs):
+            rec = icp.sphere_record(n, grid, theta, phi, radius_mm=float(r), inclination_deg=float(a),
+                                    max_internal_hits=max_internal_hits, wavelength_nm=float(lam))
+        else:
+            rec = icp.drop_record(coeffs, n, grid, theta, phi, radius_mm=float(r), inclination_deg=float(a),
+                                  max_internal_hits=max_internal_hits, wavelength_nm=float(lam))
+        recs.append(rec)
+    return recs
+
+
+# ---------------------------------------------------------------- network
+class Surrogate:
+    """Device state of one network and its training set (iph_sur)."""
+
+    def __init__(self, ctx: icp.Context, theta: int = 450, phi: int = 900, max_records: int = 64,
+                 dataset: int = 0, seed: int = 5):
+        self.ctx = ctx
+        self.theta, self.phi, self.max_records, self.dataset = theta, phi, max_records, dataset
+        self.M = theta * phi
+        self._p = C.c_void_p()
+        cfg = icp.SurConfig(theta, phi, max_records, dataset, seed)
+        icp._check(icp.lib().iph_sur_create(ctx.ptr, C.byref(cfg), C.byref(self._p)), ctx.ptr)
+
+    def close(self):
+        if self._p and self.ctx.ptr:
+            icp.lib().iph_sur_destroy(self._p)
+        self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def param_count() -> int:
+        return int(icp.lib().iph_sur_param_count())
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.param_count(), np.float32)
+        icp._check(icp.lib().iph_sur_get_params(self._p, out.ctypes.data), self.ctx.ptr)
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.param_count()
+        icp._check(icp.lib().iph_sur_set_params(self._p, p.ctypes.data), self.ctx.ptr)
+
+    def set_record(self, slot: int, batch: icp.Batch, rec: int, cond) -> None:
+        c = (C.c_float * 4)(*[float(x) for x in cond])
+        icp._check(icp.lib().iph_sur_set_record(self._p, slot, batch.ptr, rec, c), self.ctx.ptr)
+
+    def set_target(self, slot: int, probs: np.ndarray, cond) -> None:
+        probs = np.ascontiguousarray(probs, np.float32).reshape(-1)
+        assert probs.size == self.M
+        c = (C.c_float * 4)(*[float(x) for x in cond])
+        icp._check(icp.lib().iph_sur_set_target(self._p, slot, probs.ctypes.data, c), self.ctx.ptr)
+
+    def gradient(self, slot: int):
+        """Loss and packed gradient of one training slot."""
+        loss = C.c_float()
+        g = np.zeros(self.param_count(), np.float32)
+        icp._check(icp.lib().iph_sur_gradient(self._p, slot, C.byref(loss), g.ctypes.data), self.ctx.ptr)
+        return float(loss.value), g
+
+    def train_step(self, slots, lr: float) -> np.ndarray:
+        slots = np.ascontiguousarray(slots, np.uint32)
+        loss = np.zeros(slots.size, np.float32)
+        icp._check(icp.lib().iph_sur_train_step(self._p, slots.ctypes.data, slots.size, lr, loss.ctypes.data),
+                   self.ctx.ptr)
+        return loss
+
+    def predict(self, conds: np.ndarray) -> np.ndarray:
+        """Cell probabilities [n, M] of n <= max_records records (host)."""
+        conds = np.ascontiguousarray(conds, np.float32).reshape(-1, 4)
+        out = np.zeros((conds.shape[0], self.M), np.float32)
+        icp._check(icp.lib().iph_sur_predict(self._p, conds.ctypes.data, conds.shape[0], out.ctypes.data),
+                   self.ctx.ptr)
+        return out
+
+    def compare(self, conds: np.ndarray, batch: icp.Batch, recs) -> tuple[np.ndarray, np.ndarray]:
+        """(TV, |dg|) of the predictions against traced records of a batch."""
+        conds = np.ascontiguousarray(conds, np.float32).reshape(-1, 4)
+        n = conds.shape[0]
+        r = np.ascontiguousarray(recs, np.uint32)
+        tv = np.zeros(n)
+        dg = np.zeros(n)
+        icp._check(icp.lib().iph_sur_compare(self._p, conds.ctypes.data, n, batch.ptr, r.ctypes.data, tv.ctypes.data,
+                                             dg.ctypes.data), self.ctx.ptr)
+        return tv, dg
+
+    def records(self, conds: np.ndarray, out_theta: int = 0, out_phi: int = 0):
+        """Served records: FP64 CDF arrays (phi [n, onp+1], theta [n, onp, ont+1])."""
+        conds = np.ascontiguousarray(conds, np.float32).reshape(-1, 4)
+        n = conds.shape[0]
+        ont, onp = out_theta or self.theta, out_phi or self.phi
+        phi = np.zeros((n, onp + 1), np.float64)
+        theta = np.zeros((n, onp, ont + 1), np.float64)
+        reps = (icp.CdfReport * n)()
+        pol = icp.CdfPolicy(out_theta, out_phi, 1e-10, 1e-12, 1.0)
+        icp._check(icp.lib().iph_sur_records(self._p, conds.ctypes.data, n, C.byref(pol), phi.ctypes.data,
+                                             theta.ctypes.data, reps), self.ctx.ptr)
+        return phi, theta, list(reps)
+
+    def timing(self):
+        ms = (icp.f64 * 3)()
+        icp._check(icp.lib().iph_sur_timing(self._p, ms))
+        return ms[0], ms[1], ms[2]
+
+
+def fill_training_set(sur: Surrogate, conds: np.ndarray, grid: int, chunk: int = 64,
+                      max_internal_hits: int = 9) -> float:
+    """Trace every record of `conds` and copy its cell probabilities into the
+    surrogate's training slots 0..n-1. Returns the device trace time (ms)."""
+    ms = 0.0
+    for k0 in range(0, len(conds), chunk):
+        recs = drop_records(conds[k0:k0 + chunk], grid, sur.theta, sur.phi, max_internal_hits)
+        with icp.Batch(sur.ctx, recs) as b:
+            b.run(icp.RUN_TRACE | icp.RUN_GRAPH)
+            ms += b.timing()[0]
+            for i in range(len(recs)):
+                sur.set_record(k0 + i, b, i, conds[k0 + i])
+    return ms
+
+
+def evaluate(sur: Surrogate, conds: np.ndarray, grid: int, max_internal_hits: int = 9):
+    """TV and |dg| of the surrogate against fresh traced records; also the
+    device time the tracer spent on them (ms)."""
+    tv, dg = [], []
+    ms = 0.0
+    for k0 in range(0, len(conds), sur.max_records):
+        c = conds[k0:k0 + sur.max_records]
+        recs = drop_records(c, grid, sur.theta, sur.phi, max_internal_hits)
+        with icp.Batch(sur.ctx, recs) as b:
+            b.run(icp.RUN_TRACE | icp.RUN_GRAPH)
+            ms += b.timing()[0]
+            t, g = sur.compare(c, b, list(range(len(recs))))
+        tv.append(t)
+        dg.append(g)
+    return np.concatenate(tv), np.concatenate(dg), ms
+
+
+@dataclass
+class TrainReport:
+    epochs: int
+    steps: int
+    seconds: float
+    final_loss: float
+    losses: list
+
+
+def train(sur: Surrogate, n_records: int, epochs: int, batch: int = 8, lr: float = 2e-3, lr_min: float = 2e-5,
+          seed: int = 0, time_limit: float | None = None, log=None) -> TrainReport:
+    """Adam with a cosine learning-rate schedule over the shuffled training
+    slots 0..n_records-1; with a time limit the schedule follows the elapsed
+    fraction of it (whichever of steps and time is further along)."""
+    steps_per_epoch = max(1, n_records // batch)
+    total = epochs * steps_per_epoch
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    losses = []
+    step = 0
+    lr_t = lr
+    for ep in range(epochs):
+        order = rng.permutation(n_records).astype(np.uint32)
+        ep_loss = 0.0
+        for s in range(steps_per_epoch):
+            frac = step / max(1, total - 1)
+            if time_limit:
+                frac = max(frac, min(1.0, (time.time() - t0) / time_limit))
+            lr_t = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * frac))
+            l = sur.train_step(order[s * batch:(s + 1) * batch], lr_t)
+            ep_loss += float(l.mean())
+            step += 1
+        losses.append(ep_loss / steps_per_epoch)
+        if log:
+            log(f"epoch {ep + 1}/{epochs} loss {losses[-1]:.6f} lr {lr_t:.2e} t {time.time() - t0:.1f}s")
+        if time_limit is not None and time.time() - t0 > time_limit:
+            break
+    return TrainReport(len(losses), step, time.time() - t0, losses[-1], losses)

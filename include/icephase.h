@@ -127,6 +127,7 @@ typedef struct iph_cdf_report {
 
 typedef struct iph_ctx iph_ctx;
 typedef struct iph_batch iph_batch;
+typedef struct iph_sur iph_sur;
 
 /* Replaces the ABI sentinel kernels and sizeof fingerprints
  * (device/phase_cdf.cu:65,240; hex_trace/device/hex_trace.cu:6-11;
@@ -209,6 +210,63 @@ int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, co
  * (src/phase_cdf.cpp:482-493). Refuses an existing out_dir or .part. */
 int iph_write_record(const char* out_dir, const char* metadata_json, const double* phi, const double* theta,
                      uint32_t nt, uint32_t np);
+
+/* Device export of record rec's cell probabilities (FP32, theta-major
+ * theta_count*phi_count, pole quanta spread over the end rows, normalized to
+ * one) into a device buffer of the same context (the surrogate's training
+ * set uses it). No reference counterpart; the probabilities are the cell masses
+ * PhaseCdf::build stores, normalized. */
+int iph_batch_probabilities(iph_batch* b, uint32_t rec, float* d_out);
+
+/* ---- learned surrogate (no reference counterpart: the reference has no
+ * learned part, README.md:31) ------------------------------------------------
+ * A conditional density MLP over the (u, phi) cells of a theta_count x
+ * phi_count grid, conditioned on one record's (inclination deg, wavelength nm,
+ * equal-volume radius mm, aspect ratio): 37 inputs (theta/pi, 16 Fourier pairs
+ * of the azimuth, 4 conditions), 4 hidden layers of 256 (SiLU), one log
+ * density per cell, softmax over the cells. BF16 MFMA kernels with FP32
+ * accumulation for training (forward, backward, Adam) and serving. Training
+ * records live in the surrogate's own device memory, filled from traced
+ * records; the tracer stays the judge (iph_sur_compare). */
+typedef struct iph_sur_config {
+    uint32_t theta_count, phi_count;
+    uint32_t max_records;      /* records per predict / records / compare call */
+    uint32_t dataset;          /* training-set slots (device memory) */
+    uint64_t seed;             /* parameter initialization */
+} iph_sur_config;
+int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out);
+void iph_sur_destroy(iph_sur* s);
+/* Packed FP32 parameters: W1 [256][37], b1 [256], (W [256][256], b [256]) x 3,
+ * w5 [256], b5 [1] (row-major [out][in]). */
+uint32_t iph_sur_param_count(void);
+int iph_sur_get_params(iph_sur* s, float* packed);
+int iph_sur_set_params(iph_sur* s, const float* packed);
+/* Training-set slot from a traced record of a batch of the same context, or
+ * from host probabilities (theta_count*phi_count, summing to one). cond: alpha
+ * deg, wavelength nm, radius mm, aspect. */
+int iph_sur_set_record(iph_sur* s, uint32_t slot, iph_batch* b, uint32_t rec, const float cond[4]);
+int iph_sur_set_target(iph_sur* s, uint32_t slot, const float* host_probs, const float cond[4]);
+/* Cross-entropy loss of one slot and its parameter gradient (packed, host;
+ * may be NULL). */
+int iph_sur_gradient(iph_sur* s, uint32_t slot, float* loss, float* packed_grad);
+/* One Adam step (beta 0.9/0.999) on the mean gradient of n slots; loss[n]
+ * receives their losses (may be NULL). */
+int iph_sur_train_step(iph_sur* s, const uint32_t* slots, uint32_t n, float lr, float* loss);
+/* Cell probabilities of n <= max_records records (host array n*theta*phi). */
+int iph_sur_predict(iph_sur* s, const float* conds, uint32_t n, float* probs);
+/* Predictions of n records against traced records recs[] of batch b on the
+ * device: tv[k] = 0.5 sum |p - q|, dg[k] = |g(p) - g(q)| with the exact cell
+ * mean cosine 1 - (u_i + u_{i+1}). */
+int iph_sur_compare(iph_sur* s, const float* conds, uint32_t n, iph_batch* b, const uint32_t* recs, double* tv,
+                    double* dg);
+/* Records served by the surrogate: probabilities, then the FP64 CDF arrays of
+ * each record on the device (the same CDF kernels as traced records), copied to
+ * host arrays phi [n][onp+1], theta [n][onp][ont+1]; reports[n] may be NULL. */
+int iph_sur_records(iph_sur* s, const float* conds, uint32_t n, const iph_cdf_policy* policy, double* phi,
+                    double* theta, iph_cdf_report* reports);
+/* Device time of the last calls: ms[0] network part of predict/records, ms[1]
+ * whole iph_sur_records (network + CDF), ms[2] last training step. */
+int iph_sur_timing(const iph_sur* s, double* ms /* 3 */);
 
 /* ---- host material and geometry services (no device work) ---------------- */
 

@@ -1,0 +1,117 @@
+"""CPU oracle of the learned surrogate's network, FP64 numpy - TEST
+INFRASTRUCTURE ONLY (the product never imports it; only tests/ and the
+bench's checks do).
+
+The reference has no learned part (README.md:31 of the reference says
+learning belongs to another project), so this restates the network the
+library defines (ice_crystal_phase_amd/csrc/surrogate.hip): inputs
+[t, sin(2 pi k v), cos(2 pi k v) (k=1..16), c0..c3], four SiLU layers of
+256, one log density per cell, softmax over the cells of f + log du_i, and the
+cross-entropy loss -sum P log p. Pinned by central finite differences
+(tests/test_surrogate_oracle.py), which fix every term of the gradient.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+H = 256
+NF = 16
+NIN = 1 + 2 * NF + 4
+
+
+def u_edges(nt: int) -> np.ndarray:
+    i = np.arange(nt + 1)
+    k = np.minimum(i, nt - i)
+    s = np.sin(0.5 * np.pi * k / nt) ** 2
+    return np.where(i <= nt // 2, s, 1.0 - s)
+
+
+def normalize_cond(cond) -> np.ndarray:
+    a, lam, r, asp = (float(x) for x in cond)
+    return np.array([a / 90.0, (lam - 550.0) / 150.0, (r - 1.5) / 1.5, np.log2(asp)])
+
+
+def features(nt: int, np_: int, cond) -> np.ndarray:
+    t = (np.arange(nt) + 0.5) / nt
+    v = (np.arange(np_) + 0.5) / np_
+    T, V = np.meshgrid(t, v, indexing="ij")
+    k = np.arange(1, NF + 1)
+    ang = 2 * np.pi * V.reshape(-1, 1) * k
+    c = np.broadcast_to(normalize_cond(cond), (nt * np_, 4))
+    return np.concatenate([T.reshape(-1, 1), np.sin(ang), np.cos(ang), c], axis=1)
+
+
+def log_du(nt: int, np_: int) -> np.ndarray:
+    return np.repeat(np.log(np.diff(u_edges(nt))), np_)
+
+
+def unpack(p: np.ndarray):
+    p = np.asarray(p, np.float64)
+    o = 0
+    W = []
+    b = []
+    for fan_in in (NIN, H, H, H):
+        W.append(p[o:o + H * fan_in].reshape(H, fan_in))
+        o += H * fan_in
+        b.append(p[o:o + H])
+        o += H
+    w5 = p[o:o + H]
+    b5 = p[o + H]
+    assert o + H + 1 == p.size
+    return W, b, w5, b5
+
+
+def param_count() -> int:
+    return H * NIN + H + 3 * (H * H + H) + H + 1
+
+
+def silu(z):
+    return z / (1 + np.exp(-z))
+
+
+def forward(p, X, lw):
+    W, b, w5, b5 = unpack(p)
+    h = X
+    zs, hs = [], [X]
+    for l in range(4):
+        z = h @ W[l].T + b[l]
+        h = silu(z)
+        zs.append(z)
+        hs.append(h)
+    f = h @ w5 + b5 + lw
+    return f, zs, hs
+
+
+def probs(p, nt, np_, cond) -> np.ndarray:
+    f, _, _ = forward(p, features(nt, np_, cond), log_du(nt, np_))
+    f = f - f.max()
+    e = np.exp(f)
+    return e / e.sum()
+
+
+def loss_and_grad(p, nt, np_, cond, target):
+    """Cross entropy -sum P log softmax(f) and its gradient (packed layout)."""
+    X = features(nt, np_, cond)
+    lw = log_du(nt, np_)
+    f, zs, hs = forward(p, X, lw)
+    m = f.max()
+    lse = m + np.log(np.exp(f - m).sum())
+    P = np.asarray(target, np.float64).reshape(-1)
+    loss = float(np.sum(P[P > 0] * (lse - f[P > 0])))
+    dF = np.exp(f - lse) - P
+    W, b, w5, b5 = unpack(p)
+    gW, gb = [None] * 4, [None] * 4
+    gw5 = hs[4].T @ dF
+    gb5 = dF.sum()
+    dh = np.outer(dF, w5)
+    for l in range(3, -1, -1):
+        s = 1 / (1 + np.exp(-zs[l]))
+        dz = dh * s * (1 + zs[l] * (1 - s))
+        gW[l] = dz.T @ hs[l]
+        gb[l] = dz.sum(axis=0)
+        dh = dz @ W[l]
+    out = []
+    for l in range(4):
+        out += [gW[l].reshape(-1), gb[l]]
+    out += [gw5, np.array([gb5])]
+    return loss, np.concatenate(out)

@@ -1,0 +1,149 @@
+"""GPU tests of the learned surrogate: the BF16 MFMA forward and backward
+kernels against the FP64 CPU oracle (oracle/surrogate_ref.py), the device
+export of traced records, served CDF records, and a short training run that
+must lower the loss and beat the untrained network and the family mean.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import ice_crystal_phase_amd as ip
+from ice_crystal_phase_amd import surrogate as S
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT / "oracle"))
+import surrogate_ref as ref  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _segments():
+    """(name, start, stop) of the packed parameter tensors."""
+    out, o = [], 0
+    for l, fan in enumerate((37, 256, 256, 256)):
+        out.append((f"W{l + 1}", o, o + 256 * fan))
+        o += 256 * fan
+        out.append((f"b{l + 1}", o, o + 256))
+        o += 256
+    out += [("w5", o, o + 256), ("b5", o + 256, o + 257)]
+    return out
+
+
+def test_gradient_matches_fp64_oracle(ctx):
+    print("case: surrogate gradient 45x90", flush=True)
+    nt, np_ = 45, 90
+    rng = np.random.default_rng(11)
+    tgt = (rng.random(nt * np_) ** 4).astype(np.float32)
+    tgt /= tgt.sum()
+    cond = (35.0, 520.0, 2.1, 0.82)
+    with S.Surrogate(ctx, nt, np_, max_records=2, dataset=1, seed=3) as sur:
+        p = sur.params().astype(np.float64)
+        sur.set_target(0, tgt, cond)
+        loss, g = sur.gradient(0)
+        l_ref, g_ref = ref.loss_and_grad(p, nt, np_, np.float32(cond).astype(np.float64), tgt.astype(np.float64))
+        print(f"loss {loss:.6f} oracle {l_ref:.6f}", flush=True)
+        assert abs(loss / l_ref - 1) < 2e-3, (loss, l_ref)
+        # BF16 activations and operands (8-bit significands) bound the agreement:
+        # weight gradients (sums of products over all cells) to ~1e-2, bias
+        # gradients (column sums of dZ with strong cancellation) to ~1e-1
+        for name, a, b in _segments():
+            if name == "b5":
+                # softmax is shift invariant: the gradient sum(p - P) = 1 - sum(P) ~ 0
+                assert abs(g_ref[a]) < 1e-6 and abs(g[a]) < 1e-4
+                continue
+            x, y = g[a:b].astype(np.float64), g_ref[a:b]
+            rel = np.linalg.norm(x - y) / np.linalg.norm(y)
+            cos = float(x @ y / (np.linalg.norm(x) * np.linalg.norm(y)))
+            print(f"{name}: rel {rel:.2e} cos {cos:.6f}", flush=True)
+            tol, ctol = (2e-2, 0.9998) if name.startswith("W") or name == "w5" else (1e-1, 0.995)
+            assert rel < tol and cos > ctol, name
+        total = np.linalg.norm(g.astype(np.float64) - g_ref) / np.linalg.norm(g_ref)
+        print(f"whole gradient: rel {total:.2e}", flush=True)
+        assert total < 1e-2
+        q = sur.predict(np.array([cond], np.float32))[0].astype(np.float64)
+        q_ref = ref.probs(p, nt, np_, np.float32(cond).astype(np.float64))
+        print(f"forward TV vs oracle {0.5 * np.abs(q - q_ref).sum():.2e}", flush=True)
+        assert 0.5 * np.abs(q - q_ref).sum() < 5e-3
+        assert abs(q.sum() - 1) < 1e-5
+
+
+def test_training_set_from_traced_record(ctx, oracle):
+    """A traced record copied into a training slot holds its normalized cell
+    probabilities: the gradient of that slot equals the gradient of the same
+    probabilities uploaded from the host."""
+    rec = ip.sphere_record(1.3346, 128, 45, 90, inclination_deg=20.0)
+    cond = (20.0, 550.0, 1.0, 1.0)
+    with ip.Batch(ctx, [rec]) as b, S.Surrogate(ctx, 45, 90, max_records=2, dataset=2, seed=4) as sur:
+        b.run(ip.RUN_TRACE)
+        bins = b.bins(0)
+        st = b.stats(0)
+        m = oracle.bins_to_mass(bins, st.q_forward, st.q_backward, 45, 90)
+        want = (m / m.sum()).astype(np.float32)
+        sur.set_record(0, b, 0, cond)
+        sur.set_target(1, want, cond)
+        l0, g0 = sur.gradient(0)
+        l1, g1 = sur.gradient(1)
+        assert abs(l0 - l1) <= 1e-6 * abs(l1)
+        assert np.allclose(g0, g1, rtol=1e-5, atol=1e-7)
+        tv, dg = sur.compare(np.array([cond], np.float32), b, [0])
+        q = sur.predict(np.array([cond], np.float32))[0].astype(np.float64)
+        assert abs(tv[0] - 0.5 * np.abs(q - want).sum()) < 1e-6
+
+
+def test_served_records_are_valid_cdfs(ctx):
+    nt, np_ = 90, 180
+    conds = np.array([[0.0, 550.0, 1.0, 1.0], [45.0, 450.0, 2.5, 0.75], [-80.0, 690.0, 0.3, 1.0]], np.float32)
+    with S.Surrogate(ctx, nt, np_, max_records=4, seed=9) as sur:
+        phi, theta, reps = sur.records(conds)
+        q = sur.predict(conds).astype(np.float64)
+    for r in range(3):
+        assert phi[r, 0] == 0 and phi[r, -1] == 1 and np.all(np.diff(phi[r]) >= 0)
+        assert np.all(theta[r][:, 0] == 0) and np.all(theta[r][:, -1] == 1)
+        assert reps[r].status == 0 and reps[r].l1_error < 1e-10
+        cells = (np.diff(phi[r])[:, None] * np.diff(theta[r], axis=1)).T.reshape(-1)
+        assert np.allclose(cells, q[r] / q[r].sum(), rtol=1e-5, atol=1e-12)
+
+
+def test_training_lowers_loss_and_beats_baselines(ctx):
+    """A short run on traced drop records (90 x 180 cells): the loss falls, and
+    the trained network's TV against fresh traced records is below the untrained
+    network's and below the family-mean record's."""
+    print("case: surrogate training 256 records 90x180", flush=True)
+    nt, np_ = 90, 180
+    conds = S.drop_conditions(256, seed=5, radius=(1.0, 2.5))
+    fresh = S.drop_conditions(8, seed=6, radius=(1.0, 2.5))
+    with S.Surrogate(ctx, nt, np_, max_records=8, dataset=256, seed=1) as sur:
+        S.fill_training_set(sur, conds, 512)
+        tv0, _, _ = S.evaluate(sur, fresh, 1024)
+        rep = S.train(sur, 256, epochs=12, batch=8, lr=3e-3, lr_min=1e-4, seed=2)
+        tv1, dg1, _ = S.evaluate(sur, fresh, 1024)
+        # the family mean as a predictor: mean of the training targets
+        mean = np.zeros(nt * np_)
+        recs = S.drop_records(conds[:64], 512, nt, np_)
+        with ip.Batch(ctx, recs) as b:
+            b.run(ip.RUN_TRACE)
+            for i in range(len(recs)):
+                st = b.stats(i)
+                m = np.asarray(b.bins(i), np.float64)
+                m[:np_] += st.q_forward / np_
+                m[-np_:] += st.q_backward / np_
+                mean += m / m.sum()
+        mean /= len(recs)
+        recs_f = S.drop_records(fresh, 1024, nt, np_)
+        tvm = []
+        with ip.Batch(ctx, recs_f) as b:
+            b.run(ip.RUN_TRACE)
+            for i in range(len(recs_f)):
+                st = b.stats(i)
+                m = np.asarray(b.bins(i), np.float64)
+                m[:np_] += st.q_forward / np_
+                m[-np_:] += st.q_backward / np_
+                tvm.append(0.5 * np.abs(m / m.sum() - mean).sum())
+    print(f"losses {rep.losses[0]:.4f} -> {rep.final_loss:.4f} in {rep.seconds:.1f}s; TV untrained {tv0.mean():.4f}, "
+          f"family mean {np.mean(tvm):.4f}, trained {tv1.mean():.4f} (|dg| {dg1.mean():.4f})", flush=True)
+    assert rep.final_loss < rep.losses[0]
+    assert tv1.mean() < tv0.mean() and tv1.mean() < np.mean(tvm)

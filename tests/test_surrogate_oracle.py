@@ -1,0 +1,68 @@
+"""CPU pins of the surrogate's FP64 oracle (oracle/surrogate_ref.py): central
+finite differences fix every gradient term; the parameter count and packed
+layout agree with the library's C-ABI (no device call)."""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT / "oracle"))
+import surrogate_ref as ref  # noqa: E402
+
+import ice_crystal_phase_amd as ip  # noqa: E402
+
+
+def _params(seed: int, scale: float = 1.0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-1, 1, ref.param_count())
+    p[: 256 * ref.NIN] /= np.sqrt(ref.NIN)
+    p[256 * ref.NIN:] /= 16.0
+    return p * scale
+
+
+def test_param_count_matches_library():
+    assert ip.lib().iph_sur_param_count() == ref.param_count() == 207361 - 256 * (64 - 37) + 256 * (64 - 37)
+
+
+def test_features_and_weights():
+    X = ref.features(6, 8, (30.0, 550.0, 1.5, 1.0))
+    assert X.shape == (48, 37)
+    assert np.allclose(X[:, 0].reshape(6, 8)[:, 0], (np.arange(6) + 0.5) / 6)
+    assert np.allclose(X[:, 33:], [30 / 90, 0.0, 0.0, 0.0])
+    lw = ref.log_du(6, 8)
+    assert np.isclose(np.exp(lw).reshape(6, 8)[:, 0].sum(), 1.0)
+
+
+def test_gradient_matches_finite_differences():
+    nt, np_ = 6, 8
+    rng = np.random.default_rng(7)
+    target = rng.random(nt * np_) ** 3
+    target /= target.sum()
+    cond = (-40.0, 470.0, 2.2, 0.83)
+    p = _params(1)
+    loss, g = ref.loss_and_grad(p, nt, np_, cond, target)
+    assert np.isfinite(loss) and g.shape == p.shape
+    idx = np.concatenate([rng.choice(256 * 37, 12, replace=False),                  # W1
+                          256 * 37 + rng.choice(256, 4, replace=False),             # b1
+                          256 * 38 + rng.choice(65536 + 256, 12, replace=False),    # layer 2
+                          256 * 38 + 2 * 65792 + rng.choice(65792, 8, replace=False),  # layer 4
+                          ref.param_count() - 257 + rng.choice(257, 6, replace=False)])  # w5, b5
+    h = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (ref.loss_and_grad(pp, nt, np_, cond, target)[0] - ref.loss_and_grad(pm, nt, np_, cond, target)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_probabilities_normalized_and_loss_is_cross_entropy():
+    nt, np_ = 5, 7
+    p = _params(2)
+    q = ref.probs(p, nt, np_, (10.0, 600.0, 1.0, 1.0))
+    assert np.isclose(q.sum(), 1.0) and np.all(q > 0)
+    loss, _ = ref.loss_and_grad(p, nt, np_, (10.0, 600.0, 1.0, 1.0), q)
+    assert np.isclose(loss, -np.sum(q * np.log(q)))
