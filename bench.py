@@ -8,6 +8,21 @@ record on a stratified grid, 64 inclinations evenly spaced in [-90, 90] deg x
 up to 9 internal interactions per ray, 900 x 1800 (theta, phi) cells, FP32
 tracing, FP64 CDF; zero + trace + CDF replayed as one hipGraph per step.
 
+Beside the headline figure the line carries the other configurations of the
+metric, each measured on the device with inputs resident in HBM:
+  hex        hexagonal ice column (n = 1.31, a = 50 um, c/a = 2), 2^28 rays,
+             <= 12 internal interactions, 900 x 1800 cells, FP64 tracing;
+  sphere     water sphere r = 1 mm at 550 nm, 2048^2 rays, <= 8 internal hits;
+  surrogate  the learned surrogate (trained weights in
+             ice_crystal_phase_amd/data/surrogate_drop_450x900.npy, when
+             present) serving 450 x 900 CDF records, with its total variation
+             against fresh traced records and records/s against tracing;
+  cpu_baseline      the drop port (FP32 oracle mirror) on one host core;
+  cpu_baseline_hex  the compiled reference hex tracer + compensated histogram
+                    (oracle/_ref) on one host core.
+--config uneven runs the 512-record uneven batch instead (records/s, with the
+autotuned schedule).
+
 Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -25,9 +40,6 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# Water refractive index, IAPWS R9-97 with IAPWS-95 density at 20 C and
-# 101325 Pa, at the three workload wavelengths.
-WATER_20C = {450.0: 1.339607940680526, 550.0: 1.3346825974540153, 650.0: 1.3316678740505974}
 # Mean Table-I row at 2 mm (include/rainbow/raindrop_shape.hpp:29-34 gives
 # c0..c3 = -0.0458, -0.0335, -0.1211, -0.0227 at 2.0 mm); the workload draws
 # +-5 % around it with seed 2 and checks convexity in the library.
@@ -54,7 +66,8 @@ def workload(records: int, rays_log2: int, nt: int, np_: int):
     for lam in (450.0, 550.0, 650.0):
         for k in range(n_alpha):
             alpha = -90.0 + (k + 0.5) * 180.0 / n_alpha
-            recs.append(ip.drop_record(coeffs, WATER_20C[lam], side, nt, np_, radius_mm=2.0,
+            n_water = ip.water_optics(lam).refractive_index  # IAPWS R9-97 / IAPWS-95, 20 C, 101325 Pa
+            recs.append(ip.drop_record(coeffs, n_water, side, nt, np_, radius_mm=2.0,
                                        inclination_deg=alpha, max_internal_hits=9, wavelength_nm=lam))
             if len(recs) == records:
                 return recs, coeffs, side
@@ -92,6 +105,127 @@ def cpu_baseline(rec, nt: int, np_: int, budget_s: float = 12.0):
     return n2 / dt, n2, dt
 
 
+SURROGATE_WEIGHTS = ROOT / "ice_crystal_phase_amd" / "data" / "surrogate_drop_450x900.npy"
+
+
+def run_hex(ctx, rays_log2: int = 28):
+    """Hexagonal column (n 1.31, a 50 um, c/a 2, ki (1,-0.4,0.3)), Fraunhofer
+    lobe added, 2^rays_log2 rays, <= 12 internal interactions, 900x1800."""
+    import ice_crystal_phase_amd as ip
+    rec = ip.hex_record(0.05, 0.2, 1.31, 1 << rays_log2, incident=(1.0, -0.4, 0.3), seed=12345, theta=900,
+                        phi=1800, max_internal_hits=12, tail=1e-12, wavelength_nm=550.0, diffraction=True)
+    with ip.Batch(ctx, [rec]) as b:
+        flags = ip.RUN_TRACE | ip.RUN_CDF | ip.RUN_GRAPH
+        b.run(flags)
+        b.run(flags)
+        tr, cdf, _ = b.timing()
+        st = b.stats(0)
+    return {"rays": 1 << rays_log2, "rays_per_s": (1 << rays_log2) / (tr / 1e3), "trace_ms": tr, "cdf_ms": cdf,
+            "unresolved_fraction": st.q_unresolved / float(st.q_escaped + st.q_unresolved),
+            "status_counts": list(map(int, st.status[:8])), "diffraction_captured": st.diffraction_captured,
+            "config": "hex column n=1.31 a=50um c/a=2 ki=(1,-0.4,0.3), 2^%d rays, <=12 interactions, "
+                      "Fraunhofer lobe, 900x1800 cells, FP64" % rays_log2}
+
+
+def run_sphere(ctx):
+    """Water sphere r = 1 mm at 550 nm, alpha 0, 2048^2 rays, <= 8 internal hits."""
+    import ice_crystal_phase_amd as ip
+    n = ip.water_optics(550.0).refractive_index
+    rec = ip.sphere_record(n, 2048, 900, 1800, radius_mm=1.0, inclination_deg=0.0, max_internal_hits=8)
+    with ip.Batch(ctx, [rec]) as b:
+        flags = ip.RUN_TRACE | ip.RUN_CDF | ip.RUN_GRAPH
+        b.run(flags)
+        b.run(flags)
+        tr, cdf, _ = b.timing()
+    return {"rays": 2048 * 2048, "rays_per_s": 2048 * 2048 / (tr / 1e3), "trace_ms": tr, "cdf_ms": cdf}
+
+
+def run_surrogate(ctx, n_eval: int = 16, eval_rays_log2: int = 28):
+    """Serving rate of the trained surrogate (64 records of 450 x 900 cells,
+    network + FP64 CDFs on the device) and its TV / |dg| against fresh traced
+    records; the tracer's records/s for the same family at 2^26 rays."""
+    import ice_crystal_phase_amd as ip
+    from ice_crystal_phase_amd import surrogate as S
+    if not SURROGATE_WEIGHTS.exists():
+        return None
+    p = np.load(SURROGATE_WEIGHTS, allow_pickle=False)
+    conds = S.drop_conditions(64, seed=6)
+    with S.Surrogate(ctx, 450, 900, max_records=64, seed=6) as sur:
+        sur.set_params(p)
+        sur.records(conds)
+        ms = []
+        for _ in range(3):
+            sur.records(conds)
+            ms.append(sur.timing()[1])
+        t = min(ms)
+        side = 1 << (eval_rays_log2 // 2)
+        tv, dg, trace_ms = S.evaluate(sur, conds[:n_eval], side)
+    tracer_rays_per_s = n_eval * side * side / (trace_ms / 1e3)
+    tracer_rps = tracer_rays_per_s / float(1 << 26)
+    served = 64 / (t / 1e3)
+    return {"records_per_s": served, "ms_per_64_records": t, "tv_mean": float(tv.mean()), "tv_max": float(tv.max()),
+            "dg_mean": float(dg.mean()), "tracer_records_per_s_2e26": tracer_rps,
+            "speedup_vs_tracer": served / tracer_rps, "eval_records": n_eval,
+            "eval_rays_per_record": side * side,
+            "config": "water drops alpha U[-90,90], lambda U[400,700] nm, r U[0.25,3] mm (seed 6), 450x900 cells"}
+
+
+def cpu_baseline_hex(budget_s: float = 10.0):
+    """The compiled reference hex tracer + compensated FP64 histogram
+    (oracle/_ref, built from the reference CPU sources) on one host core."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_bind as ob
+    if not ob.ref_available():
+        return None
+    n = 20000
+    s = ob.hex_settings(ki=(1.0, -0.4, 0.3), total=n, max_hits=12)
+    t0 = time.perf_counter()
+    ob.ref_hex_masses(s, 900, 1800)
+    dt = time.perf_counter() - t0
+    n2 = int(max(n, n * budget_s / max(dt, 1e-6)))
+    s = ob.hex_settings(ki=(1.0, -0.4, 0.3), total=n2, max_hits=12)
+    t0 = time.perf_counter()
+    ob.ref_hex_masses(s, 900, 1800)
+    dt = time.perf_counter() - t0
+    return {"value": n2 / dt, "unit": "rays/s", "cores": 1, "kind": "reference",
+            "sample": f"{n2} rays of the hex column record (compiled reference trace_sample + "
+                      f"PhaseMassAccumulator, 900x1800) in {dt:.1f} s"}
+
+
+def run_uneven(ctx, args):
+    """512-record uneven batch with the autotuned schedule: records/s."""
+    import ice_crystal_phase_amd as ip
+    from ice_crystal_phase_amd import autotune
+    recs = autotune.uneven_records(512, seed=4)
+    table = []
+    choice = autotune.tune(ctx, recs, shrink=16, log=lambda m: table.append(m))
+    with ip.Batch(ctx, recs) as b:
+        b.set_schedule(choice.tile_rays, choice.blocks)
+        flags = ip.RUN_TRACE | ip.RUN_CDF | ip.RUN_GRAPH
+        for _ in range(args.warmup):
+            b.run(flags)
+        t0 = time.perf_counter()
+        ms = []
+        for _ in range(args.steps):
+            b.run(flags)
+            ms.append(b.timing())
+        wall = (time.perf_counter() - t0) / args.steps
+        rays = sum(ip.record_rays(r) for r in recs)
+    step = float(np.mean([m[2] for m in ms]))
+    return {
+        "metric": "records/s of the uneven 512-record batch on 1 MI355X (autotuned schedule)",
+        "value": 512 / wall, "unit": "records/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 drop trace, f64 hex trace, f64 CDF", "data": "synthetic",
+        "config": {"workload": "512 records seed 4: 1/3 spheres, 1/3 oblate drops, 1/3 hex (aspect 0.2-4), "
+                               "r U[0.1,3] mm, lambda U[400,700] nm, alpha U[-90,90], rays 2^U[20,26], 900x1800"},
+        "device_ms_per_step": step, "trace_ms": float(np.mean([m[0] for m in ms])),
+        "cdf_ms": float(np.mean([m[1] for m in ms])), "rays_per_s": rays / wall, "rays": rays,
+        "schedule": {"tile_rays": choice.tile_rays, "blocks": choice.blocks, "proxy_ms": choice.ms},
+        "autotune": table,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +239,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
+    ap.add_argument("--config", choices=("drops", "uneven"), default="drops")
+    ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy (torch) 4x: the FETCH_SIZE/WRITE_SIZE scale reference")
     args = ap.parse_args()
@@ -124,6 +260,12 @@ def main() -> int:
         return 2
 
     import ice_crystal_phase_amd as ip
+
+    if args.config == "uneven":
+        with ip.Context(0) as ctx:
+            out = run_uneven(ctx, args)
+        print(json.dumps(out))
+        return 0
 
     recs, coeffs, side = workload(args.records, args.rays_log2, args.theta, args.phi)
     ctx = ip.Context(0)
@@ -184,13 +326,22 @@ def main() -> int:
                      "frac": (achieved / PEAK_TFLOPS_FP32) if achieved else None,
                      "traffic": None, "flops_per_launch": flops},
     }
+    batch.close()
+    if not args.no_extras:
+        out["hex"] = run_hex(ctx)
+        out["sphere"] = run_sphere(ctx)
+        out["surrogate"] = run_surrogate(ctx)
     if not args.no_cpu_baseline:
         rps, n, dt = cpu_baseline(recs[len(recs) // 2], nt, np_)
         out["cpu_baseline"] = {"value": rps, "unit": "rays/s", "cores": 1, "kind": "port",
                                "sample": f"{n} rays of record {len(recs)//2} (FP32 oracle mirror, same grid, "
                                          f"binning and fixed-point histogram) in {dt:.1f} s"}
         out["gpu_over_cpu"] = rays_per_s / rps
-    batch.close()
+        if not args.no_extras:
+            hb = cpu_baseline_hex()
+            out["cpu_baseline_hex"] = hb
+            if hb:
+                out["hex"]["gpu_over_cpu_reference"] = out["hex"]["rays_per_s"] / hb["value"]
     ctx.close()
     print(json.dumps(out))
     return 0

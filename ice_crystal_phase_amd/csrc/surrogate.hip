@@ -43,17 +43,15 @@ constexpr int RB = TM / 16;
 // different banks (32 KB per tile).
 __device__ inline int sw(int row, int col) { return row * 256 + (col ^ ((row & 7) << 3)); }
 
-__device__ __host__ inline float bf2f(__bf16 b)
+__device__ inline float bf2f(__bf16 b)
 {
     return __builtin_bit_cast(float, (uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
 }
-__device__ __host__ inline __bf16 f2bf(float x)
-{
-    uint32_t u = __builtin_bit_cast(uint32_t, x);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    return __builtin_bit_cast(__bf16, (uint16_t)(u >> 16));
-}
-__device__ inline float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// round to nearest even (v_cvt_pk_bf16_f32)
+__device__ inline __bf16 f2bf(float x) { return (__bf16)x; }
+// the activation needs no IEEE division: exp and the hardware reciprocal
+// (1 ulp) keep the layer epilogues short next to the MFMA work
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 {
