@@ -10,20 +10,25 @@
 //   h1 = silu(W1 x + b1), h2..h4 = silu(W h + b)       (4 x 256)
 //   f = w5 . h4 + b5,  p(cell) = softmax over cells of (f + log du_i)
 // so f is a log density in u (cells carry du_i / np of the u-v square).
-// The first layer splits both x and W1 into bf16 hi + lo parts (three MFMA
-// passes), so neighbouring theta rows stay distinct; hidden activations are
-// bf16, every product accumulates in fp32.
+// The first layer is separable over the grid (t depends on the theta row, the
+// Fourier features on the phi column, c on the record) and runs in fp32 on the
+// VALU: z1 = W1[:,0] t_i + (b1 + W1[:,33:37] c) + P[j], with the phi table
+// P = W1[:,1:33] fourier(v_j) rebuilt whenever the weights change, so
+// neighbouring theta rows stay distinct. Hidden activations are bf16, every
+// MFMA product accumulates in fp32. The weight gradient of layer 1 splits x
+// into bf16 hi + lo parts (two MFMA passes).
 //
 // Training loss per record: cross entropy -sum P log p against the traced
 // record's cell probabilities P. Kernels:
-//   sur_forward     fused 4-layer forward per 128-row tile (activations in LDS),
+//   sur_forward     fused 4-layer forward per 64-row tile (activations in LDS),
 //                   logits, and the pre-activations Z_l when training;
 //   sur_lse_*       log-sum-exp per record (fixed-order two-level reduction);
 //   sur_dlogits     dL/df = p - P and the loss;
 //   sur_backward    fused data gradients layers 4 -> 1 per tile (dZ_l to HBM);
 //   sur_wgrad       weight gradients dW_l = dZ_l^T H_{l-1}, split over rows,
 //                   partial tiles reduced in fixed order (deterministic);
-//   sur_adam/pack   Adam on the fp32 master weights, bf16 copies for MFMA.
+//   sur_adam/pack   Adam on the fp32 master weights, bf16 copies for MFMA,
+//                   and the layer-1 phi table.
 #include "surrogate.hpp"
 
 #include <hip/hip_runtime.h>
@@ -32,15 +37,16 @@ namespace icephase {
 namespace sur {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Rows of the activation tile of one workgroup (4 waves x 64 output columns).
-constexpr int TM = 64;
+constexpr int TM = kTileRows;
 constexpr int RB = TM / 16;
 
 // LDS index of element (row, col) of a TM x 256 bf16 tile: 8-element chunks
 // are XOR-swizzled by row so the 16 rows an MFMA fragment reads fall in
-// different banks (32 KB per tile).
+// different banks (64 KB per tile).
 __device__ inline int sw(int row, int col) { return row * 256 + (col ^ ((row & 7) << 3)); }
 
 __device__ inline float bf2f(__bf16 b)
@@ -53,30 +59,69 @@ __device__ inline __bf16 f2bf(float x) { return (__bf16)x; }
 // (1 ulp) keep the layer epilogues short next to the MFMA work
 __device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
+__device__ inline float silu(float z) { return z * sigmoid(z); }
+
+// four fp32 values as four bf16 in 8 bytes (two v_cvt_pk_bf16_f32)
+__device__ inline uint2 pack4(float a, float b, float c, float d)
+{
+    const bf16x2 lo = {f2bf(a), f2bf(b)}, hi = {f2bf(c), f2bf(d)};
+    return make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+}
+__device__ inline float4 unpack4(uint2 v)
+{
+    return make_float4(__builtin_bit_cast(float, v.x << 16), __builtin_bit_cast(float, v.x & 0xffff0000u),
+                       __builtin_bit_cast(float, v.y << 16), __builtin_bit_cast(float, v.y & 0xffff0000u));
+}
+
+// x plus the value of the lane CTRL selects within its 16-lane row (0 where
+// that lane is outside the row)
+template <int CTRL>
+__device__ inline float add_dpp(float x)
+{
+    return x + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, true));
+}
+// sum over the 16 lanes of each row, in a fixed order; lane 15 of the row holds it
+__device__ inline float row16_sum(float x)
+{
+    x = add_dpp<0x111>(x);  // row_shr:1
+    x = add_dpp<0x112>(x);  // row_shr:2
+    x = add_dpp<0x114>(x);  // row_shr:4
+    return add_dpp<0x118>(x);  // row_shr:8
+}
+
 __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// One TM-row x 64-column output slab of C = A B^T for this wave: A is the
-// LDS tile (columns [a_off, a_off + K)), B is a global
-// [N][K] bf16 matrix (row stride K), output columns [n0, n0 + 64).
-template <int K>
-__device__ inline void slab_mma(const __bf16* As, int a_off, const __bf16* __restrict__ B, int n0, f32x4 acc[RB][4])
+// This wave's 64 output columns of Z = H W^T over the TM-row tile H in LDS,
+// computed transposed (W is the MFMA A operand) so that every lane holds four
+// consecutive columns of one row:
+//   acc[rb][cb][i] = Z[16 rb + (lane & 15)][n0 + 16 cb + 4 (lane >> 4) + i].
+// W is a global [256][256] bf16 matrix, streamed in 32-wide k slices with the
+// next slice loaded while the current one feeds the MFMAs.
+__device__ inline void slab_mma(const __bf16* Hs, const __bf16* __restrict__ W, int n0, f32x4 acc[RB][4])
 {
     const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-#pragma unroll 2
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 b[4];
+    const __bf16* wrow = W + (size_t)(n0 + r) * 256 + 8 * q;
+    bf16x8 w[4];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
-            b[cb] = *reinterpret_cast<const bf16x8*>(B + (size_t)(n0 + 16 * cb + r) * K + k0 + 8 * q);
+    for (int cb = 0; cb < 4; ++cb) w[cb] = *reinterpret_cast<const bf16x8*>(wrow + 16 * 256 * cb);
+#pragma unroll 1
+    for (int k0 = 0; k0 < 256; k0 += 32) {
+        // next slice (the last one again at the end: in bounds, unused)
+        const int kn = k0 + 32 < 256 ? k0 + 32 : k0;
+        bf16x8 wn[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) wn[cb] = *reinterpret_cast<const bf16x8*>(wrow + 16 * 256 * cb + kn);
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(As + sw(16 * rb + r, a_off + k0 + 8 * q));
+            const bf16x8 h = *reinterpret_cast<const bf16x8*>(Hs + sw(16 * rb + r, k0 + 8 * q));
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = mfma(a, b[cb], acc[rb][cb]);
+            for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = mfma(w[cb], h, acc[rb][cb]);
         }
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) w[cb] = wn[cb];
     }
 }
 
@@ -88,77 +133,118 @@ __device__ inline void zero(f32x4 acc[RB][4])
         for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f32x4{0, 0, 0, 0};
 }
 
-// Static input features of tile rows into LDS: hi parts in columns [0, 64),
-// lo parts in [64, 128); the record's 4 condition values in columns 33..36.
-__device__ inline void load_inputs(__bf16* Hs, const __bf16* __restrict__ Xhi, const __bf16* __restrict__ Xlo,
-                                   uint32_t row0, const float* cond)
+// Forward of one TM-row tile of one record; grid = (Mpad / TM, records).
+// Layer 1 in fp32 (separable input), layers 2-4 bf16 MFMA with the
+// activations in one LDS tile; the output layer is folded into the layer-4
+// epilogue. TRAIN stores the pre-activations Z[l][row][n] (bf16) of record 0.
+template <bool TRAIN>
+__global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const float* __restrict__ conds,
+                                                   float* __restrict__ logits, __bf16* __restrict__ Z)
 {
-    // TM rows x 64 columns x 2 tables of 16-byte vectors
-    for (int v = threadIdx.x; v < TM * 16; v += 256) {
-        const int table = v / (TM * 8), e = v % (TM * 8), row = e >> 3, c8 = (e & 7) * 8;
-        const __bf16* src = (table ? Xlo : Xhi) + (size_t)(row0 + row) * 64 + c8;
-        *reinterpret_cast<bf16x8*>(Hs + sw(row, table * 64 + c8)) = *reinterpret_cast<const bf16x8*>(src);
+    __shared__ __align__(16) __bf16 Hs[TM * 256];
+    const uint32_t rec = blockIdx.y, row0 = blockIdx.x * TM, Mpad = st.Mpad;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c16 = lane & 15;
+    const int n0 = 64 * wave + 4 * q;  // first of the lane's columns in each 16-column block
+    f32x4 acc[RB][4];
+    {
+        // layer 1: z = W1[:,0] t_i + (b1 + W1[:,33:37] c) + P[j]
+        const float* c = conds + 4 * rec;
+        const float c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+        float wt[4][4], base[4][4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int col = n0 + 16 * cb + i;
+                const float* w = net.w1 + col * kInPad;
+                wt[cb][i] = w[0];
+                base[cb][i] = fmaf(w[36], c3, fmaf(w[35], c2, fmaf(w[34], c1, fmaf(w[33], c0, net.b[0][col]))));
+            }
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const uint32_t row = row0 + 16 * rb + c16;
+            const uint32_t ti = row / st.np, j = row - ti * st.np;
+            const float t = ((float)ti + 0.5f) / (float)st.nt;
+            const float* P = net.pphi + (size_t)j * 256 + n0;
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const float4 p = *reinterpret_cast<const float4*>(P + 16 * cb);
+                acc[rb][cb][0] = fmaf(wt[cb][0], t, base[cb][0]) + p.x;
+                acc[rb][cb][1] = fmaf(wt[cb][1], t, base[cb][1]) + p.y;
+                acc[rb][cb][2] = fmaf(wt[cb][2], t, base[cb][2]) + p.z;
+                acc[rb][cb][3] = fmaf(wt[cb][3], t, base[cb][3]) + p.w;
+            }
+        }
+    }
+    for (int layer = 0;; ++layer) {
+        if (layer > 0) {
+            zero(acc);
+            slab_mma(Hs, net.w[layer - 1], 64 * wave, acc);
+            __syncthreads();  // every wave has read the input tile
+            const float* bias = net.b[layer] + n0;
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const float4 b = *reinterpret_cast<const float4*>(bias + 16 * cb);
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    acc[rb][cb][0] += b.x;
+                    acc[rb][cb][1] += b.y;
+                    acc[rb][cb][2] += b.z;
+                    acc[rb][cb][3] += b.w;
+                }
+            }
+        }
+        if (TRAIN) {
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    const f32x4 z = acc[rb][cb];
+                    *reinterpret_cast<uint2*>(Z + ((size_t)layer * Mpad + row0 + 16 * rb + c16) * 256 + n0 + 16 * cb) =
+                        pack4(z[0], z[1], z[2], z[3]);
+                }
+        }
+        if (layer == 3) break;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const f32x4 z = acc[rb][cb];
+                *reinterpret_cast<uint2*>(Hs + sw(16 * rb + c16, n0 + 16 * cb)) =
+                    pack4(silu(z[0]), silu(z[1]), silu(z[2]), silu(z[3]));
+            }
+        __syncthreads();
+    }
+    // output layer: f = w5 . silu(z4) + b5 + log du. Lane partials over its 16
+    // columns, then over the four lane groups, then over the four waves (the
+    // LDS tile is free after the last barrier), all in a fixed order.
+    float w5[4][4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const float4 w = *reinterpret_cast<const float4*>(net.w5 + n0 + 16 * cb);
+        w5[cb][0] = w.x;
+        w5[cb][1] = w.y;
+        w5[cb][2] = w.z;
+        w5[cb][3] = w.w;
+    }
+    float* part = reinterpret_cast<float*>(Hs);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        float a = 0;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a = fmaf(silu(acc[rb][cb][i]), w5[cb][i], a);
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        if (q == 0) part[wave * TM + 16 * rb + c16] = a;
     }
     __syncthreads();
     if (threadIdx.x < TM) {
         const int row = threadIdx.x;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const __bf16 hi = f2bf(cond[c]);
-            Hs[sw(row, 33 + c)] = hi;
-            Hs[sw(row, 64 + 33 + c)] = f2bf(cond[c] - bf2f(hi));
-        }
+        const float f = (part[row] + part[TM + row]) + (part[2 * TM + row] + part[3 * TM + row]);
+        logits[(size_t)rec * Mpad + row0 + row] = (f + net.b5[0]) + st.logw[row0 + row];
     }
-    __syncthreads();
-}
-
-// Forward. grid = (Mpad / TM, records). TRAIN stores the pre-activations
-// Z[l][row][n] (bf16) of record 0.
-template <bool TRAIN>
-__global__ void __launch_bounds__(256, 2) sur_forward(Net net, const __bf16* __restrict__ Xhi, const __bf16* __restrict__ Xlo,
-                                                   const float* __restrict__ conds, const float* __restrict__ logw,
-                                                   uint32_t Mpad, float* __restrict__ logits, __bf16* __restrict__ Z)
-{
-    __shared__ __align__(16) __bf16 Hs[TM * 256];
-    const uint32_t rec = blockIdx.y, row0 = blockIdx.x * TM;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c16 = lane & 15;
-    const int n0 = 64 * wave;
-    load_inputs(Hs, Xhi, Xlo, row0, conds + 4 * rec);
-    f32x4 acc[RB][4];
-    for (int layer = 0; layer < 4; ++layer) {
-        zero(acc);
-        if (layer == 0) {
-            slab_mma<64>(Hs, 0, net.w1hi, n0, acc);   // x_hi W1_hi
-            slab_mma<64>(Hs, 64, net.w1hi, n0, acc);  // x_lo W1_hi
-            slab_mma<64>(Hs, 0, net.w1lo, n0, acc);   // x_hi W1_lo
-        } else {
-            slab_mma<256>(Hs, 0, net.w[layer - 1], n0, acc);
-        }
-        __syncthreads();  // every wave has read the input tile
-        const float* bias = net.b[layer];
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const int col = n0 + 16 * cb + c16;
-            const float bc = bias[col];
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int row = 16 * rb + 4 * q + i;
-                    const float z = acc[rb][cb][i] + bc;
-                    if (TRAIN) Z[((size_t)layer * Mpad + row0 + row) * 256 + col] = f2bf(z);
-                    Hs[sw(row, col)] = f2bf(z * sigmoid(z));
-                }
-        }
-        __syncthreads();
-    }
-    // output layer: four threads per row, 64 products each
-    const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
-    float s = 0;
-    for (int k = 0; k < 64; ++k) s += bf2f(Hs[sw(row, 64 * part + k)]) * net.w5[64 * part + k];
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    if (part == 0) logits[(size_t)rec * Mpad + row0 + row] = s + net.b5[0] + logw[row0 + row];
 }
 
 // log-sum-exp over the rows of each record: per-block (max, sum exp) partials,
@@ -193,17 +279,27 @@ __global__ void __launch_bounds__(256) sur_lse_partial(const float* __restrict__
     if (threadIdx.x == 0) part[(size_t)rec * gridDim.x + blockIdx.x] = sh[0];
 }
 
+// lane k combines parts k, k + 64, ...; the 64 lane sums are added in lane
+// order (FP64), so the result does not depend on scheduling
 __global__ void __launch_bounds__(64) sur_lse_final(const float2* __restrict__ part, uint32_t nparts,
                                                     float* __restrict__ lse)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double sh[64];
     const float2* p = part + (size_t)blockIdx.x * nparts;
     float m = -INFINITY;
-    for (uint32_t k = 0; k < nparts; ++k) m = fmaxf(m, p[k].x);
+    for (uint32_t k = threadIdx.x; k < nparts; k += 64) m = fmaxf(m, p[k].x);
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) m = fmaxf(m, __shfl_xor(m, w, 64));
     double s = 0;
-    for (uint32_t k = 0; k < nparts; ++k)
+    for (uint32_t k = threadIdx.x; k < nparts; k += 64)
         if (p[k].x > -INFINITY) s += (double)p[k].y * (double)__expf(p[k].x - m);
-    lse[blockIdx.x] = m + (float)log(s);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < 64; ++k) t += sh[k];
+        lse[blockIdx.x] = m + (float)log(t);
+    }
 }
 
 // dL/df = p - P (record 0), loss partials sum P (lse - f).
@@ -264,76 +360,145 @@ __global__ void __launch_bounds__(256, 2) sur_backward(Net net, const float* __r
     const uint32_t row0 = blockIdx.x * TM;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c16 = lane & 15;
     const int n0 = 64 * wave;
+    const int n0q = n0 + 4 * q;  // first of the lane's columns in each 16-column block
     {
-        // thread = column n of layer 4
-        const int n = threadIdx.x;
-        const float w5 = net.w5[n];
+        // layer 4: dZ4 = dF w5 silu'(Z4); partial dw5 = sum_rows dF silu(Z4),
+        // db4 = sum_rows dZ4, db5 = sum_rows dF. Lanes hold four consecutive
+        // columns of one row, as in the hidden-layer epilogues below.
         const __bf16* z4 = Z + ((size_t)3 * Mpad + row0) * 256;
         __bf16* dz4 = dZ + ((size_t)3 * Mpad + row0) * 256;
-        float gw = 0, gb = 0, gz = 0;
-        for (int row = 0; row < TM; ++row) {
-            const float d = dF[row0 + row];
-            const float z = bf2f(z4[(size_t)row * 256 + n]);
-            const float s = sigmoid(z);
-            gw += d * (z * s);
-            gb += d;
-            const float dz = d * w5 * (s * (1.0f + z * (1.0f - s)));
-            gz += dz;
-            const __bf16 g = f2bf(dz);
-            dz4[(size_t)row * 256 + n] = g;
-            Gs[sw(row, n)] = g;
+        float d[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) d[rb] = dF[row0 + 16 * rb + c16];
+        float* gw_out = part_w5 + (size_t)blockIdx.x * 256 + n0q;
+        float* gz_out = part_db + ((size_t)3 * tiles + blockIdx.x) * 256 + n0q;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int col = n0q + 16 * cb;
+            const float4 w5 = *reinterpret_cast<const float4*>(net.w5 + col);
+            const float wv[4] = {w5.x, w5.y, w5.z, w5.w};
+            float gw[4] = {0, 0, 0, 0}, gz[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                const int row = 16 * rb + c16;
+                const float4 z = unpack4(*reinterpret_cast<const uint2*>(z4 + (size_t)row * 256 + col));
+                const float zv[4] = {z.x, z.y, z.z, z.w};
+                float dz[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float sg = sigmoid(zv[i]);
+                    gw[i] += d[rb] * (zv[i] * sg);
+                    dz[i] = d[rb] * wv[i] * (sg * (1.0f + zv[i] * (1.0f - sg)));
+                    gz[i] += dz[i];
+                }
+                const uint2 g = pack4(dz[0], dz[1], dz[2], dz[3]);
+                *reinterpret_cast<uint2*>(dz4 + (size_t)row * 256 + col) = g;
+                *reinterpret_cast<uint2*>(Gs + sw(row, col)) = g;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                gw[i] = row16_sum(gw[i]);
+                gz[i] = row16_sum(gz[i]);
+            }
+            if (c16 == 15) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    gw_out[16 * cb + i] = gw[i];
+                    gz_out[16 * cb + i] = gz[i];
+                }
+            }
         }
-        part_w5[(size_t)blockIdx.x * 256 + n] = gw;
-        part_db[((size_t)3 * tiles + blockIdx.x) * 256 + n] = gz;
-        if (n == 0) part_b5[blockIdx.x] = gb;
+        if (threadIdx.x < 16) {
+            float gb = 0;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) gb += d[rb];
+            gb = row16_sum(gb);
+            if (threadIdx.x == 15) part_b5[blockIdx.x] = gb;
+        }
     }
     __syncthreads();
     f32x4 acc[RB][4];
     for (int layer = 3; layer >= 1; --layer) {
-        // dH_{layer-1} = dZ_layer W_layer  (B = W^T stored [in][out])
-        zero(acc);
-        slab_mma<256>(Gs, 0, net.wt[layer - 1], n0, acc);
-        __syncthreads();
+        // dH_{layer-1} = dZ_layer W_layer  (W^T stored [in][out] is the A operand);
+        // the pre-activations the epilogue needs are loaded before the MFMAs
         const __bf16* zp = Z + ((size_t)(layer - 1) * Mpad + row0) * 256;
         __bf16* dzp = dZ + ((size_t)(layer - 1) * Mpad + row0) * 256;
+        uint2 zr[RB][4];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+                zr[rb][cb] = *reinterpret_cast<const uint2*>(zp + (size_t)(16 * rb + c16) * 256 + n0q + 16 * cb);
+        zero(acc);
+        slab_mma(Gs, net.wt[layer - 1], n0, acc);
+        __syncthreads();  // every wave has read dZ_layer
+        float* db = part_db + ((size_t)(layer - 1) * tiles + blockIdx.x) * 256 + n0q;
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
-            const int col = n0 + 16 * cb + c16;
-            float colsum = 0;
+            const int col = n0q + 16 * cb;
+            float cs0 = 0, cs1 = 0, cs2 = 0, cs3 = 0;
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int row = 16 * rb + 4 * q + i;
-                    const float z = bf2f(zp[(size_t)row * 256 + col]);
-                    const float dz = acc[rb][cb][i] * silu_grad(z);
-                    colsum += dz;
-                    const __bf16 g = f2bf(dz);
-                    dzp[(size_t)row * 256 + col] = g;
-                    Gs[sw(row, col)] = g;
-                }
-            // the four lane groups hold the other rows of this column
-            colsum += __shfl_xor(colsum, 16, 64);
-            colsum += __shfl_xor(colsum, 32, 64);
-            if (q == 0) part_db[((size_t)(layer - 1) * tiles + blockIdx.x) * 256 + col] = colsum;
+            for (int rb = 0; rb < RB; ++rb) {
+                const int row = 16 * rb + c16;
+                const float4 z = unpack4(zr[rb][cb]);
+                const float d0 = acc[rb][cb][0] * silu_grad(z.x), d1 = acc[rb][cb][1] * silu_grad(z.y);
+                const float d2 = acc[rb][cb][2] * silu_grad(z.z), d3 = acc[rb][cb][3] * silu_grad(z.w);
+                cs0 += d0;
+                cs1 += d1;
+                cs2 += d2;
+                cs3 += d3;
+                const uint2 g = pack4(d0, d1, d2, d3);
+                *reinterpret_cast<uint2*>(dzp + (size_t)row * 256 + col) = g;
+                *reinterpret_cast<uint2*>(Gs + sw(row, col)) = g;  // unused after layer 1
+            }
+            // column sums over the tile: the 16 lanes of a row hold the other rows
+            cs0 = row16_sum(cs0);
+            cs1 = row16_sum(cs1);
+            cs2 = row16_sum(cs2);
+            cs3 = row16_sum(cs3);
+            if (c16 == 15) {
+                db[16 * cb + 0] = cs0;
+                db[16 * cb + 1] = cs1;
+                db[16 * cb + 2] = cs2;
+                db[16 * cb + 3] = cs3;
+            }
         }
         __syncthreads();
     }
 }
 
+// Four rows x four bf16 columns (v[k] = row k) written column-major into an
+// LDS tile T[col][row]: one 8-byte store per column.
+__device__ inline void store_cols(__bf16* T, int stride, const uint2 v[4])
+{
+    uint2 c[4];
+    c[0] = make_uint2((v[0].x & 0xffffu) | (v[1].x << 16), (v[2].x & 0xffffu) | (v[3].x << 16));
+    c[1] = make_uint2((v[0].x >> 16) | (v[1].x & 0xffff0000u), (v[2].x >> 16) | (v[3].x & 0xffff0000u));
+    c[2] = make_uint2((v[0].y & 0xffffu) | (v[1].y << 16), (v[2].y & 0xffffu) | (v[3].y << 16));
+    c[3] = make_uint2((v[0].y >> 16) | (v[1].y & 0xffff0000u), (v[2].y >> 16) | (v[3].y & 0xffff0000u));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) *reinterpret_cast<uint2*>(T + j * stride) = c[j];
+}
+
 // Weight gradients of one layer: dW[o][i] = sum_rows dZ[row][o] H[row][i] over
 // the rows of split s, for all 256 outputs and IN inputs. H = silu(Zprev) for
 // hidden layers; for the input layer H = x_hi + x_lo (two passes). Waves:
-// (IN/64) x 4, each 64 outputs x 64 inputs.
+// (IN/64) x 4, each 64 outputs x 64 inputs. Rows go in chunks of 32: every
+// thread stages 4 x 4 blocks of dZ and H into column-major LDS tiles (the MFMA
+// reads 8 consecutive rows of one column), loading the next chunk's blocks
+// into registers while the MFMAs consume the current one.
 template <int IN>
 __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ dZ, const __bf16* __restrict__ Zprev,
                                                     const __bf16* __restrict__ Xhi, const __bf16* __restrict__ Xlo,
                                                     const float* __restrict__ cond, uint32_t rows_per_split,
                                                     uint32_t Mpad, float* __restrict__ part)
 {
-    constexpr int T = IN * 4;          // threads
-    constexpr int S = 40;              // LDS stride (bf16) of a 32-row transposed chunk
+    constexpr int T = IN * 4;             // threads
+    constexpr int S = 40;                 // LDS stride (bf16) of a 32-row column
     constexpr int NH = IN == 64 ? 2 : 1;  // input layer: hi and lo tables
+    constexpr int NT = 512 + 8 * (IN / 4) * NH;  // 4 x 4 blocks per chunk: dZ, then H
+    constexpr int PER = NT / T;                  // blocks per thread
+    static_assert(NT % T == 0, "whole blocks per thread");
     __shared__ __align__(16) __bf16 dZT[256 * S];
     __shared__ __align__(16) __bf16 HT[NH][IN * S];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c16 = lane & 15;
@@ -345,41 +510,79 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0, 0, 0, 0};
-    for (uint32_t r0 = r_begin; r0 < r_end; r0 += 32) {
-        // stage the dZ chunk (32 rows x 256) transposed
-        for (int e = threadIdx.x; e < 32 * 32; e += T) {
-            const int row = e >> 5, c8 = (e & 31) * 8;
-            const bf16x8 v = *reinterpret_cast<const bf16x8*>(dZ + (size_t)(r0 + row) * 256 + c8);
+    // block p of this thread: task = threadIdx.x + p T; rows 4 (task & 7) ..,
+    // columns 4 (task >> 3) .. of dZ (task < 512) or of H
+    uint2 v[PER][4];
+    auto load = [&](uint32_t r0) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) dZT[(c8 + j) * S + row] = v[j];
+        for (int p = 0; p < PER; ++p) {
+            const int task = threadIdx.x + p * T, rg = task & 7;
+            const bool is_dz = task < 512;
+            const int c = 4 * ((is_dz ? task : task - 512) >> 3);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const size_t row = r0 + 4 * rg + k;
+                const __bf16* src;
+                if (is_dz)
+                    src = dZ + row * 256 + c;
+                else if (IN == 64)
+                    src = (c < 64 ? Xhi : Xlo) + row * 64 + (c & 63);
+                else
+                    src = Zprev + row * 256 + c;
+                v[p][k] = *reinterpret_cast<const uint2*>(src);
+            }
         }
-        // stage the H chunk (32 rows x IN) transposed
-        for (int e = threadIdx.x; e < 32 * (IN / 8); e += T) {
-            const int row = e / (IN / 8), c8 = (e % (IN / 8)) * 8;
-            if (IN == 64) {
-                bf16x8 hi = *reinterpret_cast<const bf16x8*>(Xhi + (size_t)(r0 + row) * 64 + c8);
-                bf16x8 lo = *reinterpret_cast<const bf16x8*>(Xlo + (size_t)(r0 + row) * 64 + c8);
+    };
+    float ch[4], cl[4];  // condition inputs (columns 33..36 of the input layer), hi and lo
+    if (IN == 64) {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int c = c8 + j;
-                    if (c >= 33 && c < 37) {
-                        const __bf16 h = f2bf(cond[c - 33]);
-                        hi[j] = h;
-                        lo[j] = f2bf(cond[c - 33] - bf2f(h));
+        for (int k = 0; k < 4; ++k) {
+            ch[k] = bf2f(f2bf(cond[k]));
+            cl[k] = cond[k] - ch[k];
+        }
+    }
+    load(r_begin);
+    for (uint32_t r0 = r_begin; r0 < r_end; r0 += 32) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int task = threadIdx.x + p * T, rg = task & 7;
+            if (task < 512) {
+                store_cols(dZT + 4 * (task >> 3) * S + 4 * rg, S, v[p]);
+            } else if (IN == 64) {
+                const int c = 4 * ((task - 512) >> 3), table = c >> 6, cc = c & 63;
+                if (cc == 32 || cc == 36) {
+                    // substitute the record's condition for columns 33..36
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        float4 x = unpack4(v[p][k]);
+                        float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int col = cc + j;
+                            if (col >= 33 && col <= 36) xs[j] = table ? cl[col - 33] : ch[col - 33];
+                        }
+                        v[p][k] = pack4(xs[0], xs[1], xs[2], xs[3]);
                     }
-                    HT[0][c * S + row] = hi[j];
-                    HT[NH - 1][c * S + row] = lo[j];
                 }
+                store_cols(&HT[table][cc * S + 4 * rg], S, v[p]);
             } else {
-                const bf16x8 z = *reinterpret_cast<const bf16x8*>(Zprev + (size_t)(r0 + row) * 256 + c8);
+                const int c = 4 * ((task - 512) >> 3);
+                float h[4][4];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const float zz = bf2f(z[j]);
-                    HT[0][(c8 + j) * S + row] = f2bf(zz * sigmoid(zz));
+                for (int k = 0; k < 4; ++k) {
+                    const float4 z = unpack4(v[p][k]);
+                    h[k][0] = silu(z.x);
+                    h[k][1] = silu(z.y);
+                    h[k][2] = silu(z.z);
+                    h[k][3] = silu(z.w);
                 }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    *reinterpret_cast<uint2*>(&HT[0][(c + j) * S + 4 * rg]) = pack4(h[0][j], h[1][j], h[2][j], h[3][j]);
             }
         }
         __syncthreads();
+        if (r0 + 32 < r_end) load(r0 + 32);
         bf16x8 a[4];
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob) a[ob] = *reinterpret_cast<const bf16x8*>(dZT + (o0 + 16 * ob + c16) * S + 8 * q);
@@ -450,31 +653,45 @@ __global__ void __launch_bounds__(256) sur_adam(float* __restrict__ w, float* __
     g[e] = 0.0f;
 }
 
-// bf16 operand copies of the fp32 master weights.
-__global__ void __launch_bounds__(256) sur_pack(const float* __restrict__ p, __bf16* __restrict__ w1hi,
-                                                __bf16* __restrict__ w1lo, __bf16* __restrict__ w, __bf16* __restrict__ wt)
+// bf16 operand copies (W and W^T) of the fp32 master weights of layers 2-4.
+__global__ void __launch_bounds__(256) sur_pack(const float* __restrict__ p, __bf16* __restrict__ w,
+                                                __bf16* __restrict__ wt)
 {
     const uint32_t e = blockIdx.x * 256 + threadIdx.x;
-    if (e < 256 * 64) {
-        const float x = p[kOffW1 + e];
-        const __bf16 h = f2bf(x);
-        w1hi[e] = h;
-        w1lo[e] = f2bf(x - bf2f(h));
-    }
     if (e < 3 * 65536) {
         const uint32_t l = e >> 16, k = e & 65535, o = k >> 8, i = k & 255;
-        const __bf16 b = f2bf(p[kOffW2 + l * (65536 + 256) + k]);
+        const __bf16 b = f2bf(p[kOffW2 + l * kLayerStride + k]);
         w[e] = b;
         wt[(size_t)l * 65536 + i * 256 + o] = b;
     }
 }
 
+// phi table of layer 1: P[j][o] = sum_k W1[o][1 + k] feat[j][k] (the 32
+// Fourier features of column j), fp32; block j, thread o.
+__global__ void __launch_bounds__(256) sur_phi_table(const float* __restrict__ p, const float* __restrict__ feat,
+                                                     float* __restrict__ P)
+{
+    const uint32_t j = blockIdx.x, o = threadIdx.x;
+    const float* w = p + kOffW1 + o * kInPad + 1;
+    const float* f = feat + (size_t)j * 2 * kFourier;
+    float s = 0;
+    for (int k = 0; k < 2 * kFourier; ++k) s = fmaf(w[k], f[k], s);
+    P[(size_t)j * 256 + o] = s;
+}
+
+// out[0] = sum of x[0..n) in FP64: 256 strided lane sums added in lane order
 __global__ void __launch_bounds__(256) sur_sum(const float* __restrict__ x, uint32_t n, float* __restrict__ out)
 {
-    if (threadIdx.x || blockIdx.x) return;
+    __shared__ double sh[256];
     double s = 0;
-    for (uint32_t k = 0; k < n; ++k) s += x[k];
-    out[0] = (float)s;
+    for (uint32_t k = threadIdx.x; k < n; k += 256) s += x[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < 256; ++k) t += sh[k];
+        out[0] = (float)t;
+    }
 }
 
 __global__ void sur_clear_u64(unsigned long long* p)
@@ -564,11 +781,9 @@ hipError_t forward(const Net& net, const Static& st, const float* d_conds, uint3
 {
     const dim3 grid(st.Mpad / TM, nrec);
     if (train)
-        hipLaunchKernelGGL(sur_forward<true>, grid, dim3(256), 0, s, net, st.xhi, st.xlo, d_conds, st.logw, st.Mpad,
-                           logits, Z);
+        hipLaunchKernelGGL(sur_forward<true>, grid, dim3(256), 0, s, net, st, d_conds, logits, Z);
     else
-        hipLaunchKernelGGL(sur_forward<false>, grid, dim3(256), 0, s, net, st.xhi, st.xlo, d_conds, st.logw, st.Mpad,
-                           logits, Z);
+        hipLaunchKernelGGL(sur_forward<false>, grid, dim3(256), 0, s, net, st, d_conds, logits, Z);
     return hipGetLastError();
 }
 
@@ -584,7 +799,7 @@ hipError_t dlogits(const Static& st, const float* logits, const float* lse_v, co
 {
     const uint32_t blocks = (st.Mpad + 255) / 256;
     hipLaunchKernelGGL(sur_dlogits, dim3(blocks), dim3(256), 0, s, logits, lse_v, target, st.M, st.Mpad, dF, loss_part);
-    hipLaunchKernelGGL(sur_sum, dim3(1), dim3(64), 0, s, loss_part, blocks, loss);
+    hipLaunchKernelGGL(sur_sum, dim3(1), dim3(256), 0, s, loss_part, blocks, loss);
     return hipGetLastError();
 }
 
@@ -632,9 +847,10 @@ hipError_t adam(float* w, float* m, float* v, float* g, uint32_t n, float lr, fl
     return hipGetLastError();
 }
 
-hipError_t pack(const float* p, __bf16* w1hi, __bf16* w1lo, __bf16* w, __bf16* wt, hipStream_t s)
+hipError_t pack(const float* p, __bf16* w, __bf16* wt, const float* feat, uint32_t np, float* pphi, hipStream_t s)
 {
-    hipLaunchKernelGGL(sur_pack, dim3(3 * 65536 / 256), dim3(256), 0, s, p, w1hi, w1lo, w, wt);
+    hipLaunchKernelGGL(sur_pack, dim3(3 * 65536 / 256), dim3(256), 0, s, p, w, wt);
+    hipLaunchKernelGGL(sur_phi_table, dim3(np), dim3(256), 0, s, p, feat, pphi);
     return hipGetLastError();
 }
 

@@ -31,14 +31,14 @@ constexpr uint32_t kPacked = kParams - 256 * (kInPad - kInputs);
 constexpr uint32_t kLseParts = 256;
 constexpr uint32_t kCompareBlocks = 512;
 constexpr uint32_t kReduceGroups = 32;
-constexpr uint32_t kTileRows = 64;  // rows of a forward/backward workgroup tile
+constexpr uint32_t kTileRows = 128;  // rows of a forward/backward workgroup tile
 
 inline uint32_t off_w(int layer) { return layer == 0 ? kOffW1 : kOffW2 + (layer - 1) * kLayerStride; }
 inline uint32_t off_b(int layer) { return layer == 0 ? kOffB1 : kOffW2 + (layer - 1) * kLayerStride + 65536; }
 
 struct Net {
-    const __bf16* w1hi;  // [256][64]
-    const __bf16* w1lo;
+    const float* w1;     // fp32 [256][64] master weights of layer 1
+    const float* pphi;   // [np][256] layer-1 phi table (Fourier part of W1 x)
     const __bf16* w[3];   // layers 2..4, [out][in]
     const __bf16* wt[3];  // transposes, [in][out]
     const float* b[4];
@@ -51,6 +51,7 @@ struct Static {
     const __bf16* xhi;  // [Mpad][64] static inputs, hi part
     const __bf16* xlo;  //                          lo part
     const float* logw;  // [Mpad] log du of the cell's row, -inf for padding rows
+    const float* feat;  // [np][32] Fourier features sin(2 pi k v_j), cos(2 pi k v_j) (fp32)
     uint32_t rows_per_split, splits;
 };
 
@@ -71,7 +72,8 @@ hipError_t reduce(const float* part, uint32_t splits, uint32_t n, float* out, bo
                   hipStream_t s);
 hipError_t adam(float* w, float* m, float* v, float* g, uint32_t n, float lr, float inv_batch, float c1, float c2,
                 hipStream_t s);
-hipError_t pack(const float* p, __bf16* w1hi, __bf16* w1lo, __bf16* w, __bf16* wt, hipStream_t s);
+// bf16 W, W^T of layers 2-4 and the layer-1 phi table [np][256] from the fp32 parameters
+hipError_t pack(const float* p, __bf16* w, __bf16* wt, const float* feat, uint32_t np, float* pphi, hipStream_t s);
 // Partial sums of |p - q| and (p - q) * row_cos[row] over kCompareBlocks blocks.
 hipError_t compare(const float* p, const float* q, uint32_t M, uint32_t np, const double* row_cos, double* part,
                    hipStream_t s);

@@ -66,8 +66,8 @@ struct iph_sur {
     Static st{};
     uint64_t step = 0;
     DevBuf<float> params, adam_m, adam_v, grad;
-    DevBuf<__bf16> w1hi, w1lo, w, wt, xhi, xlo, Z, dZ;
-    DevBuf<float> logw, logits, dF, lse, loss_part, loss, conds, part, part2, part_w5, part_b5, part_db;
+    DevBuf<__bf16> w, wt, xhi, xlo, Z, dZ;
+    DevBuf<float> pphi, feat, logw, logits, dF, lse, loss_part, loss, conds, part, part2, part_w5, part_b5, part_db;
     DevBuf<float2> lse_part;
     DevBuf<double> mass;
     // training set: probabilities [dataset][M], normalized conditions [dataset][4]
@@ -89,8 +89,8 @@ struct iph_sur {
     Net net() const
     {
         Net n;
-        n.w1hi = w1hi.p;
-        n.w1lo = w1lo.p;
+        n.w1 = params.p + kOffW1;
+        n.pphi = pphi.p;
         for (int l = 0; l < 3; ++l) {
             n.w[l] = w.p + (size_t)l * 65536;
             n.wt[l] = wt.p + (size_t)l * 65536;
@@ -154,8 +154,15 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     std::string err;
     if (!make_u_edges(st.nt, ue, err)) return fail(ctx, IPH_E_INVALID, err);
     std::vector<uint16_t> xh((size_t)st.Mpad * 64, 0), xl((size_t)st.Mpad * 64, 0);
-    std::vector<float> lw(st.Mpad, -INFINITY);
+    std::vector<float> lw(st.Mpad, -INFINITY), feat((size_t)st.np * 2 * kFourier);
     const double two_pi = 6.283185307179586476925286766559;
+    for (uint32_t j = 0; j < st.np; ++j) {
+        const double v = (j + 0.5) / st.np;
+        for (int k = 1; k <= kFourier; ++k) {
+            feat[(size_t)j * 2 * kFourier + k - 1] = (float)std::sin(two_pi * k * v);
+            feat[(size_t)j * 2 * kFourier + kFourier + k - 1] = (float)std::cos(two_pi * k * v);
+        }
+    }
     for (uint32_t i = 0; i < st.nt; ++i)
         for (uint32_t j = 0; j < st.np; ++j) {
             const size_t r = (size_t)i * st.np + j;
@@ -177,18 +184,20 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     IPH_HIP(ctx, s->xhi.alloc(xh.size()));
     IPH_HIP(ctx, s->xlo.alloc(xl.size()));
     IPH_HIP(ctx, s->logw.alloc(st.Mpad));
+    IPH_HIP(ctx, s->feat.alloc(feat.size()));
+    IPH_HIP(ctx, hipMemcpy(s->feat.p, feat.data(), feat.size() * 4, hipMemcpyHostToDevice));
     IPH_HIP(ctx, hipMemcpy(s->xhi.p, xh.data(), xh.size() * 2, hipMemcpyHostToDevice));
     IPH_HIP(ctx, hipMemcpy(s->xlo.p, xl.data(), xl.size() * 2, hipMemcpyHostToDevice));
     IPH_HIP(ctx, hipMemcpy(s->logw.p, lw.data(), lw.size() * 4, hipMemcpyHostToDevice));
     st.xhi = s->xhi.p;
     st.xlo = s->xlo.p;
     st.logw = s->logw.p;
+    st.feat = s->feat.p;
     IPH_HIP(ctx, s->params.alloc(kParamsPad));
     IPH_HIP(ctx, s->adam_m.alloc(kParamsPad));
     IPH_HIP(ctx, s->adam_v.alloc(kParamsPad));
     IPH_HIP(ctx, s->grad.alloc(kParamsPad));
-    IPH_HIP(ctx, s->w1hi.alloc(256 * 64));
-    IPH_HIP(ctx, s->w1lo.alloc(256 * 64));
+    IPH_HIP(ctx, s->pphi.alloc((size_t)st.np * 256));
     IPH_HIP(ctx, s->w.alloc(3 * 65536));
     IPH_HIP(ctx, s->wt.alloc(3 * 65536));
     IPH_HIP(ctx, s->Z.alloc(zsz));
@@ -276,7 +285,7 @@ int iph_sur_set_params(iph_sur* s, const float* packed)
     for (float v : dev)
         if (!std::isfinite(v)) return sur_fail(s, "Parameters must be finite.");
     IPH_HIP(ctx, hipMemcpy(s->params.p, dev.data(), kParamsPad * 4, hipMemcpyHostToDevice));
-    IPH_HIP(ctx, pack(s->params.p, s->w1hi.p, s->w1lo.p, s->w.p, s->wt.p, ctx->stream));
+    IPH_HIP(ctx, pack(s->params.p, s->w.p, s->wt.p, s->feat.p, s->st.np, s->pphi.p, ctx->stream));
     IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return IPH_OK;
 }
@@ -380,7 +389,7 @@ int iph_sur_train_step(iph_sur* s, const uint32_t* slots, uint32_t n, float lr, 
     const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)s->step)));
     const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)s->step)));
     IPH_HIP(ctx, adam(s->params.p, s->adam_m.p, s->adam_v.p, s->grad.p, kParams, lr, 1.0f / n, c1, c2, q));
-    IPH_HIP(ctx, pack(s->params.p, s->w1hi.p, s->w1lo.p, s->w.p, s->wt.p, q));
+    IPH_HIP(ctx, pack(s->params.p, s->w.p, s->wt.p, s->feat.p, s->st.np, s->pphi.p, q));
     IPH_HIP(ctx, hipEventRecord(s->ev[1], q));
     if (loss) IPH_HIP(ctx, hipMemcpyAsync(loss, s->loss.p, n * 4, hipMemcpyDeviceToHost, q));
     IPH_HIP(ctx, hipStreamSynchronize(q));

@@ -160,24 +160,6 @@ class Surrogate:
         return ms[0], ms[1], ms[2]
 
 
-def ramp_init(params: np.ndarray, seed: int = 0, slope=(8.0, 200.0)) -> np.ndarray:
-    """Multi-scale ramp initialization of the first layer's theta column: unit k
-    gets slope +-w_k (log-uniform in `slope`) and a transition at t0_k ~ U(0, 1)
-    (bias -w_k t0_k), so the first SiLU layer starts as a basis of ramps of
-    widths 1/200 .. 1/8 of the theta range placed across it; the other inputs
-    keep their initialization."""
-    p = np.array(params, np.float32, copy=True)
-    rng = np.random.default_rng(seed)
-    n_in = 37
-    W1 = p[:256 * n_in].reshape(256, n_in)
-    b1 = p[256 * n_in:256 * n_in + 256]
-    w = np.exp(rng.uniform(np.log(slope[0]), np.log(slope[1]), 256)) * rng.choice([-1.0, 1.0], 256)
-    t0 = rng.uniform(0.0, 1.0, 256)
-    W1[:, 0] = w
-    b1[:] = -w * t0
-    return p
-
-
 def fill_training_set(sur: Surrogate, conds: np.ndarray, grid: int, chunk: int = 64,
                       max_internal_hits: int = 9) -> float:
     """Trace every record of `conds` and copy its cell probabilities into the

@@ -39,8 +39,6 @@ def main(argv=None) -> int:
     ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
     ap.add_argument("--out", default=None, help="write the trained parameters (.npy)")
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--t-ramp", action="store_true",
-                    help="multi-scale ramp initialization of the theta input column of layer 1")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -52,8 +50,6 @@ def main(argv=None) -> int:
                                             seed=a.seed) as sur:
         if a.init:
             sur.set_params(np.load(a.init, allow_pickle=False))
-        elif a.t_ramp:
-            sur.set_params(S.ramp_init(sur.params(), seed=a.seed))
         t0 = time.time()
         trace_ms = S.fill_training_set(sur, conds, a.grid)
         log(f"training set: {a.records} records traced in {trace_ms / 1e3:.2f} s device ({time.time() - t0:.1f} s wall)")
