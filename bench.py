@@ -106,6 +106,7 @@ def cpu_baseline(rec, nt: int, np_: int, budget_s: float = 12.0):
 
 
 SURROGATE_WEIGHTS = ROOT / "ice_crystal_phase_amd" / "data" / "surrogate_drop_450x900.npy"
+SURROGATE_GATE = ROOT / "ice_crystal_phase_amd" / "data" / "surrogate_drop_450x900.gate.json"
 
 
 def run_hex(ctx, rays_log2: int = 28):
@@ -140,17 +141,21 @@ def run_sphere(ctx):
     return {"rays": 2048 * 2048, "rays_per_s": 2048 * 2048 / (tr / 1e3), "trace_ms": tr, "cdf_ms": cdf}
 
 
-def run_surrogate(ctx, n_eval: int = 16, eval_rays_log2: int = 28):
-    """Serving rate of the trained surrogate (64 records of 450 x 900 cells,
-    network + FP64 CDFs on the device) and its TV / |dg| against fresh traced
-    records; the tracer's records/s for the same family at 2^26 rays."""
+def run_surrogate(ctx, n_eval: int = 64, eval_grid: int = 16384, tracer_grid: int = 8192, train_steps: int = 3):
+    """The trained surrogate on 64 fresh water-drop records (seed 6) of 450 x 900
+    cells: its serving rate (network + FP64 CDF records on the device), the
+    tracer's rate on the same records at tracer_grid^2 rays (trace + CDF), its TV
+    and |dg| against eval_grid^2-ray traced records, the validated domain (gate)
+    and the mixed service that traces the records outside it, and the time of
+    one training step (8 records)."""
     import ice_crystal_phase_amd as ip
     from ice_crystal_phase_amd import surrogate as S
     if not SURROGATE_WEIGHTS.exists():
         return None
     p = np.load(SURROGATE_WEIGHTS, allow_pickle=False)
+    gate = S.Gate.load(SURROGATE_GATE) if SURROGATE_GATE.exists() else None
     conds = S.drop_conditions(64, seed=6)
-    with S.Surrogate(ctx, 450, 900, max_records=64, seed=6) as sur:
+    with S.Surrogate(ctx, 450, 900, max_records=64, dataset=8, seed=6) as sur:
         sur.set_params(p)
         sur.records(conds)
         ms = []
@@ -158,16 +163,42 @@ def run_surrogate(ctx, n_eval: int = 16, eval_rays_log2: int = 28):
             sur.records(conds)
             ms.append(sur.timing()[1])
         t = min(ms)
-        side = 1 << (eval_rays_log2 // 2)
-        tv, dg, trace_ms = S.evaluate(sur, conds[:n_eval], side)
-    tracer_rays_per_s = n_eval * side * side / (trace_ms / 1e3)
-    tracer_rps = tracer_rays_per_s / float(1 << 26)
+        recs = S.drop_records(conds, tracer_grid, 450, 900)
+        with ip.Batch(ctx, recs) as b:
+            flags = ip.RUN_TRACE | ip.RUN_CDF | ip.RUN_GRAPH
+            b.run(flags)
+            b.run(flags)
+            tr, cdf, _ = b.timing()
+        tracer_rps = len(recs) / ((tr + cdf) / 1e3)
+        tv, dg, _ = S.evaluate(sur, conds[:n_eval], eval_grid)
+        inside = gate.accepts(conds[:n_eval]) if gate else np.ones(n_eval, bool)
+        mixed = S.serve(sur, gate, conds, tracer_grid) if gate else None
+        S.fill_training_set(sur, conds[:8], 1024)
+        steps = []
+        for _ in range(train_steps):
+            sur.train_step(np.arange(8, dtype=np.uint32), 1e-9)
+            steps.append(sur.timing()[2])
     served = 64 / (t / 1e3)
-    return {"records_per_s": served, "ms_per_64_records": t, "tv_mean": float(tv.mean()), "tv_max": float(tv.max()),
-            "dg_mean": float(dg.mean()), "tracer_records_per_s_2e26": tracer_rps,
-            "speedup_vs_tracer": served / tracer_rps, "eval_records": n_eval,
-            "eval_rays_per_record": side * side,
-            "config": "water drops alpha U[-90,90], lambda U[400,700] nm, r U[0.25,3] mm (seed 6), 450x900 cells"}
+    out = {"records_per_s": served, "ms_per_64_records": t, "tracer_records_per_s": tracer_rps,
+           "tracer_rays_per_record": tracer_grid * tracer_grid, "speedup_vs_tracer": served / tracer_rps,
+           "tv_mean": float(tv.mean()), "tv_max": float(tv.max()), "tv_p90": float(np.quantile(tv, 0.9)),
+           "dg_mean": float(dg.mean()), "eval_records": n_eval, "eval_rays_per_record": eval_grid * eval_grid,
+           "train_step_ms_8_records": min(steps),
+           "config": "water drops alpha U[-90,90], lambda U[400,700] nm, r U[0.25,3] mm (seed 6), 450x900 cells"}
+    if gate is not None:
+        out["gate"] = {"accept_radius_bins": [[gate.radius_edges[k], gate.radius_edges[k + 1]]
+                                              for k in range(len(gate.accept)) if gate.accept[k]],
+                       "tv_limit": gate.tv_limit, "quantile": gate.quantile}
+        out["inside"] = int(inside.sum())
+        if inside.any():
+            out["tv_inside_mean"] = float(tv[inside].mean())
+            out["tv_inside_max"] = float(tv[inside].max())
+            out["dg_inside_mean"] = float(dg[inside].mean())
+        n_in = int(mixed.by_surrogate.sum())
+        out["mixed_service"] = {"records": 64, "by_surrogate": n_in,
+                                "records_per_s": 64 / ((mixed.surrogate_ms + mixed.tracer_ms) / 1e3),
+                                "surrogate_ms": mixed.surrogate_ms, "tracer_ms": mixed.tracer_ms}
+    return out
 
 
 def cpu_baseline_hex(budget_s: float = 10.0):
@@ -239,7 +270,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
-    ap.add_argument("--config", choices=("drops", "uneven"), default="drops")
+    ap.add_argument("--config", choices=("drops", "uneven", "surrogate"), default="drops")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy (torch) 4x: the FETCH_SIZE/WRITE_SIZE scale reference")
@@ -264,6 +295,20 @@ def main() -> int:
     if args.config == "uneven":
         with ip.Context(0) as ctx:
             out = run_uneven(ctx, args)
+        print(json.dumps(out))
+        return 0
+    if args.config == "surrogate":
+        with ip.Context(0) as ctx:
+            sur = run_surrogate(ctx)
+        if sur is None:
+            print("bench.py: no trained surrogate weights at " + str(SURROGATE_WEIGHTS), file=sys.stderr)
+            return 1
+        out = {"metric": "surrogate records/s at TV<=1e-2 (450x900 water-drop records, 1 MI355X)",
+               "value": sur["records_per_s"], "unit": "records/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+               "ms_per_step": sur["ms_per_64_records"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "bf16 MFMA (fp32 accumulate), f64 CDF", "data": "synthetic",
+               "config": {"workload": sur.pop("config")}}
+        out.update(sur)
         print(json.dumps(out))
         return 0
 

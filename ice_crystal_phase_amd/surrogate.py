@@ -6,15 +6,18 @@ A conditional density MLP over the (u, phi) cells, conditioned on a record's
 on the GPU by the library's BF16 MFMA kernels on the cell probabilities of
 traced records (copied device to device into the surrogate's training set),
 and served as FP64 CDF records by the same CDF kernels as traced records. The
-plain tracer stays the judge: `compare` measures total variation and |g| error
-against traced records on the device.
+plain tracer stays the judge and the fallback: `compare` measures total
+variation and |g| error against traced records on the device, `fit_gate`
+turns such measurements on a validation set into the domain the network may
+serve, and `serve` traces every record outside it.
 """
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
 import time
-from dataclasses import dataclass
+from dataclasses import asdict, dataclass, field
 
 import numpy as np
 
@@ -161,11 +164,13 @@ class Surrogate:
 
 
 def fill_training_set(sur: Surrogate, conds: np.ndarray, grid: int, chunk: int = 64,
-                      max_internal_hits: int = 9) -> float:
+                      max_internal_hits: int = 9, log=None) -> float:
     """Trace every record of `conds` and copy its cell probabilities into the
     surrogate's training slots 0..n-1. Returns the device trace time (ms)."""
     ms = 0.0
     for k0 in range(0, len(conds), chunk):
+        if log and k0 and k0 % 1024 == 0:
+            log(f"traced {k0}/{len(conds)} training records ({ms / 1e3:.1f} s device)")
         recs = drop_records(conds[k0:k0 + chunk], grid, sur.theta, sur.phi, max_internal_hits)
         with icp.Batch(sur.ctx, recs) as b:
             b.run(icp.RUN_TRACE | icp.RUN_GRAPH)
@@ -230,3 +235,95 @@ def train(sur: Surrogate, n_records: int, epochs: int, batch: int = 8, lr: float
         if time_limit is not None and time.time() - t0 > time_limit:
             break
     return TrainReport(len(losses), step, time.time() - t0, losses[-1], losses)
+
+
+# ---------------------------------------------------------------- validated domain and fallback
+RADIUS_EDGES = tuple(0.25 + 0.25 * k for k in range(12))  # 0.25 .. 3.0 mm, 11 bins
+
+
+@dataclass
+class Gate:
+    """Validated domain of a trained network: the radius bins whose validation
+    records reached total variation <= tv_limit at the given quantile. Records in
+    the other bins (or outside the edges) are traced."""
+    tv_limit: float
+    quantile: float
+    radius_edges: list
+    accept: list
+    tv_quantile: list = field(default_factory=list)
+    tv_mean: list = field(default_factory=list)
+    counts: list = field(default_factory=list)
+    validation: dict = field(default_factory=dict)
+
+    def accepts(self, conds: np.ndarray) -> np.ndarray:
+        r = np.asarray(conds, np.float64).reshape(-1, 4)[:, 2]
+        k = np.searchsorted(np.asarray(self.radius_edges), r, side="right") - 1
+        ok = (k >= 0) & (k < len(self.accept))
+        out = np.zeros(r.shape, bool)
+        out[ok] = np.asarray(self.accept, bool)[k[ok]]
+        return out
+
+    def save(self, path) -> None:
+        with open(path, "w") as f:
+            json.dump(asdict(self), f, indent=1)
+
+    @staticmethod
+    def load(path) -> "Gate":
+        with open(path) as f:
+            return Gate(**json.load(f))
+
+
+def fit_gate(sur: Surrogate, conds: np.ndarray, grid: int, tv_limit: float = 1e-2, quantile: float = 0.9,
+             edges=RADIUS_EDGES) -> Gate:
+    """Measure the network against traced validation records (grid^2 rays) and
+    accept the radius bins whose TV quantile is within tv_limit."""
+    tv, dg, _ = evaluate(sur, conds, grid)
+    r = np.asarray(conds, np.float64)[:, 2]
+    k = np.searchsorted(np.asarray(edges), r, side="right") - 1
+    accept, tq, tm, n = [], [], [], []
+    for b in range(len(edges) - 1):
+        t = tv[k == b]
+        n.append(int(t.size))
+        tq.append(float(np.quantile(t, quantile)) if t.size else float("nan"))
+        tm.append(float(t.mean()) if t.size else float("nan"))
+        accept.append(bool(t.size >= 4 and tq[-1] <= tv_limit))
+    return Gate(tv_limit, quantile, list(edges), accept, tq, tm, n,
+                {"records": int(len(conds)), "grid": int(grid), "tv_mean": float(tv.mean()),
+                 "dg_mean": float(dg.mean())})
+
+
+@dataclass
+class Served:
+    phi: np.ndarray          # [n, np + 1]
+    theta: np.ndarray        # [n, np, nt + 1]
+    by_surrogate: np.ndarray  # [n] bool
+    surrogate_ms: float      # device time of the network + CDF records
+    tracer_ms: float         # device time of the fallback trace + CDF
+
+
+def serve(sur: Surrogate, gate: Gate, conds: np.ndarray, grid: int, max_internal_hits: int = 9) -> Served:
+    """FP64 CDF records of `conds`: the network for records inside the gate's
+    domain, the tracer (grid^2 rays) for the others."""
+    conds = np.ascontiguousarray(conds, np.float32).reshape(-1, 4)
+    n = conds.shape[0]
+    use = gate.accepts(conds)
+    phi = np.zeros((n, sur.phi + 1))
+    theta = np.zeros((n, sur.phi, sur.theta + 1))
+    s_ms = t_ms = 0.0
+    idx = np.flatnonzero(use)
+    for k0 in range(0, idx.size, sur.max_records):
+        part = idx[k0:k0 + sur.max_records]
+        p, t, _ = sur.records(conds[part])
+        phi[part], theta[part] = p, t
+        s_ms += sur.timing()[1]
+    idx = np.flatnonzero(~use)
+    for k0 in range(0, idx.size, 64):
+        part = idx[k0:k0 + 64]
+        recs = drop_records(conds[part], grid, sur.theta, sur.phi, max_internal_hits)
+        with icp.Batch(sur.ctx, recs) as b:
+            b.run(icp.RUN_TRACE | icp.RUN_CDF | icp.RUN_GRAPH)
+            tr, cdf, _ = b.timing()
+            t_ms += tr + cdf
+            for i, r in enumerate(part):
+                phi[r], theta[r], _ = b.cdf(i)
+    return Served(phi, theta, use, s_ms, t_ms)

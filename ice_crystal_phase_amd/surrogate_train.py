@@ -1,8 +1,10 @@
 """Train the learned surrogate on tracer records and evaluate it against fresh
 traced records.
 
-    python -m ice_crystal_phase_amd.surrogate_train --records 4096 --grid 2048 \
-        --time-limit 300 --out surrogate.npy
+    python -m ice_crystal_phase_amd.surrogate_train --records 4096 --grid 8192 \
+        --time-limit 900 --out surrogate.npy --gate 256 --gate-out surrogate.gate.json
+    python -m ice_crystal_phase_amd.surrogate_train --epochs 0 --init surrogate.npy \
+        --gate 256 --gate-out surrogate.gate.json          # evaluation and gate only
 
 Training set: `--records` water-drop records (seed 5) at theta x phi cells,
 traced with grid^2 rays each and copied device to device into the
@@ -39,6 +41,10 @@ def main(argv=None) -> int:
     ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
     ap.add_argument("--out", default=None, help="write the trained parameters (.npy)")
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--gate", type=int, default=0,
+                    help="validation records (seed + 2) for the served domain; 0 = no gate")
+    ap.add_argument("--gate-out", default=None, help="write the gate (.json)")
+    ap.add_argument("--tv-limit", type=float, default=1e-2)
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -46,26 +52,44 @@ def main(argv=None) -> int:
 
     conds = S.drop_conditions(a.records, seed=a.seed)
     fresh = S.drop_conditions(a.eval, seed=a.seed + 1)
-    with icp.Context(0) as ctx, S.Surrogate(ctx, a.theta, a.phi, max_records=64, dataset=a.records,
+    train = a.epochs > 0
+    if not train and not a.init:
+        ap.error("--epochs 0 (evaluation only) needs --init")
+    with icp.Context(0) as ctx, S.Surrogate(ctx, a.theta, a.phi, max_records=64, dataset=a.records if train else 1,
                                             seed=a.seed) as sur:
         if a.init:
             sur.set_params(np.load(a.init, allow_pickle=False))
-        t0 = time.time()
-        trace_ms = S.fill_training_set(sur, conds, a.grid)
-        log(f"training set: {a.records} records traced in {trace_ms / 1e3:.2f} s device ({time.time() - t0:.1f} s wall)")
-        # the epoch count is capped by --time-limit; the cosine schedule spans --epochs
-        rep = S.train(sur, a.records, a.epochs, a.batch, a.lr, a.lr_min, seed=a.seed, time_limit=a.time_limit,
-                      log=log)
-        step_ms = sur.timing()[2]
-        if a.out:
-            np.save(a.out, sur.params())
+        out = {"records": a.records, "grid": a.grid, "cells": [a.theta, a.phi]}
+        if train:
+            t0 = time.time()
+            trace_ms = S.fill_training_set(sur, conds, a.grid, log=log)
+            log(f"training set: {a.records} records traced in {trace_ms / 1e3:.2f} s device "
+                f"({time.time() - t0:.1f} s wall)")
+            # the epoch count is capped by --time-limit; the cosine schedule spans --epochs
+            rep = S.train(sur, a.records, a.epochs, a.batch, a.lr, a.lr_min, seed=a.seed,
+                          time_limit=a.time_limit, log=log)
+            out.update({"epochs": rep.epochs, "steps": rep.steps, "train_seconds": rep.seconds,
+                        "loss_first": rep.losses[0], "loss_final": rep.final_loss,
+                        "step_ms_last": sur.timing()[2]})
+            if a.out:
+                np.save(a.out, sur.params())
+        gate = None
+        if a.gate:
+            gate = S.fit_gate(sur, S.drop_conditions(a.gate, seed=a.seed + 2), a.eval_grid, tv_limit=a.tv_limit)
+            log(f"gate: accept {gate.accept} tv_q90 {[round(x, 4) for x in gate.tv_quantile]}")
+            if a.gate_out:
+                gate.save(a.gate_out)
         tv, dg, eval_ms = S.evaluate(sur, fresh, a.eval_grid)
-        out = {"records": a.records, "grid": a.grid, "cells": [a.theta, a.phi], "epochs": rep.epochs,
-               "steps": rep.steps, "train_seconds": rep.seconds, "loss_first": rep.losses[0],
-               "loss_final": rep.final_loss, "step_ms_last": step_ms, "eval_records": a.eval,
-               "eval_grid": a.eval_grid, "tv_mean": float(tv.mean()), "tv_max": float(tv.max()),
-               "tv_p90": float(np.quantile(tv, 0.9)), "dg_mean": float(dg.mean()), "dg_max": float(dg.max()),
-               "eval_trace_ms": eval_ms}
+        out.update({"eval_records": a.eval, "eval_grid": a.eval_grid, "tv_mean": float(tv.mean()),
+                    "tv_max": float(tv.max()), "tv_p90": float(np.quantile(tv, 0.9)), "dg_mean": float(dg.mean()),
+                    "dg_max": float(dg.max()), "eval_trace_ms": eval_ms})
+        if gate is not None:
+            inside = gate.accepts(fresh)
+            out["gate_accept"] = gate.accept
+            out["fresh_inside"] = int(inside.sum())
+            if inside.any():
+                out["tv_inside_mean"] = float(tv[inside].mean())
+                out["tv_inside_max"] = float(tv[inside].max())
         print(json.dumps(out), flush=True)
     return 0
 

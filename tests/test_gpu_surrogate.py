@@ -147,3 +147,24 @@ def test_training_lowers_loss_and_beats_baselines(ctx):
           f"family mean {np.mean(tvm):.4f}, trained {tv1.mean():.4f} (|dg| {dg1.mean():.4f})", flush=True)
     assert rep.final_loss < rep.losses[0]
     assert tv1.mean() < tv0.mean() and tv1.mean() < np.mean(tvm)
+
+
+def test_serve_traces_outside_the_gate(ctx):
+    """Records inside the validated domain come from the network, the others
+    from the tracer: each equals the corresponding direct computation."""
+    print("case: surrogate serve with tracer fallback 45x90", flush=True)
+    nt, np_ = 45, 90
+    conds = np.array([[10.0, 520.0, 0.5, 1.0], [-40.0, 610.0, 1.2, 0.93], [60.0, 450.0, 2.0, 0.83],
+                      [-85.0, 680.0, 2.6, 0.75]], np.float32)
+    gate = S.Gate(1e-2, 0.9, list(S.RADIUS_EDGES), [True] * 5 + [False] * 6)
+    with S.Surrogate(ctx, nt, np_, max_records=4, seed=2) as sur:
+        out = S.serve(sur, gate, conds, grid=64)
+        assert out.by_surrogate.tolist() == [True, True, False, False]
+        p, t, _ = sur.records(conds[:2])
+        assert np.array_equal(out.phi[:2], p) and np.array_equal(out.theta[:2], t)
+    recs = S.drop_records(conds[2:], 64, nt, np_)
+    with ip.Batch(ctx, recs) as b:
+        b.run(ip.RUN_TRACE | ip.RUN_CDF)
+        for i in range(2):
+            phi, theta, _ = b.cdf(i)
+            assert np.array_equal(out.phi[2 + i], phi) and np.array_equal(out.theta[2 + i], theta)
