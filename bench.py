@@ -109,6 +109,21 @@ SURROGATE_WEIGHTS = ROOT / "ice_crystal_phase_amd" / "data" / "surrogate_drop_45
 SURROGATE_GATE = ROOT / "ice_crystal_phase_amd" / "data" / "surrogate_drop_450x900.gate.json"
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "trace_drop_oblate_192x2e26_traffic.md"
+
+
+def drop_traffic(args):
+    """HBM bytes per launch of icephase_trace_drop (FETCH_SIZE + WRITE_SIZE, each
+    scaled by the copy-kernel calibration) from the committed rocprofv3 summary
+    of this workload; None for other workloads or without the summary."""
+    if (args.records, args.rays_log2, args.theta, args.phi) != (192, 26, 900, 1800) or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def run_hex(ctx, rays_log2: int = 28):
     """Hexagonal column (n 1.31, a 50 um, c/a 2, ki (1,-0.4,0.3)), Fraunhofer
     lobe added, 2^rays_log2 rays, <= 12 internal interactions, 900x1800."""
@@ -201,6 +216,75 @@ def run_surrogate(ctx, n_eval: int = 64, eval_grid: int = 16384, tracer_grid: in
     return out
 
 
+def _hex_ref_sample(n: int) -> float:
+    """Seconds the compiled reference hex tracer + compensated FP64 histogram
+    (oracle/_ref, built from the reference CPU sources) takes for n rays of the
+    hex column record on this process's core."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_bind as ob
+    s = ob.hex_settings(ki=(1.0, -0.4, 0.3), total=n, max_hits=12)
+    t0 = time.perf_counter()
+    ob.ref_hex_masses(s, 900, 1800)
+    return time.perf_counter() - t0
+
+
+def _drop_port_sample(job) -> float:
+    """Seconds the FP32 oracle mirror of the drop kernel takes for rays
+    [first, first + count) of the 192-record workload's middle record."""
+    coeffs, alpha, n_int, grid, hits, nt, np_, first, count = job
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_bind as ob
+    r = ob.drop_record(coeffs=coeffs, sphere=False, alpha_deg=alpha, n_int=n_int, grid=grid, max_hits=hits)
+    t0 = time.perf_counter()
+    ob.drop_hist(r, nt, np_, precision=32, qbits=ob.quantum_bits(grid * grid), first=first, count=count)
+    return time.perf_counter() - t0
+
+
+def host_workers() -> int:
+    """Worker processes for the all-core CPU baselines: the host cores this
+    process may use, at most 16 (one GPU's share of a GPU node)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baselines_all_cores(rec, rec_index: int, nt: int, np_: int, budget_s: float = 10.0):
+    """The two CPU baselines on every host core (independent processes over
+    disjoint ray ranges; histogram partitioning is result-invariant). Runs
+    before this process touches the GPU, in forked workers."""
+    import multiprocessing as mp
+    w = host_workers()
+    out = {}
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_bind as ob
+    with mp.get_context("fork").Pool(w) as pool:
+        if ob.ref_available():
+            n = 20000
+            dt = _hex_ref_sample(n)
+            n2 = int(max(n, n * budget_s / max(dt, 1e-6)))
+            t0 = time.perf_counter()
+            pool.map(_hex_ref_sample, [n2] * w)
+            wall = time.perf_counter() - t0
+            out["hex_reference"] = {"value": w * n2 / wall, "unit": "rays/s", "cores": w, "kind": "reference",
+                                    "sample": f"{w} processes x {n2} rays of the hex column record (compiled "
+                                              f"reference trace_sample + PhaseMassAccumulator, 900x1800) in "
+                                              f"{wall:.1f} s"}
+        coeffs = tuple(rec.shape_coefficients)
+        alpha = math.degrees(math.atan2(-rec.incident[1], rec.incident[0]))
+        total = rec.grid_w * rec.grid_h
+        start = total // 2 - rec.grid_w * (rec.grid_h // 8)
+        job = (coeffs, alpha, rec.interior_index, rec.grid_w, rec.max_internal_hits, nt, np_, start, 1 << 14)
+        dt = _drop_port_sample(job)
+        per = int(min(rec.grid_w * (rec.grid_h // 4) // w, max(1 << 14, (1 << 14) * budget_s / max(dt, 1e-6))))
+        jobs = [job[:7] + (start + k * per, per) for k in range(w)]
+        t0 = time.perf_counter()
+        pool.map(_drop_port_sample, jobs)
+        wall = time.perf_counter() - t0
+        out["drop_port"] = {"value": w * per / wall, "unit": "rays/s", "cores": w, "kind": "port",
+                            "sample": f"{w} processes x {per} consecutive rays through the middle of record "
+                                      f"{rec_index} (FP32 oracle mirror, same grid, "
+                                      f"binning and fixed-point histogram) in {wall:.1f} s"}
+    return out
+
+
 def cpu_baseline_hex(budget_s: float = 10.0):
     """The compiled reference hex tracer + compensated FP64 histogram
     (oracle/_ref, built from the reference CPU sources) on one host core."""
@@ -209,15 +293,9 @@ def cpu_baseline_hex(budget_s: float = 10.0):
     if not ob.ref_available():
         return None
     n = 20000
-    s = ob.hex_settings(ki=(1.0, -0.4, 0.3), total=n, max_hits=12)
-    t0 = time.perf_counter()
-    ob.ref_hex_masses(s, 900, 1800)
-    dt = time.perf_counter() - t0
+    dt = _hex_ref_sample(n)
     n2 = int(max(n, n * budget_s / max(dt, 1e-6)))
-    s = ob.hex_settings(ki=(1.0, -0.4, 0.3), total=n2, max_hits=12)
-    t0 = time.perf_counter()
-    ob.ref_hex_masses(s, 900, 1800)
-    dt = time.perf_counter() - t0
+    dt = _hex_ref_sample(n2)
     return {"value": n2 / dt, "unit": "rays/s", "cores": 1, "kind": "reference",
             "sample": f"{n2} rays of the hex column record (compiled reference trace_sample + "
                       f"PhaseMassAccumulator, 900x1800) in {dt:.1f} s"}
@@ -313,6 +391,10 @@ def main() -> int:
         return 0
 
     recs, coeffs, side = workload(args.records, args.rays_log2, args.theta, args.phi)
+    # all-core CPU baselines first, in forked workers, before this process opens the GPU
+    all_cores = None
+    if not args.no_cpu_baseline:
+        all_cores = cpu_baselines_all_cores(recs[len(recs) // 2], len(recs) // 2, args.theta, args.phi)
     ctx = ip.Context(0)
     batch = ip.Batch(ctx, recs)
     if args.tile_rays or args.blocks:
@@ -336,6 +418,7 @@ def main() -> int:
     trace_avg = float(np.mean(trace_ms)) if trace_ms[0] >= 0 else float("nan")
     cdf_avg = float(np.mean(cdf_ms)) if cdf_ms[0] >= 0 else float("nan")
     flops = census_flops(stats, ["drop"] * len(stats))
+    atomics = sum(int(st.samples) for st in stats)  # one u64 atomicAdd per binned sample
     achieved = flops / (trace_avg / 1e3) / 1e12 if trace_avg == trace_avg else None
     # CDF HBM figure (algorithmic bytes: read 8 B per cell, write phi + theta)
     nt, np_ = args.theta, args.phi
@@ -369,7 +452,9 @@ def main() -> int:
         "roofline": {"kernel": "icephase_trace_drop", "bound": "valu", "achieved": achieved,
                      "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
                      "frac": (achieved / PEAK_TFLOPS_FP32) if achieved else None,
-                     "traffic": None, "flops_per_launch": flops},
+                     "traffic": drop_traffic(args), "flops_per_launch": flops},
+        "atomics": {"count_per_launch": atomics,
+                    "per_s": atomics / (trace_avg / 1e3) if trace_avg == trace_avg else None},
     }
     batch.close()
     if not args.no_extras:
@@ -382,11 +467,18 @@ def main() -> int:
                                "sample": f"{n} rays of record {len(recs)//2} (FP32 oracle mirror, same grid, "
                                          f"binning and fixed-point histogram) in {dt:.1f} s"}
         out["gpu_over_cpu"] = rays_per_s / rps
+        out["cpu_baseline_all_cores"] = all_cores.get("drop_port")
+        out["gpu_over_cpu_all_cores"] = rays_per_s / all_cores["drop_port"]["value"]
+        if "hex_reference" in all_cores:
+            out["cpu_baseline_hex_all_cores"] = all_cores["hex_reference"]
         if not args.no_extras:
             hb = cpu_baseline_hex()
             out["cpu_baseline_hex"] = hb
             if hb:
                 out["hex"]["gpu_over_cpu_reference"] = out["hex"]["rays_per_s"] / hb["value"]
+            if "hex_reference" in all_cores:
+                out["hex"]["gpu_over_cpu_reference_all_cores"] = (out["hex"]["rays_per_s"] /
+                                                                  all_cores["hex_reference"]["value"])
     ctx.close()
     print(json.dumps(out))
     return 0

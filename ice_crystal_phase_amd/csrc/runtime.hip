@@ -152,6 +152,16 @@ struct iph_batch {
     }
 };
 
+namespace icephase {
+bool batch_record_grid(const iph_batch* b, uint32_t rec, uint32_t& nt, uint32_t& np)
+{
+    if (!b || rec >= b->nrec) return false;
+    nt = b->host[rec].nt;
+    np = b->host[rec].np;
+    return true;
+}
+}  // namespace icephase
+
 extern "C" {
 
 int iph_abi_version(void) { return IPH_ABI_VERSION; }

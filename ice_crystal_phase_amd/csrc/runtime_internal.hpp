@@ -57,6 +57,10 @@ struct DevBuf {
     }
 };
 
+// Grid (theta_count, phi_count) of record rec of a batch; false for an invalid
+// record index.
+bool batch_record_grid(const iph_batch* b, uint32_t rec, uint32_t& nt, uint32_t& np);
+
 }  // namespace icephase
 
 #define IPH_HIP(ctx, call)                                                                                  \

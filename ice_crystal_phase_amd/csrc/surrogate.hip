@@ -3,18 +3,19 @@
 // accumulation, trained and served by the kernels in this file.
 //
 // Network (per cell of an nt x np grid, conditioned on one record):
-//   x = [t, sin(2 pi k v), cos(2 pi k v) (k = 1..16), c0..c3]   (37 inputs)
-//       t = cell-centre theta / pi (the grid's uniform coordinate of u),
-//       v = cell-centre (phi + pi) / 2 pi, c = normalized (alpha, lambda,
-//       radius, aspect);
+//   x = [t, sin(2 pi k v), cos(2 pi k v) (k = 1..16),
+//        sin(pi f t), cos(pi f t) (f in kThetaFreq, 13), c0..c3]   (63 inputs)
+//       t = (i + 1/2) / nt of the cell's theta row (the grid's uniform
+//       coordinate of u), v = cell-centre (phi + pi) / 2 pi, c = normalized
+//       (alpha, lambda, radius, aspect);
 //   h1 = silu(W1 x + b1), h2..h4 = silu(W h + b)       (4 x 256)
 //   f = w5 . h4 + b5,  p(cell) = softmax over cells of (f + log du_i)
 // so f is a log density in u (cells carry du_i / np of the u-v square).
-// The first layer is separable over the grid (t depends on the theta row, the
-// Fourier features on the phi column, c on the record) and runs in fp32 on the
-// VALU: z1 = W1[:,0] t_i + (b1 + W1[:,33:37] c) + P[j], with the phi table
-// P = W1[:,1:33] fourier(v_j) rebuilt whenever the weights change, so
-// neighbouring theta rows stay distinct. Hidden activations are bf16, every
+// The first layer is separable over the grid (the t inputs depend on the theta
+// row, the v features on the phi column, c on the record) and runs in fp32 on
+// the VALU: z1 = T[i] + (b1 + W1_c c) + P[j], with the theta table T = W1_t
+// x_t(t_i) and the phi table P = W1_v x_v(v_j) rebuilt whenever the weights
+// change, so neighbouring theta rows stay distinct. Hidden activations are bf16, every
 // MFMA product accumulates in fp32. The weight gradient of layer 1 splits x
 // into bf16 hi + lo parts (two MFMA passes).
 //
@@ -23,7 +24,8 @@
 //   sur_forward     fused 4-layer forward per 64-row tile (activations in LDS),
 //                   logits, and the pre-activations Z_l when training;
 //   sur_lse_*       log-sum-exp per record (fixed-order two-level reduction);
-//   sur_dlogits     dL/df = p - P and the loss;
+//   sur_dlogits     dL/df = p - P and the loss (over the folded rows);
+//   sur_probs/fold  folded rows <-> full-grid cells;
 //   sur_backward    fused data gradients layers 4 -> 1 per tile (dZ_l to HBM);
 //   sur_wgrad       weight gradients dW_l = dZ_l^T H_{l-1}, split over rows,
 //                   partial tiles reduced in fixed order (deterministic);
@@ -147,32 +149,34 @@ __global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const 
     const int n0 = 64 * wave + 4 * q;  // first of the lane's columns in each 16-column block
     f32x4 acc[RB][4];
     {
-        // layer 1: z = W1[:,0] t_i + (b1 + W1[:,33:37] c) + P[j]
+        // layer 1: z = (T[i] + (b1 + W1_c c)) + P[j]
         const float* c = conds + 4 * rec;
         const float c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-        float wt[4][4], base[4][4];
+        float base[4][4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int col = n0 + 16 * cb + i;
-                const float* w = net.w1 + col * kInPad;
-                wt[cb][i] = w[0];
-                base[cb][i] = fmaf(w[36], c3, fmaf(w[35], c2, fmaf(w[34], c1, fmaf(w[33], c0, net.b[0][col]))));
+                const float* w = net.w1 + col * kInPad + kColCond;
+                base[cb][i] = fmaf(w[3], c3, fmaf(w[2], c2, fmaf(w[1], c1, fmaf(w[0], c0, net.b[0][col]))));
             }
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
             const uint32_t row = row0 + 16 * rb + c16;
-            const uint32_t ti = row / st.np, j = row - ti * st.np;
-            const float t = ((float)ti + 0.5f) / (float)st.nt;
+            uint32_t ti = row / st.nq;
+            const uint32_t j = row - ti * st.nq;  // canonical column
+            ti = ti < st.nt ? ti : st.nt - 1;     // padding rows (their logits are -inf)
+            const float* T = net.ptheta + (size_t)ti * 256 + n0;
             const float* P = net.pphi + (size_t)j * 256 + n0;
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
+                const float4 tv = *reinterpret_cast<const float4*>(T + 16 * cb);
                 const float4 p = *reinterpret_cast<const float4*>(P + 16 * cb);
-                acc[rb][cb][0] = fmaf(wt[cb][0], t, base[cb][0]) + p.x;
-                acc[rb][cb][1] = fmaf(wt[cb][1], t, base[cb][1]) + p.y;
-                acc[rb][cb][2] = fmaf(wt[cb][2], t, base[cb][2]) + p.z;
-                acc[rb][cb][3] = fmaf(wt[cb][3], t, base[cb][3]) + p.w;
+                acc[rb][cb][0] = (tv.x + base[cb][0]) + p.x;
+                acc[rb][cb][1] = (tv.y + base[cb][1]) + p.y;
+                acc[rb][cb][2] = (tv.z + base[cb][2]) + p.z;
+                acc[rb][cb][3] = (tv.w + base[cb][3]) + p.w;
             }
         }
     }
@@ -302,20 +306,27 @@ __global__ void __launch_bounds__(64) sur_lse_final(const float2* __restrict__ p
     }
 }
 
-// dL/df = p - P (record 0), loss partials sum P (lse - f).
-__global__ void __launch_bounds__(256) sur_dlogits(const float* __restrict__ logits, const float* __restrict__ lse,
-                                                   const float* __restrict__ target, uint32_t M, uint32_t Mpad,
+// log of the number of full-grid cells a folded row stands for (1 or 2)
+__device__ inline float log_mult(const Static& st, uint32_t q)
+{
+    return st.canon[q] == st.mirr[q] ? 0.0f : 0.69314718f;
+}
+
+// dL/df = p - P over the folded rows (record 0); loss partials sum P (lse - f
+// + log mult), which is the full-grid cross entropy -sum P_cell log p_cell.
+__global__ void __launch_bounds__(256) sur_dlogits(Static st, const float* __restrict__ logits,
+                                                   const float* __restrict__ lse, const float* __restrict__ target,
                                                    float* __restrict__ dF, float* __restrict__ loss_part)
 {
     __shared__ float sh[256];
     const uint32_t r = blockIdx.x * 256 + threadIdx.x;
     float l = 0;
-    if (r < Mpad) {
+    if (r < st.Mpad) {
         const float f = logits[r];
-        const float P = r < M ? target[r] : 0.0f;
+        const float P = r < st.Mh ? target[r] : 0.0f;
         const float p = f > -INFINITY ? __expf(f - lse[0]) : 0.0f;
         dF[r] = p - P;
-        if (P > 0) l = P * (lse[0] - f);
+        if (P > 0) l = P * ((lse[0] - f) + log_mult(st, r % st.nq));
     }
     sh[threadIdx.x] = l;
     __syncthreads();
@@ -326,18 +337,39 @@ __global__ void __launch_bounds__(256) sur_dlogits(const float* __restrict__ log
     if (threadIdx.x == 0) loss_part[blockIdx.x] = sh[0];
 }
 
-// probabilities p = exp(f - lse) of every record, theta-major, FP64 (CDF input)
-// and FP32 copies.
-__global__ void __launch_bounds__(256) sur_probs(const float* __restrict__ logits, const float* __restrict__ lse,
-                                                 uint32_t M, uint32_t Mpad, double* __restrict__ mass,
+// full-grid cell probabilities of every record, theta-major: a folded row's
+// p = exp(f - lse - log mult) goes to both cells of its pair. FP64 (CDF
+// input) and FP32 copies.
+__global__ void __launch_bounds__(256) sur_probs(Static st, const float* __restrict__ logits,
+                                                 const float* __restrict__ lse, double* __restrict__ mass,
                                                  float* __restrict__ probs)
 {
     const uint32_t rec = blockIdx.y;
-    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < M; r += gridDim.x * 256) {
-        const float f = logits[(size_t)rec * Mpad + r];
-        const float p = f > -INFINITY ? __expf(f - lse[rec]) : 0.0f;
-        if (mass) mass[(size_t)rec * M + r] = (double)p;
-        if (probs) probs[(size_t)rec * M + r] = p;
+    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < st.Mh; r += gridDim.x * 256) {
+        const uint32_t i = r / st.nq, q = r - i * st.nq;
+        const float f = logits[(size_t)rec * st.Mpad + r];
+        const float p = f > -INFINITY ? __expf((f - lse[rec]) - log_mult(st, q)) : 0.0f;
+        const size_t c0 = (size_t)rec * st.M + (size_t)i * st.np + st.canon[q];
+        const size_t c1 = (size_t)rec * st.M + (size_t)i * st.np + st.mirr[q];
+        if (mass) {
+            mass[c0] = (double)p;
+            mass[c1] = (double)p;
+        }
+        if (probs) {
+            probs[c0] = p;
+            probs[c1] = p;
+        }
+    }
+}
+
+// folded training target: row (i, q) = P(i, canon q) + P(i, mirror q)
+__global__ void __launch_bounds__(256) sur_fold(Static st, const float* __restrict__ full, float* __restrict__ folded)
+{
+    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < st.Mh; r += gridDim.x * 256) {
+        const uint32_t i = r / st.nq, q = r - i * st.nq;
+        const float* row = full + (size_t)i * st.np;
+        const uint32_t a = st.canon[q], b = st.mirr[q];
+        folded[r] = a == b ? row[a] : row[a] + row[b];
     }
 }
 
@@ -533,7 +565,7 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
             }
         }
     };
-    float ch[4], cl[4];  // condition inputs (columns 33..36 of the input layer), hi and lo
+    float ch[4], cl[4];  // condition inputs of the input layer, hi and lo
     if (IN == 64) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -550,8 +582,8 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
                 store_cols(dZT + 4 * (task >> 3) * S + 4 * rg, S, v[p]);
             } else if (IN == 64) {
                 const int c = 4 * ((task - 512) >> 3), table = c >> 6, cc = c & 63;
-                if (cc == 32 || cc == 36) {
-                    // substitute the record's condition for columns 33..36
+                if (cc + 3 >= kColCond && cc < kColCond + 4) {
+                    // substitute the record's condition for its four columns
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         float4 x = unpack4(v[p][k]);
@@ -559,7 +591,8 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const int col = cc + j;
-                            if (col >= 33 && col <= 36) xs[j] = table ? cl[col - 33] : ch[col - 33];
+                            if (col >= kColCond && col < kColCond + 4)
+                                xs[j] = table ? cl[col - kColCond] : ch[col - kColCond];
                         }
                         v[p][k] = pack4(xs[0], xs[1], xs[2], xs[3]);
                     }
@@ -666,17 +699,29 @@ __global__ void __launch_bounds__(256) sur_pack(const float* __restrict__ p, __b
     }
 }
 
-// phi table of layer 1: P[j][o] = sum_k W1[o][1 + k] feat[j][k] (the 32
-// Fourier features of column j), fp32; block j, thread o.
+// Layer-1 tables, fp32; block j, thread o:
+//   phi table   P[q][o] = sum_k W1[o][kColPhi + k] feat[q][k]     (16 phi features of canonical column q)
+//   theta table T[i][o] = W1[o][0] t_i + sum_k W1[o][kColTheta + k] tfeat[i][1 + k]
 __global__ void __launch_bounds__(256) sur_phi_table(const float* __restrict__ p, const float* __restrict__ feat,
                                                      float* __restrict__ P)
 {
     const uint32_t j = blockIdx.x, o = threadIdx.x;
-    const float* w = p + kOffW1 + o * kInPad + 1;
-    const float* f = feat + (size_t)j * 2 * kFourier;
+    const float* w = p + kOffW1 + o * kInPad + kColPhi;
+    const float* f = feat + (size_t)j * kFourier;
     float s = 0;
-    for (int k = 0; k < 2 * kFourier; ++k) s = fmaf(w[k], f[k], s);
+    for (int k = 0; k < kFourier; ++k) s = fmaf(w[k], f[k], s);
     P[(size_t)j * 256 + o] = s;
+}
+
+__global__ void __launch_bounds__(256) sur_theta_table(const float* __restrict__ p, const float* __restrict__ tfeat,
+                                                       float* __restrict__ T)
+{
+    const uint32_t i = blockIdx.x, o = threadIdx.x;
+    const float* w = p + kOffW1 + o * kInPad;
+    const float* f = tfeat + (size_t)i * kThetaIn;
+    float s = w[0] * f[0];
+    for (int k = 0; k < 2 * kThetaFreqs; ++k) s = fmaf(w[kColTheta + k], f[1 + k], s);
+    T[(size_t)i * 256 + o] = s;
 }
 
 // out[0] = sum of x[0..n) in FP64: 256 strided lane sums added in lane order
@@ -798,7 +843,7 @@ hipError_t dlogits(const Static& st, const float* logits, const float* lse_v, co
                    float* loss_part, float* loss, hipStream_t s)
 {
     const uint32_t blocks = (st.Mpad + 255) / 256;
-    hipLaunchKernelGGL(sur_dlogits, dim3(blocks), dim3(256), 0, s, logits, lse_v, target, st.M, st.Mpad, dF, loss_part);
+    hipLaunchKernelGGL(sur_dlogits, dim3(blocks), dim3(256), 0, s, st, logits, lse_v, target, dF, loss_part);
     hipLaunchKernelGGL(sur_sum, dim3(1), dim3(256), 0, s, loss_part, blocks, loss);
     return hipGetLastError();
 }
@@ -806,7 +851,13 @@ hipError_t dlogits(const Static& st, const float* logits, const float* lse_v, co
 hipError_t probs(const Static& st, const float* logits, const float* lse_v, uint32_t nrec, double* mass, float* p,
                  hipStream_t s)
 {
-    hipLaunchKernelGGL(sur_probs, dim3(256, nrec), dim3(256), 0, s, logits, lse_v, st.M, st.Mpad, mass, p);
+    hipLaunchKernelGGL(sur_probs, dim3(256, nrec), dim3(256), 0, s, st, logits, lse_v, mass, p);
+    return hipGetLastError();
+}
+
+hipError_t fold(const Static& st, const float* full, float* folded, hipStream_t s)
+{
+    hipLaunchKernelGGL(sur_fold, dim3(256), dim3(256), 0, s, st, full, folded);
     return hipGetLastError();
 }
 
@@ -847,10 +898,11 @@ hipError_t adam(float* w, float* m, float* v, float* g, uint32_t n, float lr, fl
     return hipGetLastError();
 }
 
-hipError_t pack(const float* p, __bf16* w, __bf16* wt, const float* feat, uint32_t np, float* pphi, hipStream_t s)
+hipError_t pack(const float* p, __bf16* w, __bf16* wt, const Static& st, float* pphi, float* ptheta, hipStream_t s)
 {
     hipLaunchKernelGGL(sur_pack, dim3(3 * 65536 / 256), dim3(256), 0, s, p, w, wt);
-    hipLaunchKernelGGL(sur_phi_table, dim3(np), dim3(256), 0, s, p, feat, pphi);
+    hipLaunchKernelGGL(sur_phi_table, dim3(st.nq), dim3(256), 0, s, p, st.feat, pphi);
+    hipLaunchKernelGGL(sur_theta_table, dim3(st.nt), dim3(256), 0, s, p, st.tfeat, ptheta);
     return hipGetLastError();
 }
 

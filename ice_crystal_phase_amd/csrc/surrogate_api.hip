@@ -67,7 +67,8 @@ struct iph_sur {
     uint64_t step = 0;
     DevBuf<float> params, adam_m, adam_v, grad;
     DevBuf<__bf16> w, wt, xhi, xlo, Z, dZ;
-    DevBuf<float> pphi, feat, logw, logits, dF, lse, loss_part, loss, conds, part, part2, part_w5, part_b5, part_db;
+    DevBuf<uint32_t> canon, mirr;
+    DevBuf<float> pphi, ptheta, feat, tfeat, logw, logits, dF, lse, loss_part, loss, conds, part, part2, part_w5, part_b5, part_db;
     DevBuf<float2> lse_part;
     DevBuf<double> mass;
     // training set: probabilities [dataset][M], normalized conditions [dataset][4]
@@ -91,6 +92,7 @@ struct iph_sur {
         Net n;
         n.w1 = params.p + kOffW1;
         n.pphi = pphi.p;
+        n.ptheta = ptheta.p;
         for (int l = 0; l < 3; ++l) {
             n.w[l] = w.p + (size_t)l * 65536;
             n.wt[l] = wt.p + (size_t)l * 65536;
@@ -144,8 +146,20 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     Static& st = s->st;
     st.nt = cfg->theta_count;
     st.np = cfg->phi_count;
+    if (st.np % 2) return fail(ctx, IPH_E_INVALID, "The surrogate needs an even phi_count (mirror-symmetric records).");
     st.M = st.nt * st.np;
-    st.Mpad = (st.M + 127) / 128 * 128;
+    // folded grid: canonical column j pairs with mirror(j) = (np/2 - 1 - j) mod np
+    std::vector<uint32_t> canon, mirr;
+    for (uint32_t j = 0; j < st.np; ++j) {
+        const uint32_t m = (uint32_t)(((int64_t)st.np / 2 - 1 - (int64_t)j + st.np) % st.np);
+        if (j <= m) {
+            canon.push_back(j);
+            mirr.push_back(m);
+        }
+    }
+    st.nq = (uint32_t)canon.size();
+    st.Mh = st.nt * st.nq;
+    st.Mpad = (st.Mh + 127) / 128 * 128;
     // weight-gradient splits: rows per split a multiple of 32, ~256 splits
     st.rows_per_split = std::max<uint32_t>(32, (st.Mpad / 256 + 31) / 32 * 32);
     st.splits = (st.Mpad + st.rows_per_split - 1) / st.rows_per_split;
@@ -154,38 +168,54 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     std::string err;
     if (!make_u_edges(st.nt, ue, err)) return fail(ctx, IPH_E_INVALID, err);
     std::vector<uint16_t> xh((size_t)st.Mpad * 64, 0), xl((size_t)st.Mpad * 64, 0);
-    std::vector<float> lw(st.Mpad, -INFINITY), feat((size_t)st.np * 2 * kFourier);
-    const double two_pi = 6.283185307179586476925286766559;
-    for (uint32_t j = 0; j < st.np; ++j) {
-        const double v = (j + 0.5) / st.np;
-        for (int k = 1; k <= kFourier; ++k) {
-            feat[(size_t)j * 2 * kFourier + k - 1] = (float)std::sin(two_pi * k * v);
-            feat[(size_t)j * 2 * kFourier + kFourier + k - 1] = (float)std::cos(two_pi * k * v);
+    std::vector<float> lw(st.Mpad, -INFINITY), feat((size_t)st.nq * kFourier), tfeat((size_t)st.nt * kThetaIn);
+    const double pi = 3.141592653589793238462643383280, two_pi = 2 * pi;
+    // record-independent inputs in FP64: xt[i] = [t, sin(pi f t), cos(pi f t)],
+    // xv[q] = [cos(2 pi k (v - 1/4))] of canonical column q
+    std::vector<double> xt((size_t)st.nt * kThetaIn), xv((size_t)st.nq * kFourier);
+    for (uint32_t i = 0; i < st.nt; ++i) {
+        const double t = (i + 0.5) / st.nt;
+        double* x = &xt[(size_t)i * kThetaIn];
+        x[0] = t;
+        for (int m = 0; m < kThetaFreqs; ++m) {
+            x[1 + m] = std::sin(pi * kThetaFreq[m] * t);
+            x[1 + kThetaFreqs + m] = std::cos(pi * kThetaFreq[m] * t);
         }
     }
+    for (uint32_t q = 0; q < st.nq; ++q) {
+        const double w = (canon[q] + 0.5) / st.np - 0.25;
+        for (int k = 1; k <= kFourier; ++k) xv[(size_t)q * kFourier + k - 1] = std::cos(two_pi * k * w);
+    }
+    for (size_t e = 0; e < tfeat.size(); ++e) tfeat[e] = (float)xt[e];
+    for (size_t e = 0; e < feat.size(); ++e) feat[e] = (float)xv[e];
     for (uint32_t i = 0; i < st.nt; ++i)
-        for (uint32_t j = 0; j < st.np; ++j) {
-            const size_t r = (size_t)i * st.np + j;
+        for (uint32_t q = 0; q < st.nq; ++q) {
+            const size_t r = (size_t)i * st.nq + q;
             double x[kStatic];
-            x[0] = (i + 0.5) / st.nt;
-            const double v = (j + 0.5) / st.np;
-            for (int k = 1; k <= kFourier; ++k) {
-                x[k] = std::sin(two_pi * k * v);
-                x[kFourier + k] = std::cos(two_pi * k * v);
-            }
+            x[0] = xt[(size_t)i * kThetaIn];
+            for (int k = 0; k < kFourier; ++k) x[kColPhi + k] = xv[(size_t)q * kFourier + k];
+            for (int k = 0; k < 2 * kThetaFreqs; ++k) x[kColTheta + k] = xt[(size_t)i * kThetaIn + 1 + k];
             for (int c = 0; c < kStatic; ++c) {
                 const uint16_t h = host_bf16((float)x[c]);
                 xh[r * 64 + c] = h;
                 xl[r * 64 + c] = host_bf16((float)(x[c] - (double)host_f32(h)));
             }
-            lw[r] = (float)std::log(ue[i + 1] - ue[i]);
+            lw[r] = (float)(std::log(ue[i + 1] - ue[i]) + (canon[q] == mirr[q] ? 0.0 : std::log(2.0)));
         }
+    IPH_HIP(ctx, s->canon.alloc(st.nq));
+    IPH_HIP(ctx, s->mirr.alloc(st.nq));
+    IPH_HIP(ctx, hipMemcpy(s->canon.p, canon.data(), st.nq * 4, hipMemcpyHostToDevice));
+    IPH_HIP(ctx, hipMemcpy(s->mirr.p, mirr.data(), st.nq * 4, hipMemcpyHostToDevice));
+    st.canon = s->canon.p;
+    st.mirr = s->mirr.p;
     const size_t zsz = (size_t)4 * st.Mpad * 256;
     IPH_HIP(ctx, s->xhi.alloc(xh.size()));
     IPH_HIP(ctx, s->xlo.alloc(xl.size()));
     IPH_HIP(ctx, s->logw.alloc(st.Mpad));
     IPH_HIP(ctx, s->feat.alloc(feat.size()));
     IPH_HIP(ctx, hipMemcpy(s->feat.p, feat.data(), feat.size() * 4, hipMemcpyHostToDevice));
+    IPH_HIP(ctx, s->tfeat.alloc(tfeat.size()));
+    IPH_HIP(ctx, hipMemcpy(s->tfeat.p, tfeat.data(), tfeat.size() * 4, hipMemcpyHostToDevice));
     IPH_HIP(ctx, hipMemcpy(s->xhi.p, xh.data(), xh.size() * 2, hipMemcpyHostToDevice));
     IPH_HIP(ctx, hipMemcpy(s->xlo.p, xl.data(), xl.size() * 2, hipMemcpyHostToDevice));
     IPH_HIP(ctx, hipMemcpy(s->logw.p, lw.data(), lw.size() * 4, hipMemcpyHostToDevice));
@@ -193,11 +223,13 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     st.xlo = s->xlo.p;
     st.logw = s->logw.p;
     st.feat = s->feat.p;
+    st.tfeat = s->tfeat.p;
     IPH_HIP(ctx, s->params.alloc(kParamsPad));
     IPH_HIP(ctx, s->adam_m.alloc(kParamsPad));
     IPH_HIP(ctx, s->adam_v.alloc(kParamsPad));
     IPH_HIP(ctx, s->grad.alloc(kParamsPad));
-    IPH_HIP(ctx, s->pphi.alloc((size_t)st.np * 256));
+    IPH_HIP(ctx, s->pphi.alloc((size_t)st.nq * 256));
+    IPH_HIP(ctx, s->ptheta.alloc((size_t)st.nt * 256));
     IPH_HIP(ctx, s->w.alloc(3 * 65536));
     IPH_HIP(ctx, s->wt.alloc(3 * 65536));
     IPH_HIP(ctx, s->Z.alloc(zsz));
@@ -215,7 +247,7 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     IPH_HIP(ctx, s->part_b5.alloc(st.Mpad / kTileRows));
     IPH_HIP(ctx, s->part_db.alloc((size_t)4 * (st.Mpad / kTileRows) * 256));
     IPH_HIP(ctx, s->mass.alloc((size_t)cfg->max_records * st.M));
-    IPH_HIP(ctx, s->data.alloc((size_t)cfg->dataset * st.M));
+    IPH_HIP(ctx, s->data.alloc((size_t)cfg->dataset * st.Mh));
     IPH_HIP(ctx, s->data_conds.alloc((size_t)4 * std::max<uint32_t>(cfg->dataset, 1)));
     s->filled.assign(cfg->dataset, 0);
     IPH_HIP(ctx, s->pred.alloc((size_t)cfg->max_records * st.M));
@@ -285,7 +317,7 @@ int iph_sur_set_params(iph_sur* s, const float* packed)
     for (float v : dev)
         if (!std::isfinite(v)) return sur_fail(s, "Parameters must be finite.");
     IPH_HIP(ctx, hipMemcpy(s->params.p, dev.data(), kParamsPad * 4, hipMemcpyHostToDevice));
-    IPH_HIP(ctx, pack(s->params.p, s->w.p, s->wt.p, s->feat.p, s->st.np, s->pphi.p, ctx->stream));
+    IPH_HIP(ctx, pack(s->params.p, s->w.p, s->wt.p, s->st, s->pphi.p, s->ptheta.p, ctx->stream));
     IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return IPH_OK;
 }
@@ -301,7 +333,7 @@ static int accumulate_gradient(iph_sur* s, uint32_t slot, uint32_t k)
     const Static& st = s->st;
     const Net n = s->net();
     const float* cond = s->data_conds.p + 4 * (size_t)slot;
-    const float* d_target = s->data.p + (size_t)slot * st.M;
+    const float* d_target = s->data.p + (size_t)slot * st.Mh;
     const uint32_t loss_slot = k;
     IPH_HIP(ctx, forward(n, st, cond, 1, true, s->logits.p, s->Z.p, q));
     IPH_HIP(ctx, lse(st, s->logits.p, 1, s->lse_part.p, s->lse.p, q));
@@ -331,8 +363,13 @@ int iph_sur_set_record(iph_sur* s, uint32_t slot, iph_batch* b, uint32_t rec, co
     float c[4];
     if (!normalize_cond(cond, c)) return sur_fail(s, "Invalid condition (alpha, wavelength, radius, aspect).");
     IPH_HIP(ctx, hipSetDevice(ctx->device));
-    const int rc = iph_batch_probabilities(b, rec, s->data.p + (size_t)slot * s->st.M);
+    uint32_t bnt = 0, bnp = 0;
+    if (!batch_record_grid(b, rec, bnt, bnp) || bnt != s->st.nt || bnp != s->st.np)
+        return sur_fail(s, "The traced record's grid differs from the surrogate's.");
+    const int rc = iph_batch_probabilities(b, rec, s->traced.p);
     if (rc != IPH_OK) return fail(ctx, rc, std::string("iph_batch_probabilities: ") + iph_last_error(nullptr));
+    IPH_HIP(ctx, fold(s->st, s->traced.p, s->data.p + (size_t)slot * s->st.Mh, ctx->stream));
+    IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     IPH_HIP(ctx, hipMemcpy(s->data_conds.p + 4 * (size_t)slot, c, sizeof c, hipMemcpyHostToDevice));
     s->filled[slot] = 1;
     return IPH_OK;
@@ -346,7 +383,9 @@ int iph_sur_set_target(iph_sur* s, uint32_t slot, const float* host_probs, const
     if (!normalize_cond(cond, c)) return sur_fail(s, "Invalid condition (alpha, wavelength, radius, aspect).");
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    IPH_HIP(ctx, hipMemcpy(s->data.p + (size_t)slot * s->st.M, host_probs, (size_t)s->st.M * 4, hipMemcpyHostToDevice));
+    IPH_HIP(ctx, hipMemcpy(s->traced.p, host_probs, (size_t)s->st.M * 4, hipMemcpyHostToDevice));
+    IPH_HIP(ctx, fold(s->st, s->traced.p, s->data.p + (size_t)slot * s->st.Mh, ctx->stream));
+    IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     IPH_HIP(ctx, hipMemcpy(s->data_conds.p + 4 * (size_t)slot, c, sizeof c, hipMemcpyHostToDevice));
     s->filled[slot] = 1;
     return IPH_OK;
@@ -389,7 +428,7 @@ int iph_sur_train_step(iph_sur* s, const uint32_t* slots, uint32_t n, float lr, 
     const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)s->step)));
     const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)s->step)));
     IPH_HIP(ctx, adam(s->params.p, s->adam_m.p, s->adam_v.p, s->grad.p, kParams, lr, 1.0f / n, c1, c2, q));
-    IPH_HIP(ctx, pack(s->params.p, s->w.p, s->wt.p, s->feat.p, s->st.np, s->pphi.p, q));
+    IPH_HIP(ctx, pack(s->params.p, s->w.p, s->wt.p, s->st, s->pphi.p, s->ptheta.p, q));
     IPH_HIP(ctx, hipEventRecord(s->ev[1], q));
     if (loss) IPH_HIP(ctx, hipMemcpyAsync(loss, s->loss.p, n * 4, hipMemcpyDeviceToHost, q));
     IPH_HIP(ctx, hipStreamSynchronize(q));
@@ -444,6 +483,9 @@ int iph_sur_compare(iph_sur* s, const float* conds, uint32_t n, iph_batch* b, co
     if (rc != IPH_OK) return rc;
     std::vector<double> part((size_t)kCompareBlocks * 2);
     for (uint32_t k = 0; k < n; ++k) {
+        uint32_t bnt = 0, bnp = 0;
+        if (!batch_record_grid(b, recs[k], bnt, bnp) || bnt != s->st.nt || bnp != s->st.np)
+            return sur_fail(s, "The traced record's grid differs from the surrogate's.");
         rc = iph_batch_probabilities(b, recs[k], s->traced.p);
         if (rc != IPH_OK) return fail(ctx, rc, std::string("iph_batch_probabilities: ") + iph_last_error(nullptr));
         IPH_HIP(ctx, compare(s->pred.p + (size_t)k * s->st.M, s->traced.p, s->st.M, s->st.np, s->row_cos.p,

@@ -222,9 +222,12 @@ int iph_batch_probabilities(iph_batch* b, uint32_t rec, float* d_out);
  * learned part, README.md:31) ------------------------------------------------
  * A conditional density MLP over the (u, phi) cells of a theta_count x
  * phi_count grid, conditioned on one record's (inclination deg, wavelength nm,
- * equal-volume radius mm, aspect ratio): 37 inputs (theta/pi, 16 Fourier pairs
- * of the azimuth, 4 conditions), 4 hidden layers of 256 (SiLU), one log
- * density per cell, softmax over the cells. BF16 MFMA kernels with FP32
+ * equal-volume radius mm, aspect ratio): 47 inputs (the row coordinate t, 16
+ * Fourier features of the azimuth, even about v = 1/4, 13 Fourier pairs of t,
+ * 4 conditions), 4 hidden layers of 256 (SiLU), one log density per cell,
+ * softmax over the cells. Water-drop records are mirror symmetric in the
+ * azimuth (cell j and (phi_count/2 - 1 - j) mod phi_count hold the same
+ * mass), and so is the network: phi_count must be even. BF16 MFMA kernels with FP32
  * accumulation for training (forward, backward, Adam) and serving. Training
  * records live in the surrogate's own device memory, filled from traced
  * records; the tracer stays the judge (iph_sur_compare). */
@@ -236,7 +239,7 @@ typedef struct iph_sur_config {
 } iph_sur_config;
 int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out);
 void iph_sur_destroy(iph_sur* s);
-/* Packed FP32 parameters: W1 [256][37], b1 [256], (W [256][256], b [256]) x 3,
+/* Packed FP32 parameters: W1 [256][47], b1 [256], (W [256][256], b [256]) x 3,
  * w5 [256], b5 [1] (row-major [out][in]). */
 uint32_t iph_sur_param_count(void);
 int iph_sur_get_params(iph_sur* s, float* packed);

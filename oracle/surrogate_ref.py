@@ -5,7 +5,8 @@ bench's checks do).
 The reference has no learned part (README.md:31 of the reference says
 learning belongs to another project), so this restates the network the
 library defines (ice_crystal_phase_amd/csrc/surrogate.hip): inputs
-[t, sin(2 pi k v), cos(2 pi k v) (k=1..16), c0..c3], four SiLU layers of
+[t, cos(2 pi k (v - 1/4)) (k=1..16), sin(pi f t), cos(pi f t) (f in
+THETA_FREQ), c0..c3], four SiLU layers of
 256, one log density per cell, softmax over the cells of f + log du_i, and the
 cross-entropy loss -sum P log p. Pinned by central finite differences
 (tests/test_surrogate_oracle.py), which fix every term of the gradient.
@@ -16,7 +17,8 @@ import numpy as np
 
 H = 256
 NF = 16
-NIN = 1 + 2 * NF + 4
+THETA_FREQ = np.array([1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96], np.float64)
+NIN = 1 + NF + 2 * THETA_FREQ.size + 4
 
 
 def u_edges(nt: int) -> np.ndarray:
@@ -36,9 +38,10 @@ def features(nt: int, np_: int, cond) -> np.ndarray:
     v = (np.arange(np_) + 0.5) / np_
     T, V = np.meshgrid(t, v, indexing="ij")
     k = np.arange(1, NF + 1)
-    ang = 2 * np.pi * V.reshape(-1, 1) * k
+    ang = 2 * np.pi * (V.reshape(-1, 1) - 0.25) * k
+    tang = np.pi * T.reshape(-1, 1) * THETA_FREQ
     c = np.broadcast_to(normalize_cond(cond), (nt * np_, 4))
-    return np.concatenate([T.reshape(-1, 1), np.sin(ang), np.cos(ang), c], axis=1)
+    return np.concatenate([T.reshape(-1, 1), np.cos(ang), np.sin(tang), np.cos(tang), c], axis=1)
 
 
 def log_du(nt: int, np_: int) -> np.ndarray:

@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 def _segments():
     """(name, start, stop) of the packed parameter tensors."""
     out, o = [], 0
-    for l, fan in enumerate((37, 256, 256, 256)):
+    for l, fan in enumerate((ref.NIN, 256, 256, 256)):
         out.append((f"W{l + 1}", o, o + 256 * fan))
         o += 256 * fan
         out.append((f"b{l + 1}", o, o + 256))
@@ -168,3 +168,25 @@ def test_serve_traces_outside_the_gate(ctx):
         for i in range(2):
             phi, theta, _ = b.cdf(i)
             assert np.array_equal(out.phi[2 + i], phi) and np.array_equal(out.theta[2 + i], theta)
+
+
+def test_mirror_symmetry_of_records_and_network(ctx):
+    """Drop records are mirror symmetric in phi (v -> 1/2 - v: an axisymmetric
+    drop lit in a vertical plane), bit for bit on the tracer's fixed-point
+    histogram; the network is symmetric by construction (its phi features are
+    even about v = 1/4), which is what lets it run on the folded grid."""
+    print("case: mirror symmetry 45x90 and 30x64", flush=True)
+    for nt, np_ in ((45, 90), (30, 64)):
+        mirror = (np_ // 2 - 1 - np.arange(np_)) % np_
+        recs = S.drop_records(np.array([[35.0, 480.0, 2.4, 0.72], [-70.0, 640.0, 0.8, 0.96]], np.float32), 256, nt, np_)
+        with ip.Batch(ctx, recs) as b:
+            b.run(ip.RUN_TRACE)
+            for i in range(2):
+                h = np.asarray(b.bins(i)).reshape(nt, np_)
+                assert np.array_equal(h, h[:, mirror])
+        with S.Surrogate(ctx, nt, np_, max_records=2, seed=4) as sur:
+            q = sur.predict(np.array([[35.0, 480.0, 2.4, 0.72], [-70.0, 640.0, 0.8, 0.96]], np.float32))
+        for r in range(2):
+            p = q[r].reshape(nt, np_)
+            assert np.array_equal(p, p[:, mirror])
+            assert abs(p.sum() - 1) < 1e-5

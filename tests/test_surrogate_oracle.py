@@ -24,14 +24,24 @@ def _params(seed: int, scale: float = 1.0) -> np.ndarray:
 
 
 def test_param_count_matches_library():
-    assert ip.lib().iph_sur_param_count() == ref.param_count() == 207361 - 256 * (64 - 37) + 256 * (64 - 37)
+    assert ip.lib().iph_sur_param_count() == ref.param_count() == 256 * 47 + 256 + 3 * 65792 + 257
 
 
 def test_features_and_weights():
     X = ref.features(6, 8, (30.0, 550.0, 1.5, 1.0))
-    assert X.shape == (48, 37)
-    assert np.allclose(X[:, 0].reshape(6, 8)[:, 0], (np.arange(6) + 0.5) / 6)
-    assert np.allclose(X[:, 33:], [30 / 90, 0.0, 0.0, 0.0])
+    assert X.shape == (48, 47)
+    t = (np.arange(6) + 0.5) / 6
+    assert np.allclose(X[:, 0].reshape(6, 8)[:, 0], t)
+    # phi features depend on the column only, theta features on the row only
+    v = (np.arange(8) + 0.5) / 8
+    assert np.allclose(X[:8, 1 + 2], np.cos(2 * np.pi * 3 * (v - 0.25)))
+    assert np.allclose(X[::8, 17 + 4], np.sin(np.pi * 6 * t)) and np.allclose(X[::8, 30 + 12], np.cos(np.pi * 96 * t))
+    assert np.allclose(X[:, 43:], [30 / 90, 0.0, 0.0, 0.0])
+    # the phi features are even about v = 1/4: columns j and np/2 - 1 - j (mod np) match
+    Xr = X.reshape(6, 8, 47)
+    for j in range(8):
+        assert np.array_equal(Xr[:, j, 1:17], Xr[:, (8 // 2 - 1 - j) % 8, 1:17]) or np.allclose(
+            Xr[:, j, 1:17], Xr[:, (8 // 2 - 1 - j) % 8, 1:17], rtol=0, atol=1e-12)
     lw = ref.log_du(6, 8)
     assert np.isclose(np.exp(lw).reshape(6, 8)[:, 0].sum(), 1.0)
 
@@ -45,10 +55,12 @@ def test_gradient_matches_finite_differences():
     p = _params(1)
     loss, g = ref.loss_and_grad(p, nt, np_, cond, target)
     assert np.isfinite(loss) and g.shape == p.shape
-    idx = np.concatenate([rng.choice(256 * 37, 12, replace=False),                  # W1
-                          256 * 37 + rng.choice(256, 4, replace=False),             # b1
-                          256 * 38 + rng.choice(65536 + 256, 12, replace=False),    # layer 2
-                          256 * 38 + 2 * 65792 + rng.choice(65792, 8, replace=False),  # layer 4
+    n1 = 256 * ref.NIN
+    idx = np.concatenate([rng.choice(n1, 12, replace=False),                           # W1
+                          np.arange(43, n1, ref.NIN)[rng.choice(256, 4, replace=False)],  # W1, condition column
+                          n1 + rng.choice(256, 4, replace=False),                      # b1
+                          n1 + 256 + rng.choice(65536 + 256, 12, replace=False),       # layer 2
+                          n1 + 256 + 2 * 65792 + rng.choice(65792, 8, replace=False),  # layer 4
                           ref.param_count() - 257 + rng.choice(257, 6, replace=False)])  # w5, b5
     h = 1e-6
     for i in idx:
