@@ -307,17 +307,24 @@ def run_uneven(ctx, args):
     from ice_crystal_phase_amd import autotune
     recs = autotune.uneven_records(512, seed=4)
     table = []
-    choice = autotune.tune(ctx, recs, shrink=16, log=lambda m: table.append(m))
+
+    def log(m):
+        table.append(m)
+        print("autotune: " + m, file=sys.stderr, flush=True)
+
+    choice = autotune.tune(ctx, recs, shrink=64, log=log)
     with ip.Batch(ctx, recs) as b:
         b.set_schedule(choice.tile_rays, choice.blocks)
         flags = ip.RUN_TRACE | ip.RUN_CDF | ip.RUN_GRAPH
         for _ in range(args.warmup):
             b.run(flags)
+            print(f"uneven warmup step {b.timing()[2]:.0f} ms", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         ms = []
         for _ in range(args.steps):
             b.run(flags)
             ms.append(b.timing())
+            print(f"uneven step {ms[-1][2]:.0f} ms", file=sys.stderr, flush=True)
         wall = (time.perf_counter() - t0) / args.steps
         rays = sum(ip.record_rays(r) for r in recs)
     step = float(np.mean([m[2] for m in ms]))
@@ -371,6 +378,9 @@ def main() -> int:
     import ice_crystal_phase_amd as ip
 
     if args.config == "uneven":
+        # one timed pass of the batch (~9 G rays, a third of them FP64 hex rays) after one warmup pass
+        if args.steps == ap.get_default("steps"):
+            args.steps = 1
         with ip.Context(0) as ctx:
             out = run_uneven(ctx, args)
         print(json.dumps(out))

@@ -384,7 +384,8 @@ __device__ inline float silu_grad(float z)
 __global__ void __launch_bounds__(256, 2) sur_backward(Net net, const float* __restrict__ dF,
                                                    const __bf16* __restrict__ Z, uint32_t Mpad,
                                                    __bf16* __restrict__ dZ, float* __restrict__ part_w5,
-                                                   float* __restrict__ part_b5, float* __restrict__ part_db)
+                                                   float* __restrict__ part_b5, float* __restrict__ part_db,
+                                                   int accumulate)
 {
     // part_db: [4 layers][tiles][256] column sums of the fp32 dZ (bias gradients)
     const uint32_t tiles = gridDim.x;
@@ -435,8 +436,8 @@ __global__ void __launch_bounds__(256, 2) sur_backward(Net net, const float* __r
             if (c16 == 15) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    gw_out[16 * cb + i] = gw[i];
-                    gz_out[16 * cb + i] = gz[i];
+                    gw_out[16 * cb + i] = accumulate ? gw_out[16 * cb + i] + gw[i] : gw[i];
+                    gz_out[16 * cb + i] = accumulate ? gz_out[16 * cb + i] + gz[i] : gz[i];
                 }
             }
         }
@@ -445,7 +446,7 @@ __global__ void __launch_bounds__(256, 2) sur_backward(Net net, const float* __r
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) gb += d[rb];
             gb = row16_sum(gb);
-            if (threadIdx.x == 15) part_b5[blockIdx.x] = gb;
+            if (threadIdx.x == 15) part_b5[blockIdx.x] = accumulate ? part_b5[blockIdx.x] + gb : gb;
         }
     }
     __syncthreads();
@@ -489,10 +490,10 @@ __global__ void __launch_bounds__(256, 2) sur_backward(Net net, const float* __r
             cs2 = row16_sum(cs2);
             cs3 = row16_sum(cs3);
             if (c16 == 15) {
-                db[16 * cb + 0] = cs0;
-                db[16 * cb + 1] = cs1;
-                db[16 * cb + 2] = cs2;
-                db[16 * cb + 3] = cs3;
+                db[16 * cb + 0] = accumulate ? db[16 * cb + 0] + cs0 : cs0;
+                db[16 * cb + 1] = accumulate ? db[16 * cb + 1] + cs1 : cs1;
+                db[16 * cb + 2] = accumulate ? db[16 * cb + 2] + cs2 : cs2;
+                db[16 * cb + 3] = accumulate ? db[16 * cb + 3] + cs3 : cs3;
             }
         }
         __syncthreads();
@@ -523,7 +524,7 @@ template <int IN>
 __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ dZ, const __bf16* __restrict__ Zprev,
                                                     const __bf16* __restrict__ Xhi, const __bf16* __restrict__ Xlo,
                                                     const float* __restrict__ cond, uint32_t rows_per_split,
-                                                    uint32_t Mpad, float* __restrict__ part)
+                                                    uint32_t Mpad, float* __restrict__ part, int accumulate)
 {
     constexpr int T = IN * 4;             // threads
     constexpr int S = 40;                 // LDS stride (bf16) of a 32-row column
@@ -635,7 +636,10 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
 #pragma unroll
         for (int ib = 0; ib < 4; ++ib)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) P[(size_t)(o0 + 16 * ob + 4 * q + i) * IN + i0 + 16 * ib + c16] = acc[ob][ib][i];
+            for (int i = 0; i < 4; ++i) {
+                float* d = P + (size_t)(o0 + 16 * ob + 4 * q + i) * IN + i0 + 16 * ib + c16;
+                *d = accumulate ? *d + acc[ob][ib][i] : acc[ob][ib][i];
+            }
 }
 
 // Two-pass fixed-order reduction over splits: out[e] (+)= sum_s part[s * n + e].
@@ -862,22 +866,23 @@ hipError_t fold(const Static& st, const float* full, float* folded, hipStream_t 
 }
 
 hipError_t backward(const Net& net, const Static& st, const float* dF, const __bf16* Z, __bf16* dZ, float* part_w5,
-                    float* part_b5, float* part_db, hipStream_t s)
+                    float* part_b5, float* part_db, bool accumulate, hipStream_t s)
 {
     hipLaunchKernelGGL(sur_backward, dim3(st.Mpad / TM), dim3(256), 0, s, net, dF, Z, st.Mpad, dZ, part_w5, part_b5,
-                       part_db);
+                       part_db, accumulate ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t wgrad(const Static& st, int layer, const __bf16* dZl, const __bf16* Zprev, const float* d_cond,
-                 float* part, hipStream_t s)
+                 float* part, bool accumulate, hipStream_t s)
 {
+    const int a = accumulate ? 1 : 0;
     if (layer == 0)
         hipLaunchKernelGGL(sur_wgrad<64>, dim3(st.splits), dim3(256), 0, s, dZl, nullptr, st.xhi, st.xlo, d_cond,
-                           st.rows_per_split, st.Mpad, part);
+                           st.rows_per_split, st.Mpad, part, a);
     else
         hipLaunchKernelGGL(sur_wgrad<256>, dim3(st.splits), dim3(1024), 0, s, dZl, Zprev, nullptr, nullptr, nullptr,
-                           st.rows_per_split, st.Mpad, part);
+                           st.rows_per_split, st.Mpad, part, a);
     return hipGetLastError();
 }
 

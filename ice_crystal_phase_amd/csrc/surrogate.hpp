@@ -87,11 +87,14 @@ hipError_t probs(const Static& st, const float* logits, const float* lse, uint32
                  hipStream_t s);
 // folded training target [Mh] from full-grid probabilities [M]
 hipError_t fold(const Static& st, const float* full, float* folded, hipStream_t s);
-// part_db: [4][Mpad/kTileRows][256] fp32 column sums of dZ (bias gradients)
+// part_db: [4][Mpad/kTileRows][256] fp32 column sums of dZ (bias gradients).
+// accumulate: add into the partial buffers (records of one step, in order)
+// instead of overwriting them.
 hipError_t backward(const Net& net, const Static& st, const float* dF, const __bf16* Z, __bf16* dZ, float* part_w5,
-                    float* part_b5, float* part_db, hipStream_t s);
+                    float* part_b5, float* part_db, bool accumulate, hipStream_t s);
+// part: [splits][256][in] weight-gradient partials of one layer
 hipError_t wgrad(const Static& st, int layer, const __bf16* dZl, const __bf16* Zprev, const float* d_cond, float* part,
-                 hipStream_t s);
+                 bool accumulate, hipStream_t s);
 // part2: kReduceGroups * n floats of scratch
 hipError_t reduce(const float* part, uint32_t splits, uint32_t n, float* out, bool accumulate, float* part2,
                   hipStream_t s);

@@ -87,6 +87,12 @@ struct iph_sur {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ms[3] = {0, 0, 0};
 
+    // weight-gradient split partials of layer l: [splits][256][in]
+    float* part_layer(int l) const
+    {
+        const size_t per = (size_t)st.splits * 256;
+        return part.p + (l == 0 ? 0 : per * (64 + (size_t)(l - 1) * 256));
+    }
     Net net() const
     {
         Net n;
@@ -241,7 +247,7 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     IPH_HIP(ctx, s->loss_part.alloc((st.Mpad + 255) / 256));
     IPH_HIP(ctx, s->loss.alloc(std::max<uint32_t>(cfg->max_records, 1024)));
     IPH_HIP(ctx, s->conds.alloc((size_t)4 * std::max<uint32_t>(cfg->max_records, 1024)));
-    IPH_HIP(ctx, s->part.alloc((size_t)st.splits * 256 * 256));
+    IPH_HIP(ctx, s->part.alloc((size_t)st.splits * 256 * (64 + 3 * 256)));
     IPH_HIP(ctx, s->part2.alloc((size_t)kReduceGroups * 256 * 256));
     IPH_HIP(ctx, s->part_w5.alloc((size_t)(st.Mpad / kTileRows) * 256));
     IPH_HIP(ctx, s->part_b5.alloc(st.Mpad / kTileRows));
@@ -324,9 +330,10 @@ int iph_sur_set_params(iph_sur* s, const float* packed)
 
 }  // extern "C"
 
-// Forward + backward of training slot `slot`, gradient added into s->grad;
-// the loss goes to s->loss[k].
-static int accumulate_gradient(iph_sur* s, uint32_t slot, uint32_t k)
+// Forward + backward of training slot `slot`: the weight- and bias-gradient
+// partials of its rows go into (accumulate: are added to) the per-layer split
+// buffers; the loss goes to s->loss[k].
+static int record_gradient(iph_sur* s, uint32_t slot, uint32_t k, bool accumulate)
 {
     iph_ctx* ctx = s->ctx;
     hipStream_t q = ctx->stream;
@@ -334,21 +341,32 @@ static int accumulate_gradient(iph_sur* s, uint32_t slot, uint32_t k)
     const Net n = s->net();
     const float* cond = s->data_conds.p + 4 * (size_t)slot;
     const float* d_target = s->data.p + (size_t)slot * st.Mh;
-    const uint32_t loss_slot = k;
     IPH_HIP(ctx, forward(n, st, cond, 1, true, s->logits.p, s->Z.p, q));
     IPH_HIP(ctx, lse(st, s->logits.p, 1, s->lse_part.p, s->lse.p, q));
-    IPH_HIP(ctx, dlogits(st, s->logits.p, s->lse.p, d_target, s->dF.p, s->loss_part.p, s->loss.p + loss_slot, q));
-    IPH_HIP(ctx, backward(n, st, s->dF.p, s->Z.p, s->dZ.p, s->part_w5.p, s->part_b5.p, s->part_db.p, q));
+    IPH_HIP(ctx, dlogits(st, s->logits.p, s->lse.p, d_target, s->dF.p, s->loss_part.p, s->loss.p + k, q));
+    IPH_HIP(ctx, backward(n, st, s->dF.p, s->Z.p, s->dZ.p, s->part_w5.p, s->part_b5.p, s->part_db.p, accumulate, q));
     const size_t zl = (size_t)st.Mpad * 256;
+    for (int l = 3; l >= 0; --l)
+        IPH_HIP(ctx, wgrad(st, l, s->dZ.p + l * zl, l ? s->Z.p + (l - 1) * zl : nullptr, cond, s->part_layer(l),
+                           accumulate, q));
+    return IPH_OK;
+}
+
+// Fixed-order reductions of the split partials into s->grad (added).
+static int reduce_gradient(iph_sur* s)
+{
+    iph_ctx* ctx = s->ctx;
+    hipStream_t q = ctx->stream;
+    const Static& st = s->st;
+    const uint32_t tiles = st.Mpad / kTileRows;
     for (int l = 3; l >= 0; --l) {
         const uint32_t in = l == 0 ? 64 : 256;
-        IPH_HIP(ctx, wgrad(st, l, s->dZ.p + l * zl, l ? s->Z.p + (l - 1) * zl : nullptr, cond, s->part.p, q));
-        IPH_HIP(ctx, reduce(s->part.p, st.splits, 256 * in, s->grad.p + off_w(l), true, s->part2.p, q));
-        IPH_HIP(ctx, reduce(s->part_db.p + (size_t)l * (st.Mpad / kTileRows) * 256, st.Mpad / kTileRows, 256,
-                            s->grad.p + off_b(l), true, s->part2.p, q));
+        IPH_HIP(ctx, reduce(s->part_layer(l), st.splits, 256 * in, s->grad.p + off_w(l), true, s->part2.p, q));
+        IPH_HIP(ctx, reduce(s->part_db.p + (size_t)l * tiles * 256, tiles, 256, s->grad.p + off_b(l), true, s->part2.p,
+                            q));
     }
-    IPH_HIP(ctx, reduce(s->part_w5.p, st.Mpad / kTileRows, 256, s->grad.p + kOffW5, true, s->part2.p, q));
-    IPH_HIP(ctx, reduce(s->part_b5.p, st.Mpad / kTileRows, 1, s->grad.p + kOffB5, true, s->part2.p, q));
+    IPH_HIP(ctx, reduce(s->part_w5.p, tiles, 256, s->grad.p + kOffW5, true, s->part2.p, q));
+    IPH_HIP(ctx, reduce(s->part_b5.p, tiles, 1, s->grad.p + kOffB5, true, s->part2.p, q));
     return IPH_OK;
 }
 
@@ -397,7 +415,8 @@ int iph_sur_gradient(iph_sur* s, uint32_t slot, float* loss, float* packed_grad)
     iph_ctx* ctx = s->ctx;
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     IPH_HIP(ctx, hipMemsetAsync(s->grad.p, 0, kParamsPad * 4, ctx->stream));
-    const int rc = accumulate_gradient(s, slot, 0);
+    int rc = record_gradient(s, slot, 0, false);
+    if (rc == IPH_OK) rc = reduce_gradient(s);
     if (rc != IPH_OK) return rc;
     std::vector<float> g(kParamsPad);
     float l = 0;
@@ -421,9 +440,11 @@ int iph_sur_train_step(iph_sur* s, const uint32_t* slots, uint32_t n, float lr, 
     hipStream_t q = ctx->stream;
     IPH_HIP(ctx, hipEventRecord(s->ev[0], q));
     for (uint32_t k = 0; k < n; ++k) {
-        const int rc = accumulate_gradient(s, slots[k], k);
+        const int rc = record_gradient(s, slots[k], k, k > 0);
         if (rc != IPH_OK) return rc;
     }
+    const int rc = reduce_gradient(s);
+    if (rc != IPH_OK) return rc;
     ++s->step;
     const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)s->step)));
     const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)s->step)));
