@@ -3,8 +3,8 @@
 // accumulation, trained and served by the kernels in this file.
 //
 // Network (per cell of an nt x np grid, conditioned on one record):
-//   x = [t, sin(2 pi k v), cos(2 pi k v) (k = 1..16),
-//        sin(pi f t), cos(pi f t) (f in kThetaFreq, 13), c0..c3]   (63 inputs)
+//   x = [t, cos(2 pi k (v - 1/4)) (k = 1..16),
+//        sin(pi f t), cos(pi f t) (f in kThetaFreq, 13), c0..c3]   (47 inputs)
 //       t = (i + 1/2) / nt of the cell's theta row (the grid's uniform
 //       coordinate of u), v = cell-centre (phi + pi) / 2 pi, c = normalized
 //       (alpha, lambda, radius, aspect);
@@ -15,13 +15,15 @@
 // row, the v features on the phi column, c on the record) and runs in fp32 on
 // the VALU: z1 = T[i] + (b1 + W1_c c) + P[j], with the theta table T = W1_t
 // x_t(t_i) and the phi table P = W1_v x_v(v_j) rebuilt whenever the weights
-// change, so neighbouring theta rows stay distinct. Hidden activations are bf16, every
-// MFMA product accumulates in fp32. The weight gradient of layer 1 splits x
-// into bf16 hi + lo parts (two MFMA passes).
+// change, so neighbouring theta rows stay distinct. Hidden activations are
+// bf16, every MFMA product accumulates in fp32. The weight gradient of layer 1
+// splits x into bf16 hi + lo parts (two MFMA passes). The phi features are even
+// about v = 1/4, so the network is mirror symmetric like the drop records, and
+// every kernel runs over the folded grid (surrogate.hpp, Static).
 //
 // Training loss per record: cross entropy -sum P log p against the traced
 // record's cell probabilities P. Kernels:
-//   sur_forward     fused 4-layer forward per 64-row tile (activations in LDS),
+//   sur_forward     fused 4-layer forward per 128-row tile (activations in LDS),
 //                   logits, and the pre-activations Z_l when training;
 //   sur_lse_*       log-sum-exp per record (fixed-order two-level reduction);
 //   sur_dlogits     dL/df = p - P and the loss (over the folded rows);
@@ -30,7 +32,7 @@
 //   sur_wgrad       weight gradients dW_l = dZ_l^T H_{l-1}, split over rows,
 //                   partial tiles reduced in fixed order (deterministic);
 //   sur_adam/pack   Adam on the fp32 master weights, bf16 copies for MFMA,
-//                   and the layer-1 phi table.
+//                   and the layer-1 theta and phi tables.
 #include "surrogate.hpp"
 
 #include <hip/hip_runtime.h>
