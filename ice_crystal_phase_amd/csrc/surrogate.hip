@@ -375,10 +375,11 @@ __global__ void __launch_bounds__(256) sur_fold(Static st, const float* __restri
     }
 }
 
+// silu'(z) = s (1 + z (1 - s)), s = sigmoid(z), as two FMAs
 __device__ inline float silu_grad(float z)
 {
     const float s = sigmoid(z);
-    return s * (1.0f + z * (1.0f - s));
+    return fmaf(z, fmaf(-s, s, s), s);
 }
 
 // Data gradients of one TM-row tile, layers 4 -> 1: dZ_l to HBM; output
@@ -422,8 +423,8 @@ __global__ void __launch_bounds__(256, 2) sur_backward(Net net, const float* __r
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float sg = sigmoid(zv[i]);
-                    gw[i] += d[rb] * (zv[i] * sg);
-                    dz[i] = d[rb] * wv[i] * (sg * (1.0f + zv[i] * (1.0f - sg)));
+                    gw[i] = fmaf(d[rb], zv[i] * sg, gw[i]);
+                    dz[i] = (d[rb] * wv[i]) * fmaf(zv[i], fmaf(-sg, sg, sg), sg);
                     gz[i] += dz[i];
                 }
                 const uint2 g = pack4(dz[0], dz[1], dz[2], dz[3]);
@@ -518,10 +519,11 @@ __device__ inline void store_cols(__bf16* T, int stride, const uint2 v[4])
 // Weight gradients of one layer: dW[o][i] = sum_rows dZ[row][o] H[row][i] over
 // the rows of split s, for all 256 outputs and IN inputs. H = silu(Zprev) for
 // hidden layers; for the input layer H = x_hi + x_lo (two passes). Waves:
-// (IN/64) x 4, each 64 outputs x 64 inputs. Rows go in chunks of 32: every
-// thread stages 4 x 4 blocks of dZ and H into column-major LDS tiles (the MFMA
-// reads 8 consecutive rows of one column), loading the next chunk's blocks
-// into registers while the MFMAs consume the current one.
+// (IN/64) x 4, each 64 outputs x 64 inputs. Rows go in chunks of 32 through
+// two column-major LDS buffers (the MFMA reads 8 consecutive rows of one
+// column): while the MFMAs consume one chunk, every thread loads its 4 x 4
+// blocks of the next chunk's dZ and H and stages them (silu included) into the
+// other buffer, so the staging work fills the MFMA pipeline's shadow.
 template <int IN>
 __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ dZ, const __bf16* __restrict__ Zprev,
                                                     const __bf16* __restrict__ Xhi, const __bf16* __restrict__ Xlo,
@@ -534,8 +536,8 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
     constexpr int NT = 512 + 8 * (IN / 4) * NH;  // 4 x 4 blocks per chunk: dZ, then H
     constexpr int PER = NT / T;                  // blocks per thread
     static_assert(NT % T == 0, "whole blocks per thread");
-    __shared__ __align__(16) __bf16 dZT[256 * S];
-    __shared__ __align__(16) __bf16 HT[NH][IN * S];
+    __shared__ __align__(16) __bf16 dZT[2][256 * S];
+    __shared__ __align__(16) __bf16 HT[2][NH][IN * S];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c16 = lane & 15;
     const int o0 = 64 * (wave & 3), i0 = 64 * (wave >> 2);
     const uint32_t r_begin = blockIdx.x * rows_per_split;
@@ -547,8 +549,8 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
         for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0, 0, 0, 0};
     // block p of this thread: task = threadIdx.x + p T; rows 4 (task & 7) ..,
     // columns 4 (task >> 3) .. of dZ (task < 512) or of H
-    uint2 v[PER][4];
-    auto load = [&](uint32_t r0) {
+    uint2 v[PER][4], vn[PER][4];
+    auto load = [&](uint2 (&v)[PER][4], uint32_t r0) {
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
             const int task = threadIdx.x + p * T, rg = task & 7;
@@ -576,20 +578,20 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
             cl[k] = cond[k] - ch[k];
         }
     }
-    load(r_begin);
-    for (uint32_t r0 = r_begin; r0 < r_end; r0 += 32) {
+    auto stage = [&](int buf) {
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
             const int task = threadIdx.x + p * T, rg = task & 7;
             if (task < 512) {
-                store_cols(dZT + 4 * (task >> 3) * S + 4 * rg, S, v[p]);
+                store_cols(dZT[buf] + 4 * (task >> 3) * S + 4 * rg, S, v[p]);
             } else if (IN == 64) {
                 const int c = 4 * ((task - 512) >> 3), table = c >> 6, cc = c & 63;
+                uint2 w[4] = {v[p][0], v[p][1], v[p][2], v[p][3]};
                 if (cc + 3 >= kColCond && cc < kColCond + 4) {
                     // substitute the record's condition for its four columns
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        float4 x = unpack4(v[p][k]);
+                        float4 x = unpack4(w[k]);
                         float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
@@ -597,10 +599,10 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
                             if (col >= kColCond && col < kColCond + 4)
                                 xs[j] = table ? cl[col - kColCond] : ch[col - kColCond];
                         }
-                        v[p][k] = pack4(xs[0], xs[1], xs[2], xs[3]);
+                        w[k] = pack4(xs[0], xs[1], xs[2], xs[3]);
                     }
                 }
-                store_cols(&HT[table][cc * S + 4 * rg], S, v[p]);
+                store_cols(&HT[buf][table][cc * S + 4 * rg], S, w);
             } else {
                 const int c = 4 * ((task - 512) >> 3);
                 float h[4][4];
@@ -614,23 +616,40 @@ __global__ void __launch_bounds__(IN * 4) sur_wgrad(const __bf16* __restrict__ d
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    *reinterpret_cast<uint2*>(&HT[0][(c + j) * S + 4 * rg]) = pack4(h[0][j], h[1][j], h[2][j], h[3][j]);
+                    *reinterpret_cast<uint2*>(&HT[buf][0][(c + j) * S + 4 * rg]) =
+                        pack4(h[0][j], h[1][j], h[2][j], h[3][j]);
             }
         }
-        __syncthreads();
-        if (r0 + 32 < r_end) load(r0 + 32);
+    };
+    // two chunks in flight: v holds chunk i + 1 while chunk i + 2 loads into vn
+    // (past the split's end the last chunk is loaded and staged again, unused)
+    auto clamp = [&](uint32_t r) { return r < r_end ? r : r_end - 32; };
+    load(v, r_begin);
+    stage(0);
+    load(v, clamp(r_begin + 32));
+    __syncthreads();
+    int cur = 0;
+    for (uint32_t r0 = r_begin; r0 < r_end; r0 += 32) {
+        load(vn, clamp(r0 + 64));
         bf16x8 a[4];
 #pragma unroll
-        for (int ob = 0; ob < 4; ++ob) a[ob] = *reinterpret_cast<const bf16x8*>(dZT + (o0 + 16 * ob + c16) * S + 8 * q);
+        for (int ob = 0; ob < 4; ++ob)
+            a[ob] = *reinterpret_cast<const bf16x8*>(dZT[cur] + (o0 + 16 * ob + c16) * S + 8 * q);
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
             for (int ib = 0; ib < 4; ++ib) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&HT[h][(i0 + 16 * ib + c16) * S + 8 * q]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&HT[cur][h][(i0 + 16 * ib + c16) * S + 8 * q]);
 #pragma unroll
                 for (int ob = 0; ob < 4; ++ob) acc[ob][ib] = mfma(a[ob], b, acc[ob][ib]);
             }
+        stage(cur ^ 1);
         __syncthreads();
+        cur ^= 1;
+#pragma unroll
+        for (int p = 0; p < PER; ++p)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[p][k] = vn[p][k];
     }
     float* P = part + (size_t)blockIdx.x * 256 * IN;
 #pragma unroll
