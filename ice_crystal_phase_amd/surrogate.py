@@ -97,8 +97,9 @@ class Surrogate:
         return out
 
     def set_params(self, p: np.ndarray) -> None:
-        p = np.ascontiguousarray(p, np.float32)
-        assert p.size == self.param_count()
+        p = np.ascontiguousarray(p, np.float32).reshape(-1)
+        if p.size != self.param_count():
+            raise ValueError(f"expected {self.param_count()} parameters (this network's layout), got {p.size}")
         icp._check(icp.lib().iph_sur_set_params(self._p, p.ctypes.data), self.ctx.ptr)
 
     def set_record(self, slot: int, batch: icp.Batch, rec: int, cond) -> None:
@@ -107,7 +108,8 @@ class Surrogate:
 
     def set_target(self, slot: int, probs: np.ndarray, cond) -> None:
         probs = np.ascontiguousarray(probs, np.float32).reshape(-1)
-        assert probs.size == self.M
+        if probs.size != self.M:
+            raise ValueError(f"expected {self.M} cell probabilities, got {probs.size}")
         c = (C.c_float * 4)(*[float(x) for x in cond])
         icp._check(icp.lib().iph_sur_set_target(self._p, slot, probs.ctypes.data, c), self.ctx.ptr)
 
