@@ -201,8 +201,10 @@ def run_surrogate(ctx, n_eval: int = 64, eval_grid: int = 16384, tracer_grid: in
            "train_step_ms_8_records": min(steps),
            "config": "water drops alpha U[-90,90], lambda U[400,700] nm, r U[0.25,3] mm (seed 6), 450x900 cells"}
     if gate is not None:
-        out["gate"] = {"accept_radius_bins": [[gate.radius_edges[k], gate.radius_edges[k + 1]]
-                                              for k in range(len(gate.accept)) if gate.accept[k]],
+        out["gate"] = {"accept_cells": [[[gate.radius_edges[i], gate.radius_edges[i + 1]],
+                                          [gate.alpha_edges[j], gate.alpha_edges[j + 1]]]
+                                         for i in range(len(gate.accept)) for j in range(len(gate.accept[i]))
+                                         if gate.accept[i][j]],
                        "tv_limit": gate.tv_limit, "quantile": gate.quantile}
         out["inside"] = int(inside.sum())
         if inside.any():

@@ -241,13 +241,15 @@ def train(sur: Surrogate, n_records: int, epochs: int, batch: int = 8, lr: float
 
 # ---------------------------------------------------------------- validated domain and fallback
 RADIUS_EDGES = tuple(0.25 + 0.25 * k for k in range(12))  # 0.25 .. 3.0 mm, 11 bins
+ALPHA_EDGES = (0.0, 30.0, 60.0, 90.0 + 1e-6)              # |inclination| deg, 3 bins
 
 
 @dataclass
 class Gate:
-    """Validated domain of a trained network: the radius bins whose validation
-    records reached total variation <= tv_limit at the given quantile. Records in
-    the other bins (or outside the edges) are traced."""
+    """Validated domain of a trained network: the (radius, |inclination|) cells
+    whose validation records reached total variation <= tv_limit at the given
+    quantile. Records in the other cells (or outside the edges) are traced.
+    accept, tv_quantile, tv_mean and counts are [radius bin][alpha bin]."""
     tv_limit: float
     quantile: float
     radius_edges: list
@@ -256,13 +258,28 @@ class Gate:
     tv_mean: list = field(default_factory=list)
     counts: list = field(default_factory=list)
     validation: dict = field(default_factory=dict)
+    alpha_edges: list = field(default_factory=lambda: [0.0, 90.0 + 1e-6])
+
+    def __post_init__(self):
+        # a gate over radius bins only: one alpha bin
+        if self.accept and not isinstance(self.accept[0], list):
+            self.accept = [[bool(a)] for a in self.accept]
+            self.tv_quantile = [[x] for x in self.tv_quantile]
+            self.tv_mean = [[x] for x in self.tv_mean]
+            self.counts = [[x] for x in self.counts]
+
+    def cells(self, conds: np.ndarray):
+        c = np.asarray(conds, np.float64).reshape(-1, 4)
+        kr = np.searchsorted(np.asarray(self.radius_edges), c[:, 2], side="right") - 1
+        ka = np.searchsorted(np.asarray(self.alpha_edges), np.abs(c[:, 0]), side="right") - 1
+        ok = (kr >= 0) & (kr < len(self.radius_edges) - 1) & (ka >= 0) & (ka < len(self.alpha_edges) - 1)
+        return kr, ka, ok
 
     def accepts(self, conds: np.ndarray) -> np.ndarray:
-        r = np.asarray(conds, np.float64).reshape(-1, 4)[:, 2]
-        k = np.searchsorted(np.asarray(self.radius_edges), r, side="right") - 1
-        ok = (k >= 0) & (k < len(self.accept))
-        out = np.zeros(r.shape, bool)
-        out[ok] = np.asarray(self.accept, bool)[k[ok]]
+        kr, ka, ok = self.cells(conds)
+        acc = np.asarray(self.accept, bool)
+        out = np.zeros(kr.shape, bool)
+        out[ok] = acc[kr[ok], ka[ok]]
         return out
 
     def save(self, path) -> None:
@@ -276,22 +293,32 @@ class Gate:
 
 
 def fit_gate(sur: Surrogate, conds: np.ndarray, grid: int, tv_limit: float = 1e-2, quantile: float = 0.9,
-             edges=RADIUS_EDGES) -> Gate:
+             radius_edges=RADIUS_EDGES, alpha_edges=ALPHA_EDGES, min_count: int = 8) -> Gate:
     """Measure the network against traced validation records (grid^2 rays) and
-    accept the radius bins whose TV quantile is within tv_limit."""
+    accept the (radius, |inclination|) cells holding at least min_count records
+    whose TV quantile is within tv_limit."""
     tv, dg, _ = evaluate(sur, conds, grid)
-    r = np.asarray(conds, np.float64)[:, 2]
-    k = np.searchsorted(np.asarray(edges), r, side="right") - 1
+    g = Gate(tv_limit, quantile, list(radius_edges), [[False] * (len(alpha_edges) - 1)] * (len(radius_edges) - 1),
+             alpha_edges=list(alpha_edges))
+    kr, ka, ok = g.cells(conds)
     accept, tq, tm, n = [], [], [], []
-    for b in range(len(edges) - 1):
-        t = tv[k == b]
-        n.append(int(t.size))
-        tq.append(float(np.quantile(t, quantile)) if t.size else float("nan"))
-        tm.append(float(t.mean()) if t.size else float("nan"))
-        accept.append(bool(t.size >= 4 and tq[-1] <= tv_limit))
-    return Gate(tv_limit, quantile, list(edges), accept, tq, tm, n,
-                {"records": int(len(conds)), "grid": int(grid), "tv_mean": float(tv.mean()),
-                 "dg_mean": float(dg.mean())})
+    for i in range(len(radius_edges) - 1):
+        row = [[], [], [], []]
+        for j in range(len(alpha_edges) - 1):
+            t = tv[ok & (kr == i) & (ka == j)]
+            q = float(np.quantile(t, quantile)) if t.size else float("nan")
+            row[0].append(bool(t.size >= min_count and q <= tv_limit))
+            row[1].append(q)
+            row[2].append(float(t.mean()) if t.size else float("nan"))
+            row[3].append(int(t.size))
+        accept.append(row[0]), tq.append(row[1]), tm.append(row[2]), n.append(row[3])
+    c = np.asarray(conds, np.float64)
+    g.accept, g.tv_quantile, g.tv_mean, g.counts = accept, tq, tm, n
+    g.validation = {"records": int(len(conds)), "grid": int(grid), "tv_mean": float(tv.mean()),
+                    "dg_mean": float(dg.mean()),
+                    "per_record": [[round(float(a), 3), round(float(l), 2), round(float(r), 4), round(float(t), 6)]
+                                   for (a, l, r, _), t in zip(c, tv)]}
+    return g
 
 
 @dataclass

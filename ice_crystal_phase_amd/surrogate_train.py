@@ -45,6 +45,8 @@ def main(argv=None) -> int:
                     help="validation records (seed + 2) for the served domain; 0 = no gate")
     ap.add_argument("--gate-out", default=None, help="write the gate (.json)")
     ap.add_argument("--tv-limit", type=float, default=1e-2)
+    ap.add_argument("--gate-alpha", action="store_true",
+                    help="gate over radius x |inclination| cells (default: radius bins only)")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -75,8 +77,9 @@ def main(argv=None) -> int:
                 np.save(a.out, sur.params())
         gate = None
         if a.gate:
-            gate = S.fit_gate(sur, S.drop_conditions(a.gate, seed=a.seed + 2), a.eval_grid, tv_limit=a.tv_limit)
-            log(f"gate: accept {gate.accept} tv_q90 {[round(x, 4) for x in gate.tv_quantile]}")
+            gate = S.fit_gate(sur, S.drop_conditions(a.gate, seed=a.seed + 2), a.eval_grid, tv_limit=a.tv_limit,
+                              alpha_edges=S.ALPHA_EDGES if a.gate_alpha else (0.0, 90.0 + 1e-6))
+            log(f"gate: accept {gate.accept} tv_q90 {[[round(x, 4) for x in r] for r in gate.tv_quantile]}")
             if a.gate_out:
                 gate.save(a.gate_out)
         tv, dg, eval_ms = S.evaluate(sur, fresh, a.eval_grid)

@@ -8,6 +8,7 @@ from ice_crystal_phase_amd import surrogate as S
 
 
 def _gate(accept):
+    """A gate over radius bins only (the flat form: one alpha bin)."""
     return S.Gate(1e-2, 0.9, list(S.RADIUS_EDGES), accept, [0.0] * 11, [0.0] * 11, [10] * 11)
 
 
@@ -25,6 +26,15 @@ def test_round_trip(tmp_path):
     p = tmp_path / "gate.json"
     g.save(p)
     h = S.Gate.load(p)
-    assert h == g
+    assert h.accept == g.accept and h.alpha_edges == g.alpha_edges and h.validation == g.validation
     conds = S.drop_conditions(32, seed=3)
     assert np.array_equal(h.accepts(conds), conds[:, 2] >= 1.0)
+
+
+def test_radius_and_inclination_cells():
+    acc = [[k < 3, k < 6, False] for k in range(11)]   # small drops: all |alpha| < 60
+    g = S.Gate(1e-2, 0.9, list(S.RADIUS_EDGES), acc, alpha_edges=list(S.ALPHA_EDGES))
+    conds = np.array([[10.0, 550, 0.3, 1], [-45.0, 550, 0.3, 1], [-75.0, 550, 0.3, 1], [29.9, 550, 1.1, 1],
+                      [-30.0, 550, 1.1, 1], [59.0, 550, 1.6, 1], [90.0, 550, 0.3, 1]])
+    # radius bins: 0.3 -> 0, 1.1 -> 3, 1.6 -> 5; |alpha| bins: [0, 30), [30, 60), [60, 90]
+    assert g.accepts(conds).tolist() == [True, True, False, False, True, True, False]
