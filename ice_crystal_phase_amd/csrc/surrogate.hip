@@ -47,6 +47,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // Rows of the activation tile of one workgroup (4 waves x 64 output columns).
 constexpr int TM = kTileRows;
 constexpr int RB = TM / 16;
+// forward tile rows and resident forward workgroups per CU (its LDS tile is kFwdRows x 512 bytes;
+// 96-row tiles at 3 workgroups per CU measured 9% slower than 128 at 2)
+constexpr int kFwdRows = 128;
+constexpr int kFwdBlocks = 2;
+static_assert(128 % kFwdRows == 0 && 128 % TM == 0, "Mpad, a multiple of 128, holds whole tiles");
 
 // LDS index of element (row, col) of a TM x 256 bf16 tile: 8-element chunks
 // are XOR-swizzled by row so the 16 rows an MFMA fragment reads fall in
@@ -104,7 +109,8 @@ __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 //   acc[rb][cb][i] = Z[16 rb + (lane & 15)][n0 + 16 cb + 4 (lane >> 4) + i].
 // W is a global [256][256] bf16 matrix, streamed in 32-wide k slices with the
 // next slice loaded while the current one feeds the MFMAs.
-__device__ inline void slab_mma(const __bf16* Hs, const __bf16* __restrict__ W, int n0, f32x4 acc[RB][4])
+template <int R = RB>
+__device__ inline void slab_mma(const __bf16* Hs, const __bf16* __restrict__ W, int n0, f32x4 acc[R][4])
 {
     const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
     const __bf16* wrow = W + (size_t)(n0 + r) * 256 + 8 * q;
@@ -119,7 +125,7 @@ __device__ inline void slab_mma(const __bf16* Hs, const __bf16* __restrict__ W, 
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) wn[cb] = *reinterpret_cast<const bf16x8*>(wrow + 16 * 256 * cb + kn);
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
+        for (int rb = 0; rb < R; ++rb) {
             const bf16x8 h = *reinterpret_cast<const bf16x8*>(Hs + sw(16 * rb + r, k0 + 8 * q));
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = mfma(w[cb], h, acc[rb][cb]);
@@ -129,27 +135,29 @@ __device__ inline void slab_mma(const __bf16* Hs, const __bf16* __restrict__ W, 
     }
 }
 
-__device__ inline void zero(f32x4 acc[RB][4])
+template <int R = RB>
+__device__ inline void zero(f32x4 acc[R][4])
 {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int rb = 0; rb < R; ++rb)
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f32x4{0, 0, 0, 0};
 }
 
-// Forward of one TM-row tile of one record; grid = (Mpad / TM, records).
+// Forward of one TMF-row tile of one record; grid = (Mpad / TMF, records).
 // Layer 1 in fp32 (separable input), layers 2-4 bf16 MFMA with the
 // activations in one LDS tile; the output layer is folded into the layer-4
 // epilogue. TRAIN stores the pre-activations Z[l][row][n] (bf16) of record 0.
-template <bool TRAIN>
-__global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const float* __restrict__ conds,
+template <bool TRAIN, int TMF>
+__global__ void __launch_bounds__(256, kFwdBlocks) sur_forward(Net net, Static st, const float* __restrict__ conds,
                                                    float* __restrict__ logits, __bf16* __restrict__ Z)
 {
-    __shared__ __align__(16) __bf16 Hs[TM * 256];
-    const uint32_t rec = blockIdx.y, row0 = blockIdx.x * TM, Mpad = st.Mpad;
+    __shared__ __align__(16) __bf16 Hs[TMF * 256];
+    constexpr int RBF = TMF / 16;
+    const uint32_t rec = blockIdx.y, row0 = blockIdx.x * TMF, Mpad = st.Mpad;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c16 = lane & 15;
     const int n0 = 64 * wave + 4 * q;  // first of the lane's columns in each 16-column block
-    f32x4 acc[RB][4];
+    f32x4 acc[RBF][4];
     {
         // layer 1: z = (T[i] + (b1 + W1_c c)) + P[j]
         const float* c = conds + 4 * rec;
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const 
                 base[cb][i] = fmaf(w[3], c3, fmaf(w[2], c2, fmaf(w[1], c1, fmaf(w[0], c0, net.b[0][col]))));
             }
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
+        for (int rb = 0; rb < RBF; ++rb) {
             const uint32_t row = row0 + 16 * rb + c16;
             uint32_t ti = row / st.nq;
             const uint32_t j = row - ti * st.nq;  // canonical column
@@ -184,15 +192,15 @@ __global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const 
     }
     for (int layer = 0;; ++layer) {
         if (layer > 0) {
-            zero(acc);
-            slab_mma(Hs, net.w[layer - 1], 64 * wave, acc);
+            zero<RBF>(acc);
+            slab_mma<RBF>(Hs, net.w[layer - 1], 64 * wave, acc);
             __syncthreads();  // every wave has read the input tile
             const float* bias = net.b[layer] + n0;
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
                 const float4 b = *reinterpret_cast<const float4*>(bias + 16 * cb);
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) {
+                for (int rb = 0; rb < RBF; ++rb) {
                     acc[rb][cb][0] += b.x;
                     acc[rb][cb][1] += b.y;
                     acc[rb][cb][2] += b.z;
@@ -202,7 +210,7 @@ __global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const 
         }
         if (TRAIN) {
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
+            for (int rb = 0; rb < RBF; ++rb)
 #pragma unroll
                 for (int cb = 0; cb < 4; ++cb) {
                     const f32x4 z = acc[rb][cb];
@@ -212,7 +220,7 @@ __global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const 
         }
         if (layer == 3) break;
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
+        for (int rb = 0; rb < RBF; ++rb)
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
                 const f32x4 z = acc[rb][cb];
@@ -235,7 +243,7 @@ __global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const 
     }
     float* part = reinterpret_cast<float*>(Hs);
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
+    for (int rb = 0; rb < RBF; ++rb) {
         float a = 0;
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
@@ -243,12 +251,12 @@ __global__ void __launch_bounds__(256, 2) sur_forward(Net net, Static st, const 
             for (int i = 0; i < 4; ++i) a = fmaf(silu(acc[rb][cb][i]), w5[cb][i], a);
         a += __shfl_xor(a, 16, 64);
         a += __shfl_xor(a, 32, 64);
-        if (q == 0) part[wave * TM + 16 * rb + c16] = a;
+        if (q == 0) part[wave * TMF + 16 * rb + c16] = a;
     }
     __syncthreads();
-    if (threadIdx.x < TM) {
+    if (threadIdx.x < TMF) {
         const int row = threadIdx.x;
-        const float f = (part[row] + part[TM + row]) + (part[2 * TM + row] + part[3 * TM + row]);
+        const float f = (part[row] + part[TMF + row]) + (part[2 * TMF + row] + part[3 * TMF + row]);
         logits[(size_t)rec * Mpad + row0 + row] = (f + net.b5[0]) + st.logw[row0 + row];
     }
 }
@@ -849,11 +857,11 @@ hipError_t compare(const float* p, const float* q, uint32_t M, uint32_t np, cons
 hipError_t forward(const Net& net, const Static& st, const float* d_conds, uint32_t nrec, bool train, float* logits,
                    __bf16* Z, hipStream_t s)
 {
-    const dim3 grid(st.Mpad / TM, nrec);
+    const dim3 grid(st.Mpad / kFwdRows, nrec);
     if (train)
-        hipLaunchKernelGGL(sur_forward<true>, grid, dim3(256), 0, s, net, st, d_conds, logits, Z);
+        hipLaunchKernelGGL((sur_forward<true, kFwdRows>), grid, dim3(256), 0, s, net, st, d_conds, logits, Z);
     else
-        hipLaunchKernelGGL(sur_forward<false>, grid, dim3(256), 0, s, net, st, d_conds, logits, Z);
+        hipLaunchKernelGGL((sur_forward<false, kFwdRows>), grid, dim3(256), 0, s, net, st, d_conds, logits, Z);
     return hipGetLastError();
 }
 

@@ -165,7 +165,7 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     }
     st.nq = (uint32_t)canon.size();
     st.Mh = st.nt * st.nq;
-    st.Mpad = (st.Mh + 127) / 128 * 128;
+    st.Mpad = (st.Mh + 127) / 128 * 128;  // whole 128-row tiles (kTileRows, kFwdRows)
     // weight-gradient splits: rows per split a multiple of 32, ~256 splits
     st.rows_per_split = std::max<uint32_t>(32, (st.Mpad / 256 + 31) / 32 * 32);
     st.splits = (st.Mpad + st.rows_per_split - 1) / st.rows_per_split;
