@@ -19,9 +19,10 @@ metric, each measured on the device with inputs resident in HBM:
              against fresh traced records and records/s against tracing;
   cpu_baseline      the drop port (FP32 oracle mirror) on one host core;
   cpu_baseline_hex  the compiled reference hex tracer + compensated histogram
-                    (oracle/_ref) on one host core.
+                    (oracle/_ref) on one host core;
+  *_all_cores       the same on every host core (at most 16 processes).
 --config uneven runs the 512-record uneven batch instead (records/s, with the
-autotuned schedule).
+autotuned schedule); --config surrogate runs the surrogate leg alone.
 
 Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
 """
@@ -46,8 +47,9 @@ sys.path.insert(0, str(ROOT))
 BASE_COEFFS = (-0.0458, -0.0335, -0.1211, -0.0227)
 
 # Operation census of the drop kernel per event (flops, one per IEEE
-# operation including div/sqrt; see DESIGN.md "Roofline"): entry of a drop
-# ray, one internal interaction, one emission (power + FP64 binning).
+# operation including div/sqrt, counted on the FP32 mirror of the kernel and
+# on the binning): entry of a drop ray, one internal interaction, one emission
+# (power + FP64 binning). With FMA contraction off each is one instruction.
 CENSUS = {
     "drop": {"miss": 10, "entry": 1230, "hit": 1150, "emit": 102},
     "sphere": {"miss": 10, "entry": 250, "hit": 270, "emit": 102},
@@ -408,8 +410,9 @@ def main() -> int:
         print(json.dumps({"copy_bytes": a.numel() * 8, "launches": 4}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-              "and not launched (--gpus must be 1).", file=sys.stderr)
+        print("bench.py: this build runs on one GPU (--gpus must be 1); records and ray ranges would "
+              "shard across GPUs with one u64 histogram all-reduce per record, which is not launched here.",
+              file=sys.stderr)
         return 2
 
     import ice_crystal_phase_amd as ip

@@ -6,7 +6,7 @@ A conditional density MLP over the (u, phi) cells, conditioned on a record's
 on the GPU by the library's BF16 MFMA kernels on the cell probabilities of
 traced records (copied device to device into the surrogate's training set),
 and served as FP64 CDF records by the same CDF kernels as traced records. The
-plain tracer stays the judge and the fallback: `compare` measures total
+plain tracer stays the reference and the fallback: `compare` measures total
 variation and |g| error against traced records on the device, `fit_gate`
 turns such measurements on a validation set into the domain the network may
 serve, and `serve` traces every record outside it.

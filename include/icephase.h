@@ -230,7 +230,7 @@ int iph_batch_probabilities(iph_batch* b, uint32_t rec, float* d_out);
  * mass), and so is the network: phi_count must be even. BF16 MFMA kernels with FP32
  * accumulation for training (forward, backward, Adam) and serving. Training
  * records live in the surrogate's own device memory, filled from traced
- * records; the tracer stays the judge (iph_sur_compare). */
+ * records; the tracer stays the reference (iph_sur_compare). */
 typedef struct iph_sur_config {
     uint32_t theta_count, phi_count;
     uint32_t max_records;      /* records per predict / records / compare call */

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the phase-function path — TEST INFRASTRUCTURE ONLY.
  *
- * This library is the checker, never the product. Only tests/, the smoke
+ * This library checks the product and is never part of it. Only tests/, the smoke
  * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load
  * it. The product path (libicephase.so) never links, loads or calls it.
  *
