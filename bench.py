@@ -256,6 +256,19 @@ def _drop_port_sample(job) -> float:
     return time.perf_counter() - t0
 
 
+def host_cpu() -> dict:
+    """CPU model, logical CPUs of the host and those this process may use."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0))}
+
+
 def host_workers() -> int:
     """Worker processes for the all-core CPU baselines: the host cores this
     process may use, at most 16 (one GPU's share of a GPU node)."""
@@ -518,6 +531,7 @@ def main() -> int:
                                          f"binning and fixed-point histogram) in {dt:.1f} s"}
         out["gpu_over_cpu"] = rays_per_s / rps
         out["cpu_baseline_all_cores"] = all_cores.get("drop_port")
+        out["host_cpu"] = host_cpu()
         out["gpu_over_cpu_all_cores"] = rays_per_s / all_cores["drop_port"]["value"]
         if "hex_reference" in all_cores:
             out["cpu_baseline_hex_all_cores"] = all_cores["hex_reference"]
