@@ -22,7 +22,9 @@ metric, each measured on the device with inputs resident in HBM:
                     (oracle/_ref) on one host core;
   *_all_cores       the same on every host core (at most 16 processes).
 --config uneven runs the 512-record uneven batch instead (records/s, with the
-autotuned schedule); --config surrogate runs the surrogate leg alone.
+autotuned schedule); --config surrogate runs the surrogate leg alone;
+--config dataset times the in-process dataset driver against one
+rainbow_generate process per condition (records written to disk).
 
 Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
 """
@@ -394,6 +396,83 @@ def run_uneven(ctx, args):
     }
 
 
+def run_dataset(args):
+    """Dataset generation end to end (records written, checked and receipted on
+    disk): the in-process batch driver against one rainbow_generate process per
+    condition, the reference driver's way, at the reference dataset's record
+    size (3001^2 rays, 1800 x 3600 query cells coarsened to a 900 x 1800 CDF,
+    r = 1 mm Table I drop). Sample: 16 inclination cells x 3 wavelengths."""
+    import contextlib
+    import shutil
+    import subprocess
+    import tempfile
+    from ice_crystal_phase_amd import dataset as ds
+    work = Path(tempfile.mkdtemp(prefix="icephase_dataset_"))
+    raw = {"output_root": str(work / "data"), "generator": "bin/rainbow_generate", "radius_mm": 1.0,
+           "force_sphere": False, "temperature_c": 20.0, "pressure_pa": 101325.0,
+           "wavelengths": {"start_nm": 380, "stop_nm": 830, "step_nm": 225},
+           "inclinations": {"min_degrees": -90, "max_degrees": 90, "count": 16,
+                            "placement": "linear_angle_cell_centers"},
+           "solver": {"grid": 3001, "query_theta": 1800, "query_phi": 3600, "cdf_theta": 900, "cdf_phi": 1800,
+                      "stage": "incoherent", "device": 0, "focal_offsets": [0, 0, 0, 0],
+                      "allow_underresolved": False, "maximum_coarsening_tv": 1.0, "cdf_max_l1": 1e-10,
+                      "cdf_max_lost_mass": 1e-12},
+           "reserve_gib": 1}
+    cfg = work / "config.json"
+    cfg.write_text(json.dumps(raw))
+    c = ds.normalized_config(raw)
+    n = len(c["inclination_degrees"]) * len(c["wavelength_nm"])
+    traced_s = []
+    engine = ds.trace_conditions
+
+    def timed(ctx, conds, s):
+        t = time.perf_counter()
+        out = engine(ctx, conds, s)
+        traced_s.append(time.perf_counter() - t)
+        return out
+
+    nl = len(c["wavelength_nm"])
+    # one process per condition (the reference driver's engine model) for a few
+    # conditions, first, while this process has not opened the device
+    k = 4
+    t0 = time.perf_counter()
+    for j in range(k):
+        ii, ll = divmod(j, nl)
+        print(f"dataset: rainbow_generate process {j + 1}/{k}", file=sys.stderr, flush=True)
+        res = subprocess.run(ds.cli_command(work / f"alone_{j}", c, ii, ll), capture_output=True, text=True,
+                             timeout=300)
+        if res.returncode != 0:
+            raise RuntimeError("rainbow_generate failed: " + res.stderr)
+    per_process = (time.perf_counter() - t0) / k
+    ds.trace_conditions = timed
+    try:
+        print(f"dataset: in-process driver over {n} conditions", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        with contextlib.redirect_stdout(sys.stderr):  # the driver's progress lines; stdout keeps the JSON line
+            rc = ds.run(["--config", str(cfg), "--execute"])
+        wall = time.perf_counter() - t0
+    finally:
+        ds.trace_conditions = engine
+    if rc != 0:
+        raise RuntimeError("dataset driver failed")
+    same = all((work / "data" / f"records/i{j // nl:04d}/w{j % nl:04d}" / a).read_bytes() ==
+               (work / f"alone_{j}" / a).read_bytes() for j in range(k) for a in ds.ARRAYS)
+    shutil.rmtree(work)
+    return {
+        "metric": "dataset records/s written to disk on 1 MI355X (in-process batched driver)",
+        "value": n / wall, "unit": "records/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 drop trace, f64 bin/CDF", "data": "synthetic",
+        "config": {"workload": f"water drop r=1mm (Table I), {n} conditions (16 inclination cells x 3 wavelengths "
+                               "380/605/830 nm), 3001^2 rays, 1800x3600 query -> 900x1800 CDF, incoherent stage"},
+        "device_launch_s": sum(traced_s), "launches": len(traced_s),
+        "per_process_records_per_s": 1.0 / per_process,
+        "per_process_sample": f"{k} conditions, one bin/rainbow_generate process each",
+        "speedup_over_per_process": (n / wall) * per_process,
+        "records_equal_per_process": same,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,7 +486,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
-    ap.add_argument("--config", choices=("drops", "uneven", "surrogate"), default="drops")
+    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset"), default="drops")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy (torch) 4x: the FETCH_SIZE/WRITE_SIZE scale reference")
@@ -437,6 +516,9 @@ def main() -> int:
         with ip.Context(0) as ctx:
             out = run_uneven(ctx, args)
         print(json.dumps(out))
+        return 0
+    if args.config == "dataset":
+        print(json.dumps(run_dataset(args)))
         return 0
     if args.config == "surrogate":
         with ip.Context(0) as ctx:
