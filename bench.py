@@ -24,7 +24,9 @@ metric, each measured on the device with inputs resident in HBM:
 --config uneven runs the 512-record uneven batch instead (records/s, with the
 autotuned schedule); --config surrogate runs the surrogate leg alone;
 --config dataset times the in-process dataset driver against one
-rainbow_generate process per condition (records written to disk).
+rainbow_generate process per condition (records written to disk); --config
+atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
+atlas tool.
 
 Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
 """
@@ -473,6 +475,55 @@ def run_dataset(args):
     }
 
 
+def run_atlas(args):
+    """Deterministic atlas of the reference's preview configuration
+    (hex_atlas/configs/spectral_preview.json): a = 0.1 mm, L = 0.2 mm ice prism
+    at 550 nm (Warren-Brandt n), 46 x 16 incident nodes, GL 2x3 area rule with 2
+    subdivisions, 180 x 360 cells, up to 2048 internal hits, tail 1e-12.
+    Beside it, the reference's own CPU tool (oracle/_ref) over the same node
+    domain on a 10 x 6 node grid (one core), when it was built."""
+    import subprocess
+    import tempfile
+    from ice_crystal_phase_amd import icephase
+    n550 = icephase.ice_index(550.0).n
+    s = icephase.atlas_settings(0.1, 0.2, n550, 45, 15, 180, 360, 2, wavelength_nm=550.0)
+    nodes_total = 46 * 16
+    out = {}
+    with icephase.Context(0) as ctx:
+        icephase.atlas_trace(ctx, s, 0, 16)  # warm-up: module load, first allocations
+        t0 = time.perf_counter()
+        mass, nodes = icephase.atlas_trace(ctx, s)
+        wall = time.perf_counter() - t0
+    rays = sum(n.rays for n in nodes)
+    outputs = sum(n.outputs for n in nodes)
+    out.update({
+        "metric": "atlas nodes/s on 1 MI355X (deterministic area-quadrature hex atlas, mass_mm2 on the host)",
+        "value": nodes_total / wall, "unit": "nodes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 trace, u64 fixed-point bins, f64 masses", "data": "synthetic",
+        "config": {"workload": "hex prism a=0.1 mm L=0.2 mm, ice n(550 nm) WB2008, 46x16 incident nodes, "
+                               "GL2x3 area rule x 2^2 subdivisions, 180x360 cells, <=2048 internal hits, tail 1e-12"},
+        "rays": rays, "emissions": outputs, "rays_per_s": rays / wall,
+        "mass_bytes": int(mass.nbytes),
+        "max_residual_fraction": max(n.unresolved_mm2 / n.projected_area_mm2 for n in nodes),
+    })
+    ref_cli = ROOT / "oracle" / "_ref" / "ice_precompute_atlas_cpu"
+    if ref_cli.exists() and not args.no_cpu_baseline:
+        with tempfile.TemporaryDirectory() as tmp:
+            cmd = [str(ref_cli), "--out", tmp + "/ref", "--radius-mm", "0.1", "--length-mm", "0.2",
+                   "--wavelength-nm", "550", "--incident-theta-intervals", "9", "--incident-phi-intervals", "5",
+                   "--theta", "180", "--phi", "360", "--area-subdivisions", "2"]
+            t0 = time.perf_counter()
+            res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+            dt = time.perf_counter() - t0
+        if res.returncode == 0:
+            out["cpu_baseline"] = {"value": 60 / dt, "unit": "nodes/s", "cores": 1, "kind": "reference",
+                                   "sample": f"60 nodes (10x6 grid over the same incident domain) of the same "
+                                             f"prism, rule and cells, reference CPU atlas tool, {dt:.2f} s"}
+            out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -486,7 +537,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
-    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset"), default="drops")
+    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas"), default="drops")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy (torch) 4x: the FETCH_SIZE/WRITE_SIZE scale reference")
@@ -519,6 +570,9 @@ def main() -> int:
         return 0
     if args.config == "dataset":
         print(json.dumps(run_dataset(args)))
+        return 0
+    if args.config == "atlas":
+        print(json.dumps(run_atlas(args)))
         return 0
     if args.config == "surrogate":
         with ip.Context(0) as ctx:

@@ -354,16 +354,19 @@ struct Config {
 struct RayResult {
     double escaped;  // plain sum (audits use the quanta)
     double unresolved;
+    double max_balance;  // largest |R + T - incident| power of the ray's interfaces
     uint32_t status, internal_hits, tir, outputs;
 };
 
-// Trace sample `id`; sink(dir, power) is called for every escaping branch.
+// Trace one ray from a given entry point on a lit face (the deterministic
+// splitting chain of hex_trace/include/ice_crystal/hex_trace_core.hpp:102-162:
+// emit the exterior reflection, then the transmission at every internal
+// encounter, continuing the internal reflection); sink(dir, power) is called
+// for every escaping branch.
 template <class Sink>
-__device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
+__device__ inline RayResult trace_from(const Config& c, const Entry& e, Sink& sink)
 {
-    RayResult out{0, 1, kInvalid, 0, 0, 0};
-    const Entry e = pick_entry(c.prism, c.k, c.area, open_uniform(c.seed, id, 0), open_uniform(c.seed, id, 1),
-                               open_uniform(c.seed, id, 2));
+    RayResult out{0, 1, 0, kInvalid, 0, 0, 0};
     if (!e.ok || e.face >= 8) return out;
     if (!c.prism.on_face(e.pos, e.face)) {
         out.status = kAmbiguous;
@@ -376,6 +379,7 @@ __device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
         out.status = kNumeric;
         return out;
     }
+    out.max_balance = fabs(sp.balance);
     auto emit = [&](const Ray& r) -> uint32_t {
         const double pw = r.j.power();
         if (!finite64(pw) || pw < 0 || (!pw && r.j.nonzero())) return kNumeric;
@@ -426,6 +430,7 @@ __device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
         }
         ++out.internal_hits;
         if (sp.tir) ++out.tir;
+        out.max_balance = gt_max(out.max_balance, fabs(sp.balance));
         if (sp.has_t) {
             em = emit(sp.t);
             if (em != kExhausted) {
@@ -445,6 +450,15 @@ __device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
     }
     out.status = rem == 0 ? kExhausted : (rem <= c.tail ? kTail : kLimit);
     return out;
+}
+
+// Trace sample `id`: entry point drawn from the counter RNG over the lit faces.
+template <class Sink>
+__device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
+{
+    const Entry e = pick_entry(c.prism, c.k, c.area, open_uniform(c.seed, id, 0), open_uniform(c.seed, id, 1),
+                               open_uniform(c.seed, id, 2));
+    return trace_from(c, e, sink);
 }
 
 }  // namespace hex

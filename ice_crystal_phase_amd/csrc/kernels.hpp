@@ -19,6 +19,8 @@ enum StatSlot : uint32_t {
     kStatUnresolved = 15,
     kStatInternalHits = 16,
     kStatDiffraction = 17,   // quanta of the Fraunhofer lobe
+    kStatRayBalance = 18,    // area-rule records: max |escaped + unresolved - 1| of a ray (FP64 bits)
+    kStatFaceBalance = 19,   // area-rule records: max |R + T - incident| of an interface (FP64 bits)
     kStatScratch = 23,       // scratch word of host-requested reductions (probability export)
     kStatCount = 24,
 };
@@ -41,6 +43,15 @@ struct Lobe {
 
 enum RecordKind : uint32_t { kKindDrop = 0, kKindHex = 1 };
 
+// One deterministic entry of a hex record traced from an area quadrature
+// instead of the counter RNG: entry point (circumradius units) on a lit face,
+// and the ray's weight relative to the record's largest (qf <= 1).
+struct AreaRay {
+    d3 pos;
+    double qf;
+    uint32_t face, pad;
+};
+
 struct DevRecord {
     uint32_t kind;
     uint32_t nt, np, qbits;
@@ -57,6 +68,7 @@ struct DevRecord {
     double rot[9];              // hex: body -> ensemble rotation (row-major) when rotated
     uint32_t rotated, lobe_on;
     uint32_t owns;              // 1: this record's histogram (zeroed with it); 0: a node adding into another's
+    const AreaRay* area;        // hex, area kernel only: the entries of an area quadrature (rays of them)
     drop::Config drop;
     hex::Config hex;
     Lobe lobe;

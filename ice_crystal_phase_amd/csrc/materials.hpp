@@ -47,4 +47,11 @@ struct AreaResult {
 // (reference: src/projected_area.cpp:163-228).
 bool drop_projected_area(const double c[8], double radius_mm, const double k[3], AreaResult& out, std::string& err);
 
+// Warren & Brandt (2008) ice index at a vacuum wavelength in [200, 1100] nm:
+// n (used by the tracer) and kappa (reported, not applied), with the bracket.
+struct IceState {
+    double wavelength_nm, n, kappa, lower_nm, upper_nm, log_wavelength_fraction;
+};
+bool ice_index_wb2008(double wavelength_nm, IceState& out, std::string& err);
+
 }  // namespace icephase

@@ -175,6 +175,9 @@ uint32_t iph_struct_size(int which)
     case 3: return (uint32_t)sizeof(iph_cdf_policy);
     case 4: return (uint32_t)sizeof(iph_water_state);
     case 5: return (uint32_t)sizeof(iph_area);
+    case 6: return (uint32_t)sizeof(iph_ice_state);
+    case 7: return (uint32_t)sizeof(iph_atlas_settings);
+    case 8: return (uint32_t)sizeof(iph_atlas_node);
     default: return 0;
     }
 }

@@ -85,11 +85,45 @@ __device__ inline void deposit(const DevRecord& R, d3 dir, double power, EdgeLoa
     }
 }
 
-template <int KIND, class EdgeLoad>
+// Largest of v over the wave, merged into a non-negative FP64 counter by its
+// bit pattern (non-negative doubles order like their u64 bits).
+__device__ inline void wave_max_into(double v, unsigned long long* slot)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = gt_max(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// Hex tile of an area-quadrature record: ray i enters at R.area[i] and carries
+// the weight qf relative to the record's quantum.
+template <class EdgeLoad>
+__device__ inline void area_tile(const DevRecord& R, uint64_t first, uint64_t count, EdgeLoad edge, LaneCounters& c)
+{
+    double ray_balance = 0, face_balance = 0;
+    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
+        const AreaRay a = R.area[first + i];
+        const hex::Entry e{a.pos, a.face, true};
+        unsigned long long rq = 0;
+        auto sink = [&](d3 dir, double pw) { deposit(R, dir, pw * a.qf, edge, c, rq); };
+        const hex::RayResult r = hex::trace_from(R.hex, e, sink);
+        c.escaped += rq;
+        c.unresolved += quantize(r.unresolved * a.qf * R.qfactor, R.qbits);
+        c.hits += r.internal_hits;
+        count_status(c, r.status);
+        ray_balance = gt_max(ray_balance, fabs(r.escaped + r.unresolved - 1));
+        face_balance = gt_max(face_balance, r.max_balance);
+    }
+    wave_max_into(ray_balance, &R.stats[kStatRayBalance]);
+    wave_max_into(face_balance, &R.stats[kStatFaceBalance]);
+}
+
+template <int KIND, bool AREA, class EdgeLoad>
 __device__ inline void trace_tile(const DevRecord& R, uint64_t first, uint64_t count, EdgeLoad edge,
                                   LaneCounters& c)
 {
-    if constexpr (KIND == kKindHex) {
+    if constexpr (KIND == kKindHex && AREA) {
+        area_tile(R, first, count, edge, c);
+    } else if constexpr (KIND == kKindHex) {
         for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
             unsigned long long rq = 0;
             auto sink = [&](d3 dir, double pw) { deposit(R, dir, pw, edge, c, rq); };
@@ -293,7 +327,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
 
 // Persistent tile loop of one record kind. LDS_EDGES: the record's u edges are
 // staged in LDS (nt < kLdsEdges); otherwise they are read from global memory.
-template <int KIND, bool LDS_EDGES>
+template <int KIND, bool LDS_EDGES, bool AREA = false>
 __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     __shared__ double s_edges[LDS_EDGES ? kLdsEdges : 1];
@@ -334,13 +368,13 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
                 __syncthreads();
             }
             if constexpr (KIND == kKindHex)
-                trace_tile<KIND>(R, first, count, [&](int k) { return s_edges[k]; }, c);
+                trace_tile<KIND, AREA>(R, first, count, [&](int k) { return s_edges[k]; }, c);
             else
                 drop_tile(R, first, count, &s_next, [&](int k) { return s_edges[k]; }, c);
         } else {
             const double* e = R.edges;
             if constexpr (KIND == kKindHex)
-                trace_tile<KIND>(R, first, count, [&](int k) { return e[k]; }, c);
+                trace_tile<KIND, AREA>(R, first, count, [&](int k) { return e[k]; }, c);
             else
                 drop_tile(R, first, count, &s_next, [&](int k) { return e[k]; }, c);
         }
@@ -357,6 +391,17 @@ extern "C" __global__ void __launch_bounds__(256)
     icephase_trace_hex(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindHex, true>(recs, map, queue);
+}
+// hex records whose rays enter at the points of an area quadrature (atlas nodes)
+extern "C" __global__ void __launch_bounds__(256)
+    icephase_trace_hex_area(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
+{
+    trace_loop<kKindHex, true, true>(recs, map, queue);
+}
+extern "C" __global__ void __launch_bounds__(256)
+    icephase_trace_hex_area_wide(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
+{
+    trace_loop<kKindHex, false, true>(recs, map, queue);
 }
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
     icephase_trace_drop_wide(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
@@ -424,6 +469,17 @@ hipError_t launch_trace(int kind, bool wide, const DevRecord* d_recs, TileMap ma
         else
             hipLaunchKernelGGL(icephase_trace_drop, dim3(blocks), dim3(threads), 0, s, d_recs, map, d_queue);
     }
+    return hipGetLastError();
+}
+
+// hex records traced from area-quadrature entries (DevRecord::area set)
+hipError_t launch_trace_area(bool wide, const DevRecord* d_recs, TileMap map, uint32_t* d_queue, uint32_t blocks,
+                             uint32_t threads, hipStream_t s)
+{
+    if (wide)
+        hipLaunchKernelGGL(icephase_trace_hex_area_wide, dim3(blocks), dim3(threads), 0, s, d_recs, map, d_queue);
+    else
+        hipLaunchKernelGGL(icephase_trace_hex_area, dim3(blocks), dim3(threads), 0, s, d_recs, map, d_queue);
     return hipGetLastError();
 }
 

@@ -83,6 +83,25 @@ class CdfReport(C.Structure):
                 ("weight_underflow", u64), ("negative", u64), ("status", i32), ("pad", i32)]
 
 
+class IceState(C.Structure):
+    _fields_ = [("wavelength_nm", f64), ("n", f64), ("kappa", f64), ("lower_nm", f64), ("upper_nm", f64),
+                ("log_wavelength_fraction", f64)]
+
+
+class AtlasSettings(C.Structure):
+    _fields_ = [("radius_mm", f64), ("length_mm", f64), ("interior_index", f64), ("exterior_index", f64),
+                ("wavelength_nm", f64), ("incident_theta_intervals", u32), ("incident_phi_intervals", u32),
+                ("theta_count", u32), ("phi_count", u32), ("area_subdivisions", u32), ("max_internal_hits", u32),
+                ("tail_tolerance", f64), ("max_unresolved_fraction", f64), ("max_balance_error", f64)]
+
+
+class AtlasNode(C.Structure):
+    _fields_ = [("i", u32), ("j", u32), ("theta", f64), ("phi", f64), ("ki", f64 * 3), ("e0", f64 * 3),
+                ("projected_area_mm2", f64), ("escaped_mm2", f64), ("unresolved_mm2", f64),
+                ("histogram_relative_error", f64), ("rays", u64), ("outputs", u64), ("max_balance_error", f64),
+                ("max_interface_balance_error", f64)]
+
+
 _lib: C.CDLL | None = None
 
 
@@ -137,8 +156,14 @@ def lib() -> C.CDLL:
     L.iph_sur_compare.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.iph_sur_records.argtypes = [C.c_void_p, C.c_void_p, u32, P(CdfPolicy), C.c_void_p, C.c_void_p, C.c_void_p]
     L.iph_sur_timing.argtypes = [C.c_void_p, P(f64)]
+    L.iph_ice_index.argtypes = [f64, P(IceState)]
+    L.iph_atlas_node_rays.argtypes = [P(AtlasSettings), u32, u32, C.c_void_p, u64, P(u64), P(AtlasNode)]
+    L.iph_atlas_trace.argtypes = [C.c_void_p, P(AtlasSettings), u32, u32, C.c_void_p, C.c_void_p]
+    L.iph_write_npy.argtypes = [C.c_char_p, P(u64), u32, C.c_void_p]
     for which, (name, st) in enumerate((("Record", Record), ("HistStats", HistStats), ("CdfReport", CdfReport),
-                                        ("CdfPolicy", CdfPolicy), ("WaterState", WaterState), ("Area", Area))):
+                                        ("CdfPolicy", CdfPolicy), ("WaterState", WaterState), ("Area", Area),
+                                        ("IceState", IceState), ("AtlasSettings", AtlasSettings),
+                                        ("AtlasNode", AtlasNode))):
         if L.iph_struct_size(which) != C.sizeof(st):
             raise IcePhaseError(-1, f"ABI mismatch for {name}: {L.iph_struct_size(which)} != {C.sizeof(st)}")
     if L.iph_abi_version() != ABI_VERSION:
@@ -162,7 +187,8 @@ EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_des
             "iph_drop_projected_area", "iph_hex_projected_area", "iph_batch_probabilities", "iph_sur_create",
             "iph_sur_destroy", "iph_sur_param_count", "iph_sur_get_params", "iph_sur_set_params",
             "iph_sur_set_record", "iph_sur_set_target", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",
-            "iph_sur_compare", "iph_sur_records", "iph_sur_timing")
+            "iph_sur_compare", "iph_sur_records", "iph_sur_timing", "iph_ice_index", "iph_atlas_node_rays",
+            "iph_atlas_trace", "iph_write_npy")
 
 
 def u_edges(nt: int) -> np.ndarray:
@@ -205,6 +231,31 @@ def hex_projected_area(radius_mm: float, length_mm: float, incident) -> float:
     a = f64()
     _check(lib().iph_hex_projected_area(radius_mm, length_mm, k, C.byref(a)))
     return a.value
+
+
+def ice_index(wavelength_nm: float) -> IceState:
+    """Warren & Brandt (2008) ice Ih index (iph_ice_index)."""
+    st = IceState()
+    _check(lib().iph_ice_index(wavelength_nm, C.byref(st)))
+    return st
+
+
+def atlas_settings(radius_mm: float, length_mm: float, n_int: float, theta_intervals: int, phi_intervals: int,
+                   nt: int, np_: int, subdivisions: int, wavelength_nm: float = 550.0, n_ext: float = 1.0,
+                   max_internal_hits: int = 2048, tail: float = 1e-12, max_unresolved: float = 1e-10,
+                   max_balance: float = 1e-10) -> AtlasSettings:
+    return AtlasSettings(radius_mm, length_mm, n_int, n_ext, wavelength_nm, theta_intervals, phi_intervals, nt, np_,
+                         subdivisions, max_internal_hits, tail, max_unresolved, max_balance)
+
+
+def atlas_node_rays(s: AtlasSettings, i: int, j: int):
+    """Area rays of atlas node (i, j): rows (x, y, z, face, weight mm^2), and the node."""
+    n = u64()
+    node = AtlasNode()
+    _check(lib().iph_atlas_node_rays(C.byref(s), i, j, None, 0, C.byref(n), C.byref(node)))
+    out = np.zeros((n.value, 5), np.float64)
+    _check(lib().iph_atlas_node_rays(C.byref(s), i, j, out.ctypes.data, n.value, C.byref(n), C.byref(node)))
+    return out, node
 
 
 # ------------------------------------------------------------------ records
@@ -331,6 +382,17 @@ class Context:
         if check:
             _check(rc, self._p)
         return phi, theta.reshape(onp, ont + 1), rep, rc
+
+
+def atlas_trace(ctx: "Context", s: AtlasSettings, first: int = 0, count: int | None = None):
+    """Cell masses [count, nt, np] (mm^2) and node reports of atlas nodes
+    first..first+count-1 (iph_atlas_trace)."""
+    total = (s.incident_theta_intervals + 1) * (s.incident_phi_intervals + 1)
+    count = total - first if count is None else count
+    mass = np.zeros((count, s.theta_count, s.phi_count), np.float64)
+    nodes = (AtlasNode * count)()
+    _check(lib().iph_atlas_trace(ctx.ptr, C.byref(s), first, count, mass.ctypes.data, nodes), ctx.ptr)
+    return mass, list(nodes)
 
 
 @dataclass(eq=False)

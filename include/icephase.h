@@ -133,8 +133,9 @@ typedef struct iph_sur iph_sur;
  * (device/phase_cdf.cu:65,240; hex_trace/device/hex_trace.cu:6-11;
  * hex_trace/src/hex_trace_cuda.cpp:39-44). Returns IPH_ABI_VERSION. */
 int iph_abi_version(void);
-/* sizeof(iph_record), sizeof(iph_hist_stats), sizeof(iph_cdf_report) for
- * binding checks. */
+/* sizeof of the C-ABI structs for binding checks: 0 iph_record, 1
+ * iph_hist_stats, 2 iph_cdf_report, 3 iph_cdf_policy, 4 iph_water_state,
+ * 5 iph_area, 6 iph_ice_state, 7 iph_atlas_settings, 8 iph_atlas_node. */
 uint32_t iph_struct_size(int which);
 
 /* Replaces rainbow::CudaContext(device) (src/cuda_driver.cpp:148-213).
@@ -301,6 +302,64 @@ int iph_drop_projected_area(const double coefficients[8], double radius_mm, cons
                             iph_area* out);
 /* Replaces HexPrism::projected_area_mm2 (hex_trace/include/ice_crystal/hex_prism.hpp:45-51). */
 int iph_hex_projected_area(double radius_mm, double length_mm, const double incident[3], double* area_mm2);
+
+typedef struct iph_ice_state {
+    double wavelength_nm, n, kappa, lower_nm, upper_nm, log_wavelength_fraction;
+} iph_ice_state;
+/* Replaces iceCrystal::optics::evaluate_ice_wb2008 (hex_atlas/src/ice_optics.cpp:14-30):
+ * Warren & Brandt (2008) ice Ih index, n linear and log(kappa) linear in
+ * log(wavelength), knots exact, 200..1100 nm (IPH_E_INVALID outside). */
+int iph_ice_index(double wavelength_nm, iph_ice_state* out);
+
+/* ---- deterministic single-particle atlas --------------------------------- */
+
+/* Replaces iceCrystal::atlas::Settings (hex_atlas/include/ice_crystal/atlas.hpp:22-30)
+ * without the per-ray column work limits (batch_rays, max_work_bytes), which
+ * this implementation does not need. Incident nodes (i, j), i = 0..T, j = 0..P,
+ * cover theta = (pi/2) i/T, phi = (pi/6) j/P: the D6h fundamental domain. */
+typedef struct iph_atlas_settings {
+    double radius_mm, length_mm;  /* circumradius, full length */
+    double interior_index, exterior_index, wavelength_nm;
+    uint32_t incident_theta_intervals, incident_phi_intervals;
+    uint32_t theta_count, phi_count, area_subdivisions, max_internal_hits;
+    double tail_tolerance, max_unresolved_fraction, max_balance_error;
+} iph_atlas_settings;
+
+/* One incident node: its direction and frame (k = (sin t cos p, cos t, sin t
+ * sin p), e0 = (cos t cos p, -sin t, cos t sin p) normalized, e1 = k x e0) and
+ * the audits apps/precompute_atlas.cpp:148-153 reports per node. */
+typedef struct iph_atlas_node {
+    uint32_t i, j;
+    double theta, phi, ki[3], e0[3];
+    double projected_area_mm2, escaped_mm2, unresolved_mm2, histogram_relative_error;
+    uint64_t rays, outputs;
+    double max_balance_error, max_interface_balance_error;
+} iph_atlas_node;
+
+/* Replaces atlas::make_area_rule + make_node (hex_atlas/src/atlas.cpp:11-88),
+ * host only: the deterministic area rays of node (i, j), GL 2 x 3 points in
+ * each of area_subdivisions^2 equal-area cells of every lit face (6 triangular
+ * sectors on a basal face). rays: capacity x 5 doubles (x, y, z in
+ * circumradius units, face, projected weight mm^2), may be NULL to query the
+ * count. node: its direction, frame and projected area (audits zero). */
+int iph_atlas_node_rays(const iph_atlas_settings* s, uint32_t i, uint32_t j, double* rays, uint64_t capacity,
+                        uint64_t* count, iph_atlas_node* node);
+
+/* Replaces atlas::CudaTracer::trace / trace_cpu over a range of nodes
+ * (hex_atlas/src/atlas_cuda.cpp:28-57, src/atlas.cpp:139-154; the node loop of
+ * apps/precompute_atlas.cpp:139-156): nodes n = first .. first+count-1 in
+ * (i, j) order, n = i (P+1) + j, traced together in one persistent launch.
+ * Every area ray follows the deterministic splitting chain and bins its
+ * weighted emissions into the node's u64 fixed-point histogram (order-free);
+ * pole emissions are spread uniformly over the end-cap row. mass_mm2: count x
+ * theta_count x phi_count FP64 cell masses. Returns IPH_E_AUDIT with the
+ * reference's messages when a ray ends in a rejected status or a node fails its
+ * area / escaped / residual conservation audit. */
+int iph_atlas_trace(iph_ctx* ctx, const iph_atlas_settings* s, uint32_t first, uint32_t count, double* mass_mm2,
+                    iph_atlas_node* nodes);
+
+/* NPY v1.0 '<f8' C-order file of the given shape (src/npy_writer.cpp:18-95). */
+int iph_write_npy(const char* path, const uint64_t* shape, uint32_t ndim, const double* data);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,22 @@ int64_t orc_hex_trace(const orc_hex_settings* s, uint64_t first, uint64_t count,
                       orc_sample* samples, uint64_t cap, orc_ray_audit* audits);
 double orc_hex_projected_area(double radius_mm, double length_mm, const double k[3]);
 
+/* ---- deterministic atlas (hex_atlas/src/atlas.cpp) -------------------------- */
+typedef struct orc_atlas_settings {
+    double radius_mm, length_mm, n_int, n_ext, tail;
+    uint32_t theta_intervals, phi_intervals, subdivisions, theta_count, phi_count, max_hits;
+} orc_atlas_settings;
+typedef struct orc_atlas_report {
+    double theta, phi, ki[3], e0[3];
+    double projected_area_mm2, escaped_mm2, unresolved_mm2, max_balance_error, max_interface_balance_error;
+    uint64_t rays, outputs;
+} orc_atlas_report;
+/* Area rule of node (i, j): rows (x, y, z, face, weight mm^2); out may be NULL
+ * to count. Returns the ray count, -1 when cap is too small, < -1 on failure. */
+int64_t orc_atlas_rule(const orc_atlas_settings* s, uint32_t i, uint32_t j, double* out, uint64_t cap);
+/* Cell masses (mm^2, nt*np) of node (i, j); 0 on success. */
+int orc_atlas_node(const orc_atlas_settings* s, uint32_t i, uint32_t j, double* mass, orc_atlas_report* rep);
+
 /* ---- fixed-point histogram -------------------------------------------------- */
 typedef struct orc_hist_stats {
     uint64_t q_forward, q_backward;   /* pole quanta */

@@ -369,11 +369,11 @@ static orc_ray_audit finish(orc_ray_audit a, kahan esc, double residual, int sta
     a.balance = esc.value + residual - 1; a.status = (uint32_t)status;
     return a;
 }
-static orc_ray_audit trace_one(const hexcfg* c, uint64_t id, sink* sk)
+/* hex_trace_core.hpp:102-162 (trace_from_entry) */
+static orc_ray_audit trace_entry(const hexcfg* c, uint64_t id, entry_s e, sink* sk)
 {
     orc_ray_audit a; memset(&a, 0, sizeof a); a.unresolved = 1; a.status = ST_INVALID;
     kahan esc = {0, 0};
-    const entry_s e = sample_entry(c->h, c->k, uni(c->seed, id, 0), uni(c->seed, id, 1), uni(c->seed, id, 2));
     if (id >= c->total || !e.valid || e.face >= 8) return finish(a, esc, 1, ST_INVALID);
     if (!strict_face_point(c->h, e.pos, e.face)) return finish(a, esc, 1, ST_AMBIG);
     if (!(dot(pnormal(e.face), c->k) < 0)) return finish(a, esc, 1, ST_INVALID);
@@ -414,6 +414,12 @@ static orc_ray_audit trace_one(const hexcfg* c, uint64_t id, sink* sk)
     const double rem = jpow(&ray.j);
     if (!fin(rem) || rem < 0 || (!rem && jnonzero(&ray.j))) return finish(a, esc, rem, ST_NUMERIC);
     return finish(a, esc, rem, rem == 0 ? ST_EXHAUSTED : (rem <= c->tail ? ST_TAIL : ST_LIMIT));
+}
+/* hex_trace_core.hpp:163-170 (trace_sample) */
+static orc_ray_audit trace_one(const hexcfg* c, uint64_t id, sink* sk)
+{
+    const entry_s e = sample_entry(c->h, c->k, uni(c->seed, id, 0), uni(c->seed, id, 1), uni(c->seed, id, 2));
+    return trace_entry(c, id, e, sk);
 }
 
 int64_t orc_hex_trace(const orc_hex_settings* s, uint64_t first, uint64_t count,
@@ -948,4 +954,146 @@ int orc_build_cdf(const double* mass, uint32_t nt, uint32_t np, uint32_t ont, ui
 fail:
     free(e); free(ce); free(coarse); free(cols);
     return -1;
+}
+
+/* ---- deterministic atlas node (hex_atlas/src/atlas.cpp) ---------------------
+ * Node direction and frame (:69-88), the GL 2x3 area rule (:11-51), and the
+ * CPU trace (:139-154): each area ray is traced from its entry, its weighted
+ * emissions are binned into a per-ray compensated column (pole emissions spread
+ * as w/np over the end-cap row, atlas.hpp:62-73), and the columns are added to
+ * the node's compensated cells in ray order. */
+static int atlas_node_geometry(uint32_t T, uint32_t P, uint32_t i, uint32_t j, v3* k, frm* f, double* theta,
+                               double* phi)
+{
+    const double pi = 3.141592653589793238462643383279502884;
+    *theta = (pi / 2) * i / T;
+    *phi = (pi / 6) * j / P;
+    const double st = i == 0 ? 0 : (i == T ? 1 : sin(*theta));
+    const double ct = i == 0 ? 1 : (i == T ? 0 : cos(*theta));
+    const double sp = j == 0 ? 0 : (j == P ? 0.5 : sin(*phi));
+    const double cp = j == 0 ? 1 : (j == P ? SQRT3_D / 2 : cos(*phi));
+    v3 k1, e;
+    if (!nrm(mk(st * cp, ct, st * sp), &k1) || !nrm(k1, k)) return -1;
+    if (!nrm(mk(ct * cp, -st, ct * sp), &e)) return -1;
+    f->e0 = e;
+    f->e1 = crs(*k, e);
+    return frame_valid(*f, *k) ? 0 : -1;
+}
+
+int64_t orc_atlas_rule(const orc_atlas_settings* s, uint32_t i, uint32_t j, double* out, uint64_t cap)
+{
+    prism h = {s->radius_mm, s->length_mm};
+    v3 k; frm f; double theta, phi;
+    if (!prism_valid(h) || i > s->theta_intervals || j > s->phi_intervals ||
+        atlas_node_geometry(s->theta_intervals, s->phi_intervals, i, j, &k, &f, &theta, &phi))
+        return -2;
+    const uint32_t n = s->subdivisions;
+    const double ga = 1 / sqrt(3.0), gb = sqrt(3.0 / 5.0);
+    const double u[2] = {(1 - ga) / 2, (1 + ga) / 2};
+    const double v[3] = {(1 - gb) / 2, 0.5, (1 + gb) / 2};
+    const double vw[3] = {5.0 / 18, 4.0 / 9, 5.0 / 18};
+    kahan total = {0, 0};
+    uint64_t m = 0;
+    for (unsigned face = 0; face < 8; ++face) {
+        const double projected = face_area(h, face) * mx(0, -dot(pnormal(face), k));
+        if (projected == 0) continue;
+        const unsigned sectors = face < 6 ? 1 : 6;
+        for (unsigned sector = 0; sector < sectors; ++sector)
+            for (uint32_t a = 0; a < n; ++a)
+                for (uint32_t b = 0; b < n; ++b)
+                    for (unsigned gi = 0; gi < 2; ++gi)
+                        for (unsigned gj = 0; gj < 3; ++gj) {
+                            const double x = (a + u[gi]) / n, y = (b + v[gj]) / n;
+                            v3 p;
+                            if (face < 6) {
+                                const v3 nf = pnormal(face);
+                                const v3 t = mk(-nf.z, 0, nf.x);
+                                p = add(add(scl(nf, SQRT3_D / 2), scl(t, x - 0.5)), mk(0, (2 * y - 1) * half_len(h), 0));
+                            } else {
+                                p = scl(add(scl(pvertex(sector), 1 - y), scl(pvertex(sector + 1), y)), sqrt(x));
+                                p.y = (face == 6 ? 1 : -1) * half_len(h);
+                            }
+                            const double w = (projected / sectors / (double)n / n) * 0.5 * vw[gj];
+                            if (!(w > 0) || !fin(w) || !strict_face_point(h, p, face)) return -3;
+                            if (out) {
+                                if (m >= cap) return -1;
+                                double* o = out + 5 * m;
+                                o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = (double)face; o[4] = w;
+                            }
+                            kadd(&total, w);
+                            ++m;
+                        }
+    }
+    if (!m || fabs(total.value / proj_area(h, k) - 1) > 1e-12) return -4;
+    return (int64_t)m;
+}
+
+int orc_atlas_node(const orc_atlas_settings* s, uint32_t i, uint32_t j, double* mass, orc_atlas_report* rep)
+{
+    prism h = {s->radius_mm, s->length_mm};
+    v3 k; frm f; double theta, phi;
+    if (atlas_node_geometry(s->theta_intervals, s->phi_intervals, i, j, &k, &f, &theta, &phi)) return -2;
+    const int64_t m = orc_atlas_rule(s, i, j, NULL, 0);
+    if (m <= 0) return -3;
+    double* rule = (double*)malloc((size_t)m * 5 * sizeof(double));
+    const uint32_t nt = s->theta_count, np = s->phi_count;
+    const size_t cells = (size_t)nt * np;
+    kahan* acc = (kahan*)calloc(cells, sizeof(kahan));
+    kahan* col = (kahan*)calloc(cells, sizeof(kahan));
+    double* edges = (double*)malloc((nt + 1) * sizeof(double));
+    const uint64_t cap = 2 * (uint64_t)s->max_hits + 4;
+    orc_sample* smp = (orc_sample*)malloc(cap * sizeof(orc_sample));
+    int rc = 0;
+    if (!rule || !acc || !col || !edges || !smp || orc_atlas_rule(s, i, j, rule, (uint64_t)m) != m ||
+        orc_u_edges(nt, edges)) {
+        rc = -4;
+        goto done;
+    }
+    hexcfg c;
+    c.h = h; c.k = k; c.f = f; c.ni = s->n_ext; c.nt = s->n_int; c.tail = s->tail;
+    c.seed = 0; c.total = (uint64_t)m; c.max_hits = s->max_hits;
+    orc_frame g;
+    g.ki[0] = k.x; g.ki[1] = k.y; g.ki[2] = k.z;
+    g.e0[0] = f.e0.x; g.e0[1] = f.e0.y; g.e0[2] = f.e0.z;
+    g.e1[0] = f.e1.x; g.e1[1] = f.e1.y; g.e1[2] = f.e1.z;
+    memset(rep, 0, sizeof *rep);
+    rep->ki[0] = k.x; rep->ki[1] = k.y; rep->ki[2] = k.z;
+    rep->e0[0] = f.e0.x; rep->e0[1] = f.e0.y; rep->e0[2] = f.e0.z;
+    rep->theta = theta; rep->phi = phi;
+    rep->projected_area_mm2 = proj_area(h, k);
+    rep->rays = (uint64_t)m;
+    kahan escaped = {0, 0}, unresolved = {0, 0};
+    for (int64_t r = 0; r < m; ++r) {
+        const double* q = rule + 5 * r;
+        entry_s e; e.pos = mk(q[0], q[1], q[2]); e.face = (unsigned)q[3]; e.valid = 1;
+        const double w = q[4];
+        sink sk = {smp, cap, 0, 0};
+        const orc_ray_audit a = trace_entry(&c, (uint64_t)r, e, &sk);
+        if (sk.overflow || !(a.status == ST_EXHAUSTED || a.status == ST_TAIL)) { rc = -5; goto done; }
+        rep->outputs += a.outputs;
+        rep->max_balance_error = mx(rep->max_balance_error, fabs(a.balance));
+        rep->max_interface_balance_error = mx(rep->max_interface_balance_error, a.max_interface);
+        kadd(&escaped, w * a.escaped);
+        kadd(&unresolved, w * a.unresolved);
+        memset(col, 0, cells * sizeof(kahan));
+        for (uint64_t t = 0; t < sk.n; ++t) {
+            const orc_bin b = orc_phase_bin(edges, nt, np, &g, smp[t].dir);
+            const double wp = w * smp[t].power;
+            if (!b.valid || !(wp > 0) || !fin(wp)) { rc = -6; goto done; }
+            if (!b.pole) {
+                kadd(&col[b.index], wp);
+            } else {
+                const size_t row = b.pole > 0 ? 0 : nt - 1;
+                for (uint32_t jj = 0; jj < np; ++jj) kadd(&col[row * np + jj], wp / np);
+            }
+        }
+        for (size_t q2 = 0; q2 < cells; ++q2)
+            if (col[q2].value > 0) kadd(&acc[q2], col[q2].value);
+    }
+    for (size_t q2 = 0; q2 < cells; ++q2) mass[q2] = acc[q2].value;
+    rep->escaped_mm2 = escaped.value;
+    rep->unresolved_mm2 = unresolved.value;
+done:
+    free(rule); free(acc); free(col); free(edges); free(smp);
+    return rc;
 }

@@ -61,6 +61,18 @@ class RefCdfReport(C.Structure):
                 ("lost_cells", u64)]
 
 
+class AtlasSettings(C.Structure):
+    _fields_ = [("radius_mm", f64), ("length_mm", f64), ("n_int", f64), ("n_ext", f64), ("tail", f64),
+                ("theta_intervals", u32), ("phi_intervals", u32), ("subdivisions", u32), ("theta_count", u32),
+                ("phi_count", u32), ("max_hits", u32)]
+
+
+class AtlasReport(C.Structure):
+    _fields_ = [("theta", f64), ("phi", f64), ("ki", f64 * 3), ("e0", f64 * 3), ("projected_area_mm2", f64),
+                ("escaped_mm2", f64), ("unresolved_mm2", f64), ("max_balance_error", f64),
+                ("max_interface_balance_error", f64), ("rays", u64), ("outputs", u64)]
+
+
 class DropRecord(C.Structure):
     _fields_ = [("coeffs", f32 * 8), ("inner", f32), ("outer", f32), ("rmax", f32), ("ki", f32 * 3), ("e0", f32 * 3),
                 ("e1", f32 * 3), ("n_int", f32), ("n_ext", f32), ("half_extent", f32), ("grid_w", u32),
@@ -131,8 +143,32 @@ def lib() -> C.CDLL:
         L.orc_lobe_cell.argtypes = [P(f64), P(f64), u32, f64, f64, P(f64), P(f64), f64, P(Frame), u32, u32, u32,
                                     u32]
         L.orc_lobe_cell.restype = f64
+        L.orc_atlas_rule.argtypes = [P(AtlasSettings), u32, u32, C.c_void_p, u64]
+        L.orc_atlas_rule.restype = i64
+        L.orc_atlas_node.argtypes = [P(AtlasSettings), u32, u32, C.c_void_p, P(AtlasReport)]
         _lib = L
     return _lib
+
+
+REF_ATLAS_CLI = ROOT / "oracle" / "_ref" / "ice_precompute_atlas_cpu"
+
+
+def atlas_rule(st: AtlasSettings, i: int, j: int) -> np.ndarray:
+    """Oracle area rule of node (i, j): rows (x, y, z, face, weight)."""
+    n = lib().orc_atlas_rule(C.byref(st), i, j, None, 0)
+    assert n > 0, n
+    out = np.zeros((n, 5), np.float64)
+    assert lib().orc_atlas_rule(C.byref(st), i, j, out.ctypes.data, n) == n
+    return out
+
+
+def atlas_node(st: AtlasSettings, i: int, j: int):
+    """Oracle cell masses [nt, np] (mm^2) and report of node (i, j)."""
+    mass = np.zeros(st.theta_count * st.phi_count, np.float64)
+    rep = AtlasReport()
+    rc = lib().orc_atlas_node(C.byref(st), i, j, mass.ctypes.data, C.byref(rep))
+    assert rc == 0, rc
+    return mass.reshape(st.theta_count, st.phi_count), rep
 
 
 def ref_available() -> bool:

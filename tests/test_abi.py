@@ -40,6 +40,10 @@ def test_abi_version_and_struct_sizes():
     assert lib.iph_struct_size(3) == ctypes.sizeof(icephase.CdfPolicy)
     assert lib.iph_struct_size(4) == ctypes.sizeof(icephase.WaterState)
     assert lib.iph_struct_size(5) == ctypes.sizeof(icephase.Area)
+    assert lib.iph_struct_size(6) == ctypes.sizeof(icephase.IceState) == 48
+    assert lib.iph_struct_size(7) == ctypes.sizeof(icephase.AtlasSettings)
+    assert lib.iph_struct_size(8) == ctypes.sizeof(icephase.AtlasNode)
+    assert lib.iph_struct_size(9) == 0
 
 
 def test_u_edges_equal_oracle(oracle):
