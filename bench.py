@@ -26,7 +26,8 @@ autotuned schedule); --config surrogate runs the surrogate leg alone;
 --config dataset times the in-process dataset driver against one
 rainbow_generate process per condition (records written to disk); --config
 atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
-atlas tool.
+atlas tool; --config compose times the orientation composition of that atlas
+(records/s) beside the checker's NumPy sum.
 
 Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
 """
@@ -524,6 +525,94 @@ def run_atlas(args):
     return out
 
 
+def run_compose(args):
+    """Orientation composition of the preview atlas (the atlas workload above,
+    traced in memory) with the reference's preview composition config
+    (hex_atlas/configs/spectral_preview.json: azimuth 72 x tilt 12 x spin 24,
+    Gaussian 3 deg 0.9 + isotropic 0.1) at inclinations 0, 30, 60, 90 deg: one
+    record each (host coupling weights, device sum, device CDF). The checker's
+    NumPy sum (oracle/compose_ref.py, the reference's own operation order) is
+    timed beside it on the 30 deg weights, on one core."""
+    import sys as _sys
+    from ice_crystal_phase_amd import compose as cm
+    from ice_crystal_phase_amd import icephase
+    n550 = icephase.ice_index(550.0).n
+    s = icephase.atlas_settings(0.1, 0.2, n550, 45, 15, 180, 360, 2, wavelength_nm=550.0)
+    config = {"quadrature": {"azimuth": 72, "tilt": 12, "spin": 24},
+              "components": [{"kind": "horizontal_gaussian", "sigma_deg": 3, "fraction": 0.9},
+                             {"kind": "isotropic", "fraction": 0.1}]}
+    rot, wts, meta = cm.orientation_rule(config)
+    out = {}
+    with icephase.Context(0) as ctx:
+        mass, nodes = icephase.atlas_trace(ctx, s)
+        atlas = types_atlas(mass.reshape(46, 16, 180, 360), nodes, s)
+        t0 = time.perf_counter()
+        dev = icephase.AtlasMass(ctx, atlas.mass)
+        upload = time.perf_counter() - t0
+        timings, weights30 = [], None
+        for alpha in (0.0, 30.0, 60.0, 90.0):
+            k = cm.incidents({"inclinations_deg": [alpha]})[0][0]
+            t0 = time.perf_counter()
+            w, area, _, info = cm.coupling_for_rule(atlas, rot, wts, k, meta)
+            t1 = time.perf_counter()
+            composed = dev.compose(w)
+            t2 = time.perf_counter()
+            cm.make_cdf(ctx, composed, ip_u_edges(180))
+            t3 = time.perf_counter()
+            timings.append({"inclination_deg": alpha, "coupling_s": t1 - t0, "compose_s": t2 - t1,
+                            "cdf_s": t3 - t2, "terms": int((w > 0).sum()), "method": info["method"]})
+            if alpha == 30.0:
+                weights30 = w
+            print(f"compose: alpha {alpha} terms {timings[-1]['terms']} device {t2 - t1:.4f} s", file=_sys.stderr,
+                  flush=True)
+        dev.close()
+    device_s = sum(t["compose_s"] + t["cdf_s"] for t in timings)
+    total_s = sum(t["coupling_s"] + t["compose_s"] + t["cdf_s"] for t in timings)
+    out.update({
+        "metric": "composed records/s on 1 MI355X (atlas orientation composition, device sum + CDF)",
+        "value": len(timings) / total_s, "unit": "records/s", "n_gpus": 1, "steps": len(timings), "warmup": 0,
+        "ms_per_step": total_s / len(timings) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "preview atlas 46x16 nodes x 180x360 cells; orientation 72x12x24, Gaussian 3 deg "
+                               "0.9 + isotropic 0.1; inclinations 0/30/60/90 deg"},
+        "device_records_per_s": len(timings) / device_s, "atlas_upload_s": upload, "per_record": timings,
+    })
+    if not args.no_cpu_baseline:
+        _sys.path.insert(0, str(ROOT / "oracle"))
+        from compose_ref import compose_mass_ref
+        # bounded sample of the checker sum: the first node rows of the 30 deg weights
+        sub = np.zeros_like(weights30)
+        terms = np.argwhere(weights30 > 0)
+        take = terms[: max(1, len(terms) // 40)]
+        for i, j, p, q in take:
+            sub[i, j, p, q] = weights30[i, j, p, q]
+        t0 = time.perf_counter()
+        compose_mass_ref(atlas.mass, sub)
+        dt = time.perf_counter() - t0
+        per_record = dt * len(terms) / len(take)
+        out["cpu_baseline"] = {"value": 1.0 / per_record, "unit": "records/s", "cores": 1, "kind": "port",
+                               "sample": f"NumPy fixed-order sum (the reference's compose_mass loop) over "
+                                         f"{len(take)} of the {len(terms)} terms of the 30 deg record in "
+                                         f"{dt:.1f} s, scaled to the whole record"}
+        out["gpu_over_cpu"] = (1.0 / (timings[1]["compose_s"])) / out["cpu_baseline"]["value"]
+    return out
+
+
+def ip_u_edges(nt):
+    from ice_crystal_phase_amd import icephase
+    return icephase.u_edges(nt)
+
+
+def types_atlas(mass, nodes, s):
+    """In-memory atlas with the fields the coupling reads (the directory reader
+    needs files)."""
+    import types
+    area = np.array([n.projected_area_mm2 for n in nodes]).reshape(mass.shape[:2])
+    residual = np.array([n.unresolved_mm2 for n in nodes]).reshape(mass.shape[:2])
+    return types.SimpleNamespace(it=mass.shape[0] - 1, ip=mass.shape[1] - 1, nt=mass.shape[2], np=mass.shape[3],
+                                 mass=mass, area=area, residual=residual, radius=s.radius_mm, length=s.length_mm)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -537,7 +626,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
-    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas"), default="drops")
+    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas", "compose"),
+                    default="drops")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy (torch) 4x: the FETCH_SIZE/WRITE_SIZE scale reference")
@@ -573,6 +663,9 @@ def main() -> int:
         return 0
     if args.config == "atlas":
         print(json.dumps(run_atlas(args)))
+        return 0
+    if args.config == "compose":
+        print(json.dumps(run_compose(args)))
         return 0
     if args.config == "surrogate":
         with ip.Context(0) as ctx:

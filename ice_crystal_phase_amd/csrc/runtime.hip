@@ -216,6 +216,8 @@ void iph_ctx_destroy(iph_ctx* ctx)
     for (iph_batch* b : live) iph_batch_destroy(b);
     const std::vector<iph_sur*> sur_live = ctx->surrogates;
     for (iph_sur* s : sur_live) iph_sur_destroy(s);
+    const std::vector<iph_atlas_mass*> atlas_live = ctx->atlases;
+    for (iph_atlas_mass* m : atlas_live) iph_atlas_mass_destroy(m);
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);

@@ -16,6 +16,7 @@ struct iph_ctx {
     std::string err;
     std::vector<iph_batch*> batches;  // live batches; destroyed with the context
     std::vector<iph_sur*> surrogates; // live surrogates; destroyed with the context
+    std::vector<iph_atlas_mass*> atlases;  // live device atlas masses; destroyed with the context
 };
 
 namespace icephase {

@@ -160,6 +160,10 @@ def lib() -> C.CDLL:
     L.iph_atlas_node_rays.argtypes = [P(AtlasSettings), u32, u32, C.c_void_p, u64, P(u64), P(AtlasNode)]
     L.iph_atlas_trace.argtypes = [C.c_void_p, P(AtlasSettings), u32, u32, C.c_void_p, C.c_void_p]
     L.iph_write_npy.argtypes = [C.c_char_p, P(u64), u32, C.c_void_p]
+    L.iph_atlas_mass_create.argtypes = [C.c_void_p, C.c_void_p, u32, u32, u32, u32, P(C.c_void_p)]
+    L.iph_atlas_mass_destroy.argtypes = [C.c_void_p]
+    L.iph_atlas_mass_destroy.restype = None
+    L.iph_atlas_compose.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     for which, (name, st) in enumerate((("Record", Record), ("HistStats", HistStats), ("CdfReport", CdfReport),
                                         ("CdfPolicy", CdfPolicy), ("WaterState", WaterState), ("Area", Area),
                                         ("IceState", IceState), ("AtlasSettings", AtlasSettings),
@@ -188,7 +192,8 @@ EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_des
             "iph_sur_destroy", "iph_sur_param_count", "iph_sur_get_params", "iph_sur_set_params",
             "iph_sur_set_record", "iph_sur_set_target", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",
             "iph_sur_compare", "iph_sur_records", "iph_sur_timing", "iph_ice_index", "iph_atlas_node_rays",
-            "iph_atlas_trace", "iph_write_npy")
+            "iph_atlas_trace", "iph_write_npy", "iph_atlas_mass_create", "iph_atlas_mass_destroy",
+            "iph_atlas_compose")
 
 
 def u_edges(nt: int) -> np.ndarray:
@@ -393,6 +398,46 @@ def atlas_trace(ctx: "Context", s: AtlasSettings, first: int = 0, count: int | N
     nodes = (AtlasNode * count)()
     _check(lib().iph_atlas_trace(ctx.ptr, C.byref(s), first, count, mass.ctypes.data, nodes), ctx.ptr)
     return mass, list(nodes)
+
+
+class AtlasMass:
+    """Device copy of an atlas mass array [ni, nj, nt, np] for composition."""
+
+    def __init__(self, ctx: "Context", mass: np.ndarray):
+        m = np.ascontiguousarray(mass, np.float64)
+        if m.ndim != 4:
+            raise ValueError("atlas mass must be [incident_theta, incident_phi, theta, phi]")
+        self.ctx, self.shape = ctx, m.shape
+        self._p = C.c_void_p()
+        _check(lib().iph_atlas_mass_create(ctx.ptr, m.ctypes.data, *m.shape, C.byref(self._p)), ctx.ptr)
+        ctx._batches.add(self)
+
+    def compose(self, weights: np.ndarray) -> np.ndarray:
+        """Composed cell mass [nt, np] of coupling weights [ni, nj, 2, np]."""
+        ni, nj, nt, np_ = self.shape
+        w = np.ascontiguousarray(weights, np.float64)
+        if w.shape != (ni, nj, 2, np_):
+            raise ValueError(f"weights must have shape {(ni, nj, 2, np_)}")
+        out = np.zeros((nt, np_), np.float64)
+        _check(lib().iph_atlas_compose(self._p, w.ctypes.data, out.ctypes.data), self.ctx.ptr)
+        return out
+
+    def close(self) -> None:
+        if self._p and self.ctx.ptr:
+            lib().iph_atlas_mass_destroy(self._p)
+        self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass(eq=False)

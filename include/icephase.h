@@ -358,6 +358,21 @@ int iph_atlas_node_rays(const iph_atlas_settings* s, uint32_t i, uint32_t j, dou
 int iph_atlas_trace(iph_ctx* ctx, const iph_atlas_settings* s, uint32_t first, uint32_t count, double* mass_mm2,
                     iph_atlas_node* nodes);
 
+/* Device-resident copy of an atlas mass array [ni][nj][nt][np] (mm^2; ni, nj
+ * incident nodes, phi_count <= 8192), for composing many conditions from one
+ * upload. Destroyed with its context if not before. */
+typedef struct iph_atlas_mass iph_atlas_mass;
+int iph_atlas_mass_create(iph_ctx* ctx, const double* mass, uint32_t ni, uint32_t nj, uint32_t nt, uint32_t np,
+                          iph_atlas_mass** out);
+void iph_atlas_mass_destroy(iph_atlas_mass* m);
+/* Replaces compose_mass (hex_atlas/python/atlas_math.py:371-386): out [nt][np]
+ * = sum over nodes (i, j) ascending, parity 0 then 1, shift s ascending of
+ * weights[i][j][parity][s] (> 0 only) times the node's cell block shifted by s
+ * azimuth cells (parity 1: azimuth-reflected first, cell q -> np-1-q), in that
+ * order with the product rounded before each addition: bitwise the reference's
+ * sum. weights: ni x nj x 2 x np, finite and nonnegative. */
+int iph_atlas_compose(iph_atlas_mass* m, const double* weights, double* out);
+
 /* NPY v1.0 '<f8' C-order file of the given shape (src/npy_writer.cpp:18-95). */
 int iph_write_npy(const char* path, const uint64_t* shape, uint32_t ndim, const double* data);
 

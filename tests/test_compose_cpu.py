@@ -1,0 +1,188 @@
+"""Host side of the atlas composition (no device): D6h folding, the tilt
+quadrature, the orientation rule, incident interpolation and the coupling
+weights, after the reference's tests (hex_atlas/tests/test_compose.py:26-98,
+test_polar_azimuth.py). The device sum and CDF are tested in
+test_gpu_compose.py against oracle/compose_ref.py and closed forms."""
+from __future__ import annotations
+
+import math
+import sys
+from pathlib import Path
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from ice_crystal_phase_amd import compose as cm
+
+sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "oracle"))
+from compose_ref import compose_mass_ref  # noqa: E402
+
+
+def fixture():
+    """Atlas stand-in: node masses = node area x one fixed pattern."""
+    it, ip_, nt, np_ = 6, 3, 4, 12
+    area = np.zeros((it + 1, ip_ + 1))
+    mass = np.zeros((it + 1, ip_ + 1, nt, np_))
+    pattern = np.arange(1, nt * np_ + 1, dtype=float).reshape(nt, np_)
+    pattern /= pattern.sum()
+    for i in range(it + 1):
+        for j in range(ip_ + 1):
+            area[i, j] = cm.projected_area(.1, .2, cm.canonical_frame(i * math.pi / 2 / it, j * math.pi / 6 / ip_)[:, 2])
+            mass[i, j] = area[i, j] * pattern
+    return SimpleNamespace(it=it, ip=ip_, nt=nt, np=np_, area=area, mass=mass, residual=np.zeros_like(area),
+                           radius=.1, length=.2)
+
+
+def test_fold_full_sphere():
+    for theta in np.linspace(0, math.pi, 21):
+        for phi in np.linspace(0, 2 * math.pi, 43):
+            k = cm.canonical_frame(theta, phi)[:, 2]
+            t, p, q = cm.fold_direction(k)
+            assert 0 <= t <= math.pi / 2 and 0 <= p <= math.pi / 6
+            np.testing.assert_allclose(q @ cm.canonical_frame(t, p)[:, 2], k, atol=2e-14)
+            np.testing.assert_allclose(q.T @ q, np.eye(3), atol=2e-14)
+
+
+def test_fold_uses_reflections_on_the_mirrored_half():
+    # azimuth 40 deg lies past the sector's mirror line (30 deg): an improper operation
+    _, p, q = cm.fold_direction(cm.canonical_frame(1.0, math.radians(40))[:, 2])
+    assert np.linalg.det(q) < 0 and abs(p - math.radians(20)) < 1e-14
+    _, p, q = cm.fold_direction(cm.canonical_frame(1.0, math.radians(70))[:, 2])
+    assert np.linalg.det(q) > 0 and abs(p - math.radians(10)) < 1e-14
+
+
+def test_gaussian_cdf_nodes():
+    for sigma in (.1, 3, 10, 90):
+        n = 16
+        b = cm.gaussian_tilt_nodes(n, sigma)
+        s = math.radians(sigma)
+        e = math.erf(math.pi / (2 * math.sqrt(2) * s))
+        c = np.array([(1 + math.erf((x - math.pi / 2) / (math.sqrt(2) * s)) / e) / 2 for x in b])
+        np.testing.assert_allclose(c, (np.arange(n) + .5) / n, atol=6e-14, rtol=0)
+        np.testing.assert_allclose(b + b[::-1], math.pi, atol=2e-15)
+    np.testing.assert_array_equal(cm.gaussian_tilt_nodes(4, 0), [math.pi / 2] * 4)
+
+
+def test_rule_no_rng_and_weights():
+    config = {"quadrature": {"azimuth": 4, "tilt": 8, "spin": 3}, "components": [
+        {"kind": "horizontal_gaussian", "sigma_deg": 3, "fraction": .9}, {"kind": "isotropic", "fraction": .1}]}
+    a, w, m = cm.orientation_rule(config)
+    b, v, _ = cm.orientation_rule(config)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(w, v)
+    assert m["randomness"] is False
+    assert abs(math.fsum(w[:96]) - .9) < 1e-15 and abs(math.fsum(w[96:]) - .1) < 1e-15
+    # the first pose: Ry(2 pi/8) Rz(beta_0) Ry(pi/18)
+    beta0 = cm.gaussian_tilt_nodes(8, 3)[0]
+    np.testing.assert_allclose(a[0], cm.ry(math.pi / 4) @ cm.rz(beta0) @ cm.ry(math.pi / 18), atol=1e-15)
+
+
+def test_bad_rule():
+    for sigma in (-1, float("nan")):
+        with pytest.raises(ValueError):
+            cm.gaussian_tilt_nodes(10, sigma)
+    with pytest.raises(ValueError):
+        cm.orientation_rule({"quadrature": {"azimuth": 2, "tilt": 2, "spin": 2},
+                             "components": [{"kind": "isotropic", "fraction": .5}]})
+    with pytest.raises(ValueError):
+        cm.orientation_rule({"quadrature": {"azimuth": 2, "tilt": 2, "spin": 2},
+                             "components": [{"kind": "fixed", "fraction": 1, "quaternion_wxyz": [1, 0, 0, .1]}]})
+
+
+def test_interpolation_endpoints_and_partition():
+    for i in range(7):
+        for j in range(4):
+            terms = cm.interpolation(i * math.pi / 12, j * math.pi / 18, 6, 3)
+            assert abs(sum(w for _, _, w in terms) - 1) < 1e-15
+            assert abs(sum(w for ii, jj, w in terms if (ii, jj) == (i, j)) - 1) < 1e-15
+    for t in np.linspace(0, math.pi / 2, 21):
+        terms = cm.interpolation(t, .14, 6, 3)
+        assert all(w >= 0 for _, _, w in terms) and abs(sum(w for _, _, w in terms) - 1) < 1e-15
+
+
+def test_no_incident_extrapolation():
+    with pytest.raises(ValueError):
+        cm.interpolation(-.01, .1, 6, 3)
+    with pytest.raises(ValueError):
+        cm.interpolation(.1, .6, 6, 3)
+
+
+@pytest.mark.parametrize("reflected", [False, True])
+@pytest.mark.parametrize("shift", [0, 1, 2.3, -.7, 11.999])
+def test_pure_shift_pose_couples_two_neighbouring_shifts(reflected, shift):
+    """A pose that maps node (2, 1)'s frame to the output frame turned by
+    `shift` azimuth cells puts its whole weight on node (2, 1) at shifts
+    floor(shift) and floor(shift) + 1, split linearly (test_compose.py:61-77)."""
+    a = fixture()
+    f = cm.canonical_frame(math.pi / 6, math.pi / 18)
+    fw = cm.frame([1, 0, 0])
+    q = np.diag([1., 1., -1.]) if reflected else np.eye(3)
+    d = 2 * math.pi * shift / a.np
+    h = np.array([[math.cos(d), -math.sin(d), 0], [math.sin(d), math.cos(d), 0], [0, 0, 1.]])
+    if reflected:
+        h = h @ np.diag([1., -1., 1.])
+    r = fw @ h @ f.T @ q.T
+    w, expected, _ = cm.coupling(a, np.array([r]), np.ones(1), [1, 0, 0])
+    lower = math.floor(shift)
+    frac = shift - lower
+    parity = int(reflected)
+    # all but rounding-sized interpolation weight on node (2, 1) with the pose's parity
+    assert w[2, 1, parity].sum() > 1 - 1e-12 and w.sum() - w[2, 1, parity].sum() < 1e-12
+    assert abs(w[2, 1, parity, lower % 12] - (1 - frac)) < 1e-12
+    if frac > 1e-12:
+        assert abs(w[2, 1, parity, (lower + 1) % 12] - frac) < 1e-12
+    # the composed mass of those weights is the shifted node block (checker sum)
+    out = compose_mass_ref(a.mass, w)
+    original = a.mass[2, 1][:, ::-1] if reflected else a.mass[2, 1]
+    want = (1 - frac) * np.roll(original, lower, axis=1) + frac * np.roll(original, lower + 1, axis=1)
+    np.testing.assert_allclose(out, want, atol=3e-17, rtol=1e-12)
+    assert abs(out.sum() - expected) < 1e-15
+
+
+def test_area_weighted_mixture_weights():
+    a = fixture()
+    rot = np.array([np.eye(3), cm.rz(math.pi / 2)])
+    w, area, _ = cm.coupling(a, rot, np.array([.25, .75]), [1, 0, 0])
+    exact = .25 * cm.projected_area(.1, .2, [1, 0, 0]) + .75 * cm.projected_area(.1, .2, [0, -1, 0])
+    assert abs(area - exact) < 1e-15
+    assert abs(compose_mass_ref(a.mass, w).sum() - exact) < 1e-15
+
+
+def test_vertical_incidence_integrates_the_azimuth_orbit():
+    """k = -y: every isotropic/Gaussian contribution is spread evenly over all
+    shifts, so the composed mass is uniform in azimuth (test_polar_azimuth.py)."""
+    a = fixture()
+    config = {"quadrature": {"azimuth": 8, "tilt": 4, "spin": 3}, "components": [
+        {"kind": "horizontal_gaussian", "sigma_deg": 5, "fraction": .5}, {"kind": "isotropic", "fraction": .5}]}
+    r, w, meta = cm.orientation_rule(config)
+    c, area, _, info = cm.coupling_for_rule(a, r, w, [0, -1, 0], meta)
+    assert info["method"] == "analytic_uniform_world_azimuth_at_vertical_incidence"
+    assert info["effective_pose_evaluations"] == 2 * 4 * 3
+    assert np.all(c == c[..., :1])
+    out = compose_mass_ref(a.mass, c)
+    np.testing.assert_allclose(out, np.repeat(out.mean(axis=1, keepdims=True), a.np, axis=1), rtol=1e-13)
+    plain, area2, _ = cm.coupling(a, r, w, [0, -1, 0])
+    assert abs(area / area2 - 1) < 1e-13  # the orbit integral keeps the projected area
+    _, _, _, info = cm.coupling_for_rule(a, r, w, [0, -1, 1e-9], meta)
+    assert info["method"] == "original_fixed_orientation_quadrature"
+
+
+def test_residual_not_renormalized_away():
+    a = fixture()
+    a.mass *= .99
+    a.residual = .01 * a.area
+    w, area, _ = cm.coupling(a, np.array([cm.ry(.213) @ cm.rz(.7)]), np.ones(1), [1, -.4, .3])
+    assert abs(compose_mass_ref(a.mass, w).sum() / area - .99) < 1e-13
+    assert abs(float(np.sum(np.sum(w, axis=(2, 3)) * a.residual)) / area - .01) < 1e-13
+
+
+def test_incidents():
+    got = cm.incidents({"inclinations_deg": [0, 35, 90, -90]})
+    np.testing.assert_allclose(got[1][0], [math.cos(math.radians(35)), -math.sin(math.radians(35)), 0], atol=1e-16)
+    np.testing.assert_array_equal(got[2][0], [0, -1, 0])
+    np.testing.assert_array_equal(got[3][0], [0, 1, 0])
+    with pytest.raises(ValueError):
+        cm.incidents({"inclinations_deg": [91]})
+    with pytest.raises(ValueError):
+        cm.incidents({"inclinations_deg": [0], "directions": [[1, 0, 0]]})
