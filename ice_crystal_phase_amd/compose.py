@@ -436,6 +436,10 @@ def make_cdf(ctx: ip.Context, mass: np.ndarray, edges: np.ndarray, max_l1: float
     phi, theta, rep, rc = ctx.build_cdf(mass, nt, np_, max_l1=max_l1, max_lost=max_lost, check=False)
     if rc != 0:
         raise ValueError(f"CDF audit rejected l1={rep.l1_error} lost={rep.lost_mass} g={rep.g_stored}")
+    # an empty azimuth column's conditional CDF is the edge list itself: the
+    # caller's (the device fills the standard u edges, equal for an atlas grid)
+    empty = ~(mass > 0).any(axis=0)
+    theta[empty] = edges
     return phi, theta, {"cdf_mass": rep.cdf_mass, "cdf_l1_mass_error": rep.l1_error,
                         "cdf_lost_probability_mass": rep.lost_mass,
                         "cdf_maximum_cell_mass_error": rep.max_cell_error, "g": rep.g_stored,
