@@ -108,7 +108,7 @@ def gaussian_tilt_nodes(count: int, sigma_deg: float) -> np.ndarray:
     if sigma == 0:
         return np.full(count, PI / 2)
     scale = math.sqrt(2) * sigma
-    norm = math.erf(PI / (2 * scale))
+    norm = math.erf(PI / (2 * math.sqrt(2) * sigma))
     if not norm > 0:
         raise ValueError("Gaussian normalization is not representable")
     nodes = []
