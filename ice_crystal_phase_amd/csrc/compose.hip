@@ -11,10 +11,14 @@
 // the same order with the same roundings (product rounded, then added; no
 // fused multiply-add), so the result is bitwise the reference's.
 //
-// One wave per (row t, 64-wide azimuth chunk): the wave stages the node's row
-// (reversed for parity 1) in LDS, then every lane accumulates its output cell
-// over the group's shifts, reading the wrapped row from LDS. The atlas mass
-// stays resident on the device across conditions and wavelengths.
+// One wave per (row t, 64-wide azimuth chunk), about one wave per SIMD for a
+// 180 x 360 grid: the wave stages the node's row (reversed for parity 1) in
+// LDS (np doubles, dynamic), then every lane accumulates its output cell over
+// the group's terms. The sum of a cell is one serial chain of adds (its order
+// is the result), so the terms are read eight at a time (one scalar load of
+// 8 x 16 bytes) and their LDS reads and products issued ahead of the adds.
+// The atlas mass stays resident on the device across conditions and
+// wavelengths.
 #include "runtime_internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -26,23 +30,28 @@
 
 namespace {
 
-constexpr uint32_t kMaxPhi = 8192;  // LDS row: 64 KiB of doubles
+constexpr uint32_t kMaxPhi = 8192;  // LDS row: at most 64 KiB of doubles
 constexpr uint32_t kLanes = 64;
+constexpr uint32_t kUnroll = 8;
 
 struct Group {
     uint64_t cell;    // offset of the node's [nt][np] block
     uint32_t parity;  // 1: azimuth-reflected block
-    uint32_t first;   // first term in shift/weight
+    uint32_t first;   // first term
     uint32_t count;
     uint32_t pad;
 };
 
+struct Term {
+    uint32_t shift, pad;
+    double weight;
+};
+
 __global__ void __launch_bounds__(64) icephase_compose(const double* __restrict__ mass, uint32_t np,
                                                        const Group* __restrict__ groups, uint32_t ngroups,
-                                                       const uint32_t* __restrict__ shift,
-                                                       const double* __restrict__ weight, double* __restrict__ out)
+                                                       const Term* __restrict__ terms, double* __restrict__ out)
 {
-    __shared__ double row[kMaxPhi];
+    extern __shared__ double row[];
     const uint32_t t = blockIdx.y;
     const uint32_t p = blockIdx.x * kLanes + threadIdx.x;
     double acc = 0.0;
@@ -53,11 +62,23 @@ __global__ void __launch_bounds__(64) icephase_compose(const double* __restrict_
         for (uint32_t q = threadIdx.x; q < np; q += kLanes) row[q] = src[G.parity ? np - 1 - q : q];
         __syncthreads();
         if (p < np) {
-            for (uint32_t k = G.first; k < G.first + G.count; ++k) {
-                const uint32_t s = shift[k];
-                const double w = weight[k];
+            const Term* T = terms + G.first;
+            uint32_t k = 0;
+            for (; k + kUnroll <= G.count; k += kUnroll) {
+                double prod[kUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) {
+                    const uint32_t s = T[k + u].shift;
+                    const uint32_t q = p >= s ? p - s : p + np - s;
+                    prod[u] = T[k + u].weight * row[q];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) acc = acc + prod[u];
+            }
+            for (; k < G.count; ++k) {
+                const uint32_t s = T[k].shift;
                 const uint32_t q = p >= s ? p - s : p + np - s;
-                acc = acc + w * row[q];
+                acc = acc + T[k].weight * row[q];
             }
         }
     }
@@ -122,19 +143,17 @@ int iph_atlas_compose(iph_atlas_mass* m, const double* weights, double* out)
     const uint32_t np = m->np, nt = m->nt;
     const size_t cells = (size_t)nt * np;
     std::vector<Group> groups;
-    std::vector<uint32_t> shifts;
-    std::vector<double> values;
+    std::vector<Term> terms;
     for (uint32_t i = 0; i < m->ni; ++i)
         for (uint32_t j = 0; j < m->nj; ++j)
             for (uint32_t parity = 0; parity < 2; ++parity) {
                 const double* w = weights + (((size_t)i * m->nj + j) * 2 + parity) * np;
-                Group g{((size_t)i * m->nj + j) * cells, parity, (uint32_t)shifts.size(), 0, 0};
+                Group g{((size_t)i * m->nj + j) * cells, parity, (uint32_t)terms.size(), 0, 0};
                 for (uint32_t s = 0; s < np; ++s) {
                     if (!std::isfinite(w[s]) || w[s] < 0)
                         return fail(ctx, IPH_E_INVALID, "Composition weights must be finite and nonnegative.");
                     if (w[s] > 0) {
-                        shifts.push_back(s);
-                        values.push_back(w[s]);
+                        terms.push_back(Term{s, 0, w[s]});
                         ++g.count;
                     }
                 }
@@ -143,19 +162,17 @@ int iph_atlas_compose(iph_atlas_mass* m, const double* weights, double* out)
     if (groups.empty()) return fail(ctx, IPH_E_INVALID, "Composition weights are all zero.");
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     DevBuf<Group> d_groups;
-    DevBuf<uint32_t> d_shift;
-    DevBuf<double> d_weight, d_out;
+    DevBuf<Term> d_terms;
+    DevBuf<double> d_out;
     IPH_HIP(ctx, d_groups.alloc(groups.size()));
-    IPH_HIP(ctx, d_shift.alloc(shifts.size()));
-    IPH_HIP(ctx, d_weight.alloc(values.size()));
+    IPH_HIP(ctx, d_terms.alloc(terms.size()));
     IPH_HIP(ctx, d_out.alloc(cells));
     hipStream_t s = ctx->stream;
     IPH_HIP(ctx, hipMemcpyAsync(d_groups.p, groups.data(), groups.size() * sizeof(Group), hipMemcpyHostToDevice, s));
-    IPH_HIP(ctx, hipMemcpyAsync(d_shift.p, shifts.data(), shifts.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    IPH_HIP(ctx, hipMemcpyAsync(d_weight.p, values.data(), values.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    IPH_HIP(ctx, hipMemcpyAsync(d_terms.p, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice, s));
     const dim3 grid((np + kLanes - 1) / kLanes, nt);
-    hipLaunchKernelGGL(icephase_compose, grid, dim3(kLanes), 0, s, m->mass.p, np, d_groups.p, (uint32_t)groups.size(),
-                       d_shift.p, d_weight.p, d_out.p);
+    hipLaunchKernelGGL(icephase_compose, grid, dim3(kLanes), np * sizeof(double), s, m->mass.p, np, d_groups.p,
+                       (uint32_t)groups.size(), d_terms.p, d_out.p);
     IPH_HIP(ctx, hipGetLastError());
     IPH_HIP(ctx, hipMemcpyAsync(out, d_out.p, cells * sizeof(double), hipMemcpyDeviceToHost, s));
     IPH_HIP(ctx, hipStreamSynchronize(s));
