@@ -12,6 +12,7 @@ from pathlib import Path
 
 import pytest
 
+from ice_crystal_phase_amd import analysis
 from ice_crystal_phase_amd import dataset as ds
 from ice_crystal_phase_amd.record import PhaseRecord
 
@@ -71,6 +72,15 @@ def test_driver_records_equal_command_line_records(tmp_path, sphere):
             assert mine == theirs, {k: (mine[k], theirs[k]) for k in mine if mine[k] != theirs.get(k)}
             with PhaseRecord(rec) as r:
                 assert r.nt == 90 and r.np == 180
+            if not sphere and ii == 1 and ll == 0:
+                # the record comparison tool sees no difference between the two (at normal
+                # incidence: elsewhere the FP32 query_frame_axis, written unnormalized as the
+                # reference writes it, is refused by the tool's 2e-12 frame check, as there)
+                report = tmp_path / f"compare_{sphere}"
+                assert analysis.main(["compare", str(rec), str(alone), "--out", str(report), "--no-plots",
+                                      "--allow-orientation-plan-difference"]) == 0
+                cmp = json.loads((report / "comparison.json").read_text())
+                assert cmp["tv"] == 0 and cmp["g_a_minus_b"] == 0
             receipt = json.loads((out / f"receipts/i{ii:04d}_w{ll:04d}.json").read_text())
             assert receipt["record"] == ds.verify_record(rec, plan, ii, ll)
     text = _driver(cfg, "--execute", "--resume")
