@@ -11,7 +11,8 @@ the analytic uniform-azimuth orbit at vertical incidence, and the same output
 tree (records/iNNNN/wNNNN with the record contract, orientation model files,
 generation_plan.json and batch_plan.json, staged in <out>.part).
 
-The coupling weights are assembled on the host (O(poses) work). The two
+The coupling weights are assembled on the host, all poses at once as arrays
+(the pose-by-pose rule is kept as coupling_scalar, its checker). The two
 array operations run on the device through libicephase.so: the fixed-order
 shifted-block sum over every atlas node (iph_atlas_compose, bitwise the
 reference's order and roundings) with the atlas resident in HBM across
@@ -324,6 +325,10 @@ class Atlas:
 
 
 # ------------------------------------------------------------------ coupling
+_NORMALS = np.array([[1, 0, 0], [0.5, 0, math.sqrt(3) / 2], [-0.5, 0, math.sqrt(3) / 2], [-1, 0, 0],
+                     [-0.5, 0, -math.sqrt(3) / 2], [0.5, 0, -math.sqrt(3) / 2], [0, 1, 0], [0, -1, 0]])
+
+
 def coupling(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
              ki: Any) -> tuple[np.ndarray, float, np.ndarray]:
     """Convolution weights [node_theta, node_phi, parity, shift] that multiply
@@ -331,7 +336,98 @@ def coupling(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
     folded into the fundamental domain, the A-normalized node masses are
     interpolated and scaled by the exact projected area A(s), and the output
     azimuth offset of the folded frame is split between the two neighbouring
-    whole-cell shifts (atlas_math.py:273-306)."""
+    whole-cell shifts (atlas_math.py:273-306).
+
+    All poses are processed as arrays; the weights are accumulated pose by
+    pose in the scalar rule's order (np.add.at applies its terms in sequence),
+    so the result equals coupling_scalar() up to the last-bit differences of
+    array versus scalar elementary functions (checked to 1e-13)."""
+    validate_rotations(rotations, number_weights)
+    out_frame = frame(ki)
+    k = out_frame[:, 2]
+    n = len(rotations)
+    # body-frame incident directions, normalized as unit() does
+    body = np.einsum("nji,j->ni", rotations, k)
+    body = body / np.max(np.abs(body), axis=1, keepdims=True)
+    body = body / np.linalg.norm(body, axis=1, keepdims=True)
+    # D6h fold (fold_direction, vectorized)
+    theta = np.arctan2(np.hypot(body[:, 0], body[:, 2]), np.abs(body[:, 1]))
+    az = np.mod(np.arctan2(body[:, 2], body[:, 0]), 2 * PI)
+    sector = np.minimum(5, np.floor(az / (PI / 3)).astype(np.int64))
+    local = az - sector * (PI / 3)
+    mirror = local > PI / 6
+    phi = np.where(mirror, PI / 3 - local, local)
+    if np.any(phi < -64 * EPS) or np.any(phi > PI / 6 + 64 * EPS):
+        raise ValueError("Canonical azimuth outside fundamental region")
+    phi = np.minimum(PI / 6, np.maximum(0.0, phi))
+    angle = (sector + mirror.astype(np.int64)) * (PI / 3)
+    c, s_ = np.cos(-angle), np.sin(-angle)
+    sy = np.where(body[:, 1] < 0, -1.0, 1.0)
+    sz = np.where(mirror, -1.0, 1.0)
+    # q = Ry(-angle) diag(1, sy, sz)
+    q = np.zeros((n, 3, 3))
+    q[:, 0, 0], q[:, 0, 2] = c, s_ * sz
+    q[:, 1, 1] = sy
+    q[:, 2, 0], q[:, 2, 2] = -s_, c * sz
+    st, ct, sp, cp = np.sin(theta), np.cos(theta), np.sin(phi), np.cos(phi)
+    kc = np.stack([st * cp, ct, st * sp], axis=1)
+    ec = np.stack([ct * cp, -st, ct * sp], axis=1)
+    if np.max(np.abs(np.einsum("nij,nj->ni", q, kc) - body)) > 2e-12:
+        raise ValueError("Symmetry unfolding is inconsistent")
+    e = np.einsum("nij,nj->ni", rotations, np.einsum("nij,nj->ni", q, ec))
+    shift = np.arctan2(e @ out_frame[:, 1], e @ out_frame[:, 0]) / (2 * PI) * atlas.np
+    nearest = np.round(shift)
+    shift = np.where(np.abs(shift - nearest) <= 64 * EPS * atlas.np, nearest, shift)
+    lower = np.floor(shift)
+    frac = shift - lower
+    lower = lower.astype(np.int64)
+    parity = (sy * sz < 0).astype(np.int64)
+    face_areas = np.array([atlas.radius * atlas.length] * 6 + [3 * math.sqrt(3) * atlas.radius * atlas.radius / 2] * 2)
+    areas = np.maximum(0, -body @ _NORMALS.T) @ face_areas  # projected_area() of every pose
+    nw = np.asarray(number_weights, dtype=np.float64)
+    expected_area = math.fsum(map(float, nw * areas))
+    # bilinear incident interpolation (interpolation(), vectorized)
+    if (np.any(~np.isfinite(theta)) or np.any(theta < -64 * EPS) or np.any(theta > PI / 2 + 64 * EPS)):
+        raise ValueError("Incident interpolation outside canonical domain")
+    x = np.minimum(float(atlas.it), np.maximum(0.0, theta / (PI / 2) * atlas.it))
+    y = np.minimum(float(atlas.ip), np.maximum(0.0, phi / (PI / 6) * atlas.ip))
+    i0 = np.minimum(atlas.it - 1, x.astype(np.int64))
+    j0 = np.minimum(atlas.ip - 1, y.astype(np.int64))
+    ax, by = x - i0, y - j0
+    w = np.zeros((atlas.it + 1, atlas.ip + 1, 2, atlas.np), dtype=np.float64)
+    # terms in the scalar order: pose, (di, dj) in (0,0), (0,1), (1,0), (1,1), then lower, lower + 1
+    idx_i, idx_j, idx_p, idx_s, vals = [], [], [], [], []
+    for di, u_ in ((0, 1 - ax), (1, ax)):
+        for dj, v_ in ((0, 1 - by), (1, by)):
+            mix = u_ * v_
+            ii, jj = i0 + di, j0 + dj
+            cval = nw * areas * mix / atlas.area[ii, jj]
+            keep = mix > 0
+            idx_i.append(np.where(keep, ii, -1))
+            idx_j.append(jj)
+            idx_p.append(parity)
+            idx_s.append(np.stack([lower % atlas.np, (lower + 1) % atlas.np], axis=1))
+            vals.append(np.stack([cval * (1 - frac), cval * frac], axis=1))
+    I = np.stack(idx_i, axis=1)          # [n, 4]
+    J = np.stack(idx_j, axis=1)
+    Pp = np.stack(idx_p, axis=1)
+    S = np.stack(idx_s, axis=1)          # [n, 4, 2]
+    V = np.stack(vals, axis=1)           # [n, 4, 2]
+    keep = (I >= 0)[:, :, None] & np.ones((1, 1, 2), bool)
+    Ib = np.broadcast_to(I[:, :, None], S.shape)[keep]
+    Jb = np.broadcast_to(J[:, :, None], S.shape)[keep]
+    Pb = np.broadcast_to(Pp[:, :, None], S.shape)[keep]
+    np.add.at(w, (Ib, Jb, Pb, S[keep]), V[keep])
+    represented = float(np.sum(np.sum(w, axis=(2, 3)) * atlas.area))
+    if abs(represented / expected_area - 1) > 1e-11 or np.any(w < 0) or not np.isfinite(w).all():
+        raise ValueError("Orientation-to-convolution area audit failed")
+    return w, expected_area, out_frame
+
+
+def coupling_scalar(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
+                    ki: Any) -> tuple[np.ndarray, float, np.ndarray]:
+    """coupling() one pose at a time with the scalar operations of
+    atlas_math.py:273-306 (the checker of the array version)."""
     validate_rotations(rotations, number_weights)
     out_frame = frame(ki)
     k = out_frame[:, 2]

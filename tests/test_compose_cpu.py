@@ -186,3 +186,20 @@ def test_incidents():
         cm.incidents({"inclinations_deg": [91]})
     with pytest.raises(ValueError):
         cm.incidents({"inclinations_deg": [0], "directions": [[1, 0, 0]]})
+
+
+@pytest.mark.parametrize("k", [[1, 0, 0], [0.5, -0.8, 0], [1, -.4, .3], [0, -1, 0]])
+def test_array_coupling_equals_scalar_rule(k):
+    """The array coupling (all poses at once, weights accumulated in the scalar
+    order) equals the pose-by-pose rule of atlas_math.py up to last-bit
+    differences of the elementary functions."""
+    a = fixture()
+    config = {"quadrature": {"azimuth": 6, "tilt": 5, "spin": 4}, "components": [
+        {"kind": "horizontal_gaussian", "sigma_deg": 7, "fraction": .6}, {"kind": "isotropic", "fraction": .3},
+        {"kind": "fixed", "fraction": .1, "quaternion_wxyz": [math.cos(.3), 0, math.sin(.3), 0]}]}
+    r, w, _ = cm.orientation_rule(config)
+    fast, area_fast, f1 = cm.coupling(a, r, w, k)
+    slow, area_slow, f2 = cm.coupling_scalar(a, r, w, k)
+    assert np.array_equal(fast > 0, slow > 0)
+    assert np.abs(fast - slow).sum() / slow.sum() < 1e-13
+    assert abs(area_fast / area_slow - 1) < 1e-14 and np.array_equal(f1, f2)
