@@ -31,6 +31,44 @@ using namespace icephase;
 
 namespace {
 
+// Cell masses of every node from its fixed-point bins (quanta x the node's mm^2
+// per quantum; the pole quanta spread evenly over the end-cap rows, forward
+// then backward) and the exact integer total of the node's binned quanta.
+__global__ void __launch_bounds__(256)
+    icephase_atlas_mass(const unsigned long long* __restrict__ bins, const unsigned long long* __restrict__ stats,
+                        const double* __restrict__ scale, uint32_t count, uint32_t nt, uint32_t np,
+                        double* __restrict__ mass, unsigned long long* __restrict__ totals)
+{
+    __shared__ unsigned long long s_part[4];
+    const uint64_t cells = (uint64_t)nt * np;
+    for (uint32_t q = blockIdx.y; q < count; q += gridDim.y) {
+        const unsigned long long* B = bins + (uint64_t)q * cells;
+        const unsigned long long* S = stats + (uint64_t)q * kStatCount;
+        const double sc = scale[q];
+        const double fwd = (double)S[kStatForward] * sc / np, bwd = (double)S[kStatBackward] * sc / np;
+        double* M = mass + (uint64_t)q * cells;
+        unsigned long long part = 0;
+        for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells;
+             c += (uint64_t)gridDim.x * blockDim.x) {
+            const unsigned long long b = B[c];
+            part += b;
+            double m = (double)b * sc;
+            if (c < np) m += fwd;
+            if (c >= cells - np) m += bwd;
+            M[c] = m;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+            if (t) atomicAdd(&totals[q], t);
+        }
+        __syncthreads();
+    }
+}
+
 constexpr double kPi = 3.141592653589793238462643383279502884;
 constexpr uint64_t kAtlasTileRays = 256;  // one workgroup's lanes per tile
 
@@ -320,11 +358,27 @@ int iph_atlas_trace(iph_ctx* ctx, const iph_atlas_settings* s, uint32_t first, u
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->cus * 8, map.total));
     IPH_HIP(ctx, launch_zero(d_recs.p, count, d_queue.p, st));
     IPH_HIP(ctx, launch_trace_area(nt + 1 > 2048, d_recs.p, map, d_queue.p + 1, blocks, 256, st));
-    std::vector<unsigned long long> bins(count * cells), stats((size_t)count * kStatCount);
-    IPH_HIP(ctx, hipMemcpyAsync(bins.data(), d_bins.p, bins.size() * sizeof(unsigned long long),
-                                hipMemcpyDeviceToHost, st));
+    // FP64 masses and exact quanta totals on the device, straight into the caller's array
+    DevBuf<double> d_scale, d_mass;
+    DevBuf<unsigned long long> d_totals;
+    IPH_HIP(ctx, d_scale.alloc(count));
+    IPH_HIP(ctx, d_mass.alloc(count * cells));
+    IPH_HIP(ctx, d_totals.alloc(count));
+    IPH_HIP(ctx, hipMemcpyAsync(d_scale.p, scale.data(), count * sizeof(double), hipMemcpyHostToDevice, st));
+    IPH_HIP(ctx, hipMemsetAsync(d_totals.p, 0, count * sizeof(unsigned long long), st));
+    {
+        const uint32_t gx = (uint32_t)std::min<uint64_t>((cells + 255) / 256, 64);
+        const uint32_t gy = std::min<uint32_t>(count, 65535);
+        hipLaunchKernelGGL(icephase_atlas_mass, dim3(gx, gy), dim3(256), 0, st, d_bins.p, d_stats.p, d_scale.p, count,
+                           nt, np, d_mass.p, d_totals.p);
+        IPH_HIP(ctx, hipGetLastError());
+    }
+    std::vector<unsigned long long> stats((size_t)count * kStatCount), totals(count);
+    IPH_HIP(ctx, hipMemcpyAsync(mass_mm2, d_mass.p, count * cells * sizeof(double), hipMemcpyDeviceToHost, st));
     IPH_HIP(ctx, hipMemcpyAsync(stats.data(), d_stats.p, stats.size() * sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, st));
+    IPH_HIP(ctx, hipMemcpyAsync(totals.data(), d_totals.p, count * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                st));
     IPH_HIP(ctx, hipStreamSynchronize(st));
 
     // readout and audits (hex_atlas/src/atlas.cpp:89-138)
@@ -352,23 +406,13 @@ int iph_atlas_trace(iph_ctx* ctx, const iph_atlas_settings* s, uint32_t first, u
         node.escaped_mm2 = (double)S[kStatEscaped] * sc;
         node.unresolved_mm2 = (double)S[kStatUnresolved] * sc;
         node.outputs = S[kStatSamples];
-        double* m = mass_mm2 + (size_t)q * cells;
-        const unsigned long long* B = bins.data() + (size_t)q * cells;
-        for (uint64_t c = 0; c < cells; ++c) m[c] = (double)B[c] * sc;
-        // pole emissions: uniform over the end-cap row
-        const double fwd = (double)S[kStatForward] * sc / np, bwd = (double)S[kStatBackward] * sc / np;
-        for (uint32_t c = 0; c < np; ++c) {
-            m[c] += fwd;
-            m[(size_t)(nt - 1) * np + c] += bwd;
-        }
-        Kahan sum;
-        for (uint64_t c = 0; c < cells; ++c) {
-            if (!std::isfinite(m[c]) || m[c] < 0) return fail(ctx, IPH_E_AUDIT, where + "Invalid atlas cell mass.");
-            sum.add(m[c]);
-        }
-        if (!(sum.v > 0) || !(node.escaped_mm2 > 0))
+        // the cell masses are nonnegative multiples of the quantum; their total
+        // is exact in integers (binned + pole quanta), which equals the escaped
+        // quanta when no emission was lost
+        const unsigned long long binned = totals[q] + S[kStatForward] + S[kStatBackward];
+        if (!(binned > 0) || !(node.escaped_mm2 > 0) || !std::isfinite(sc))
             return fail(ctx, IPH_E_AUDIT, where + "Incomplete atlas node.");
-        node.histogram_relative_error = std::fabs(sum.v / node.escaped_mm2 - 1);
+        node.histogram_relative_error = std::fabs((double)binned / (double)S[kStatEscaped] - 1);
         const double area = node.projected_area_mm2;
         if (node.histogram_relative_error > s->max_balance_error ||
             std::fabs((node.escaped_mm2 + node.unresolved_mm2) / area - 1) > s->max_balance_error ||
