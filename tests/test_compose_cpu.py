@@ -5,6 +5,7 @@ test_polar_azimuth.py). The device sum and CDF are tested in
 test_gpu_compose.py against oracle/compose_ref.py and closed forms."""
 from __future__ import annotations
 
+import json
 import math
 import sys
 from pathlib import Path
@@ -203,3 +204,77 @@ def test_array_coupling_equals_scalar_rule(k):
     assert np.array_equal(fast > 0, slow > 0)
     assert np.abs(fast - slow).sum() / slow.sum() < 1e-13
     assert abs(area_fast / area_slow - 1) < 1e-14 and np.array_equal(f1, f2)
+
+
+def _fake_atlas(path, it=2, ip_=1, nt=4, np_=8, wavelength=550.0):
+    """A well-formed atlas directory (uniform node masses) for the composer's
+    host checks; no device is involved."""
+    import json as _json
+    path.mkdir()
+    nodes, mass = [], np.zeros((it + 1, ip_ + 1, nt, np_))
+    for i in range(it + 1):
+        for j in range(ip_ + 1):
+            f = cm.canonical_frame(i * (math.pi / 2) / it, j * (math.pi / 6) / ip_)
+            area = cm.projected_area(.1, .2, f[:, 2])
+            mass[i, j] = area / (nt * np_)
+            nodes.append({"i": i, "j": j, "ki": f[:, 2].tolist(), "e0": f[:, 0].tolist(), "projected_area_mm2": area,
+                          "escaped_mm2": area, "unresolved_mm2": 0.0})
+    np.save(path / "mass_mm2.npy", mass)
+    np.save(path / "u_edges.npy", (1 - np.cos(np.linspace(0, math.pi, nt + 1))) / 2)
+    meta = {"schema": "ice.single_particle_atlas.v1", "complete": True,
+            "incident_reconstruction": "bilinear_theta_phi_in_spherical_incident_frame",
+            "symmetry": "D6h_O3_full_24_scalar_unpolarized", "incident_theta_intervals": it,
+            "incident_phi_intervals": ip_, "theta_count": nt, "phi_count": np_,
+            "geometry": {"shape": "regular_hexagonal_prism", "circumradius_mm": .1, "full_length_mm": .2},
+            "material": {"model": "constant", "wavelength_nm": wavelength, "interior_index": 1.31,
+                         "exterior_index": 1.0},
+            "transport": {"max_unresolved_fraction": 1e-10}, "nodes": nodes}
+    (path / "metadata.json").write_text(_json.dumps(meta))
+    return path
+
+
+def test_atlas_reader_audits(tmp_path):
+    a = cm.Atlas(_fake_atlas(tmp_path / "atlas"))
+    assert (a.it, a.ip, a.nt, a.np) == (2, 1, 4, 8) and a.area.shape == (3, 2)
+    a.close()
+    bad = _fake_atlas(tmp_path / "bad")
+    m = json.loads((bad / "metadata.json").read_text())
+    m["nodes"][1]["escaped_mm2"] *= 1.01
+    (bad / "metadata.json").write_text(json.dumps(m))
+    with pytest.raises(ValueError, match="audit"):
+        cm.Atlas(bad)
+    with pytest.raises(ValueError, match="Incomplete"):
+        cm.Atlas(tmp_path / "x.part")
+
+
+def test_compose_cli_failure_keeps_part_with_reason(tmp_path, monkeypatch):
+    """A failure after staging leaves <out>.part with failure.json and exit 2
+    (compose_atlas.py:221-226); here the device context is refused."""
+    atlas = _fake_atlas(tmp_path / "atlas")
+    cfg = tmp_path / "c.json"
+    cfg.write_text(json.dumps({"inclinations_deg": [0], "orientation": {
+        "quadrature": {"azimuth": 2, "tilt": 2, "spin": 2}, "components": [{"kind": "isotropic", "fraction": 1}]}}))
+
+    def no_device(device=0):
+        raise cm.ip.IcePhaseError(-2, "no device")
+
+    monkeypatch.setattr(cm.ip, "Context", no_device)
+    with pytest.raises(SystemExit) as e:
+        cm.main(["--atlas", str(atlas), "--config", str(cfg), "--out", str(tmp_path / "out")])
+    assert e.value.code == 2
+    assert not (tmp_path / "out").exists()
+    assert json.loads((tmp_path / "out.part" / "failure.json").read_text())["complete"] is False
+    assert (tmp_path / "out.part" / "orientation_model.json").exists()
+    # the staging directory is never adopted or removed
+    with pytest.raises(SystemExit):
+        cm.main(["--atlas", str(atlas), "--config", str(cfg), "--out", str(tmp_path / "out")])
+
+
+def test_compose_rejects_noncanonical_incidence_for_the_rainbow_contract(tmp_path):
+    atlas = _fake_atlas(tmp_path / "atlas")
+    cfg = tmp_path / "c.json"
+    cfg.write_text(json.dumps({"directions": [[1, 0, 0.2]], "orientation": {
+        "quadrature": {"azimuth": 1, "tilt": 1, "spin": 1}, "components": [{"kind": "isotropic", "fraction": 1}]}}))
+    with pytest.raises(ValueError, match="Noncanonical"):
+        cm.run([atlas], cfg, tmp_path / "out")
+    assert not (tmp_path / "out.part").exists()
