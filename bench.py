@@ -27,7 +27,7 @@ autotuned schedule); --config surrogate runs the surrogate leg alone;
 rainbow_generate process per condition (records written to disk); --config
 atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
 atlas tool; --config compose times the orientation composition of that atlas
-(records/s) beside the checker's NumPy sum.
+(records/s) beside the oracle's NumPy sum.
 
 Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
 """
@@ -530,7 +530,7 @@ def run_compose(args):
     traced in memory) with the reference's preview composition config
     (hex_atlas/configs/spectral_preview.json: azimuth 72 x tilt 12 x spin 24,
     Gaussian 3 deg 0.9 + isotropic 0.1) at inclinations 0, 30, 60, 90 deg: one
-    record each (host coupling weights, device sum, device CDF). The checker's
+    record each (host coupling weights, device sum, device CDF). The oracle's
     NumPy sum (oracle/compose_ref.py, the reference's own operation order) is
     timed beside it on the 30 deg weights, on one core."""
     import sys as _sys
@@ -580,7 +580,7 @@ def run_compose(args):
     if not args.no_cpu_baseline:
         _sys.path.insert(0, str(ROOT / "oracle"))
         from compose_ref import compose_mass_ref
-        # bounded sample of the checker sum: the first node rows of the 30 deg weights
+        # bounded sample of the oracle sum: the first node rows of the 30 deg weights
         sub = np.zeros_like(weights30)
         terms = np.argwhere(weights30 > 0)
         take = terms[: max(1, len(terms) // 40)]

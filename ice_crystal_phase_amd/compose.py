@@ -12,7 +12,7 @@ tree (records/iNNNN/wNNNN with the record contract, orientation model files,
 generation_plan.json and batch_plan.json, staged in <out>.part).
 
 The coupling weights are assembled on the host, all poses at once as arrays
-(the pose-by-pose rule is kept as coupling_scalar, its checker). The two
+(the pose-by-pose rule is kept as coupling_scalar, the reference form it is tested against). The two
 array operations run on the device through libicephase.so: the fixed-order
 shifted-block sum over every atlas node (iph_atlas_compose, bitwise the
 reference's order and roundings) with the atlas resident in HBM across
@@ -427,7 +427,7 @@ def coupling(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
 def coupling_scalar(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
                     ki: Any) -> tuple[np.ndarray, float, np.ndarray]:
     """coupling() one pose at a time with the scalar operations of
-    atlas_math.py:273-306 (the checker of the array version)."""
+    atlas_math.py:273-306 (the reference form the array version is tested against)."""
     validate_rotations(rotations, number_weights)
     out_frame = frame(ki)
     k = out_frame[:, 2]

@@ -133,7 +133,7 @@ def test_pure_shift_pose_couples_two_neighbouring_shifts(reflected, shift):
     assert abs(w[2, 1, parity, lower % 12] - (1 - frac)) < 1e-12
     if frac > 1e-12:
         assert abs(w[2, 1, parity, (lower + 1) % 12] - frac) < 1e-12
-    # the composed mass of those weights is the shifted node block (checker sum)
+    # the composed mass of those weights is the shifted node block (oracle sum)
     out = compose_mass_ref(a.mass, w)
     original = a.mass[2, 1][:, ::-1] if reflected else a.mass[2, 1]
     want = (1 - frac) * np.roll(original, lower, axis=1) + frac * np.roll(original, lower + 1, axis=1)

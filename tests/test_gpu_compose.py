@@ -4,7 +4,7 @@ composed mass against the oracle's CDF restatement, the reference's CDF test
 cases (hex_atlas/tests/test_compose.py:100-116), and the composer end to end
 (after hex_atlas/tests/cli_test.py: two atlases from ice_precompute_atlas,
 three inclinations, a Gaussian + isotropic orientation mixture), every record
-re-derived by the checkers from the atlas files and the coupling weights."""
+re-derived by the oracles from the atlas files and the coupling weights."""
 from __future__ import annotations
 
 import json
@@ -34,7 +34,7 @@ def _fixture_mass(seed=3, it=6, ip_=3, nt=9, np_=24):
     return rng.random((it + 1, ip_ + 1, nt, np_)) * np.exp(rng.normal(0, 3, (it + 1, ip_ + 1, nt, np_)))
 
 
-def test_device_sum_equals_checker_bitwise(ctx):
+def test_device_sum_equals_oracle_bitwise(ctx):
     mass = _fixture_mass()
     rng = np.random.default_rng(7)
     w = rng.random((7, 4, 2, 24))
@@ -144,7 +144,7 @@ def test_compose_end_to_end(tmp_path, oracle):
             assert (path / name).read_bytes() == (tmp_path / "repeat" / rec["path"] / name).read_bytes()
         assert {p.name for p in path.iterdir()} == {"metadata.json", "phi_cdf.npy", "theta_given_phi_cdf.npy",
                                                     "u_edges.npy"}
-        # the checkers re-derive the record: coupling weights -> checker sum -> oracle CDF
+        # the oracles re-derive the record: coupling weights -> oracle sum -> oracle CDF
         a = atlases[rec["wavelength_index"]]
         ki = np.asarray(rec["ki"])
         w, _, _, _ = cm.coupling_for_rule(atlases[0], rule_r, rule_w, ki, dict(rule_meta, vertical_uniform_azimuth_policy={}))
