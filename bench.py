@@ -29,7 +29,7 @@ atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
 atlas tool; --config compose times the orientation composition of that atlas
 (records/s) beside the oracle's NumPy sum.
 
-Prints one JSON line (rank 0). --gpus above 1 exits: this build runs on one GPU.
+Prints one JSON line (rank 0). --gpus above 1 exits: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -643,7 +643,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes": a.numel() * 8, "launches": 4}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU (--gpus must be 1); records and ray ranges would "
+        print("bench.py: the benchmark runs on one GPU (--gpus must be 1); records and ray ranges would "
               "shard across GPUs with one u64 histogram all-reduce per record, which is not launched here.",
               file=sys.stderr)
         return 2

@@ -373,7 +373,7 @@ def _new_dir(path: str | Path) -> Path:
 
 def _plot(*_args: Any) -> None:
     import matplotlib  # noqa: F401  (raises ImportError where matplotlib is absent)
-    raise ImportError("Plotting is not part of this build; use --no-plots.")
+    raise ImportError("Plots are not implemented in this package; use --no-plots.")
 
 
 def parser() -> argparse.ArgumentParser:
