@@ -13,7 +13,7 @@
 //
 // One wave per (row t, 64-wide azimuth chunk), about one wave per SIMD for a
 // 180 x 360 grid: the wave stages the node's row (reversed for parity 1) in
-// LDS (np doubles, dynamic), then every lane accumulates its output cell over
+// LDS (dynamic), then every lane accumulates its output cell over
 // the group's terms. The sum of a cell is one serial chain of adds (its order
 // is the result), so the terms are read eight at a time (one scalar load of
 // 8 x 16 bytes) and their LDS reads and products issued ahead of the adds.
@@ -47,6 +47,10 @@ struct Term {
     double weight;
 };
 
+// WRAP: the row is staged twice in a row (2 np doubles), so the wrapped read
+// (p - s) mod np is the plain index p + np - s (np <= 4096); otherwise the
+// wrap is a compare and select.
+template <bool WRAP>
 __global__ void __launch_bounds__(64) icephase_compose(const double* __restrict__ mass, uint32_t np,
                                                        const Group* __restrict__ groups, uint32_t ngroups,
                                                        const Term* __restrict__ terms, double* __restrict__ out)
@@ -54,12 +58,17 @@ __global__ void __launch_bounds__(64) icephase_compose(const double* __restrict_
     extern __shared__ double row[];
     const uint32_t t = blockIdx.y;
     const uint32_t p = blockIdx.x * kLanes + threadIdx.x;
+    const uint32_t pn = p + np;
     double acc = 0.0;
     for (uint32_t g = 0; g < ngroups; ++g) {
         const Group G = groups[g];
         const double* src = mass + G.cell + (uint64_t)t * np;
         __syncthreads();
-        for (uint32_t q = threadIdx.x; q < np; q += kLanes) row[q] = src[G.parity ? np - 1 - q : q];
+        for (uint32_t q = threadIdx.x; q < np; q += kLanes) {
+            const double v = src[G.parity ? np - 1 - q : q];
+            row[q] = v;
+            if (WRAP) row[q + np] = v;
+        }
         __syncthreads();
         if (p < np) {
             const Term* T = terms + G.first;
@@ -69,7 +78,7 @@ __global__ void __launch_bounds__(64) icephase_compose(const double* __restrict_
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {
                     const uint32_t s = T[k + u].shift;
-                    const uint32_t q = p >= s ? p - s : p + np - s;
+                    const uint32_t q = WRAP ? pn - s : (p >= s ? p - s : pn - s);
                     prod[u] = T[k + u].weight * row[q];
                 }
 #pragma unroll
@@ -77,7 +86,7 @@ __global__ void __launch_bounds__(64) icephase_compose(const double* __restrict_
             }
             for (; k < G.count; ++k) {
                 const uint32_t s = T[k].shift;
-                const uint32_t q = p >= s ? p - s : p + np - s;
+                const uint32_t q = WRAP ? pn - s : (p >= s ? p - s : pn - s);
                 acc = acc + T[k].weight * row[q];
             }
         }
@@ -171,8 +180,12 @@ int iph_atlas_compose(iph_atlas_mass* m, const double* weights, double* out)
     IPH_HIP(ctx, hipMemcpyAsync(d_groups.p, groups.data(), groups.size() * sizeof(Group), hipMemcpyHostToDevice, s));
     IPH_HIP(ctx, hipMemcpyAsync(d_terms.p, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice, s));
     const dim3 grid((np + kLanes - 1) / kLanes, nt);
-    hipLaunchKernelGGL(icephase_compose, grid, dim3(kLanes), np * sizeof(double), s, m->mass.p, np, d_groups.p,
-                       (uint32_t)groups.size(), d_terms.p, d_out.p);
+    if (np <= kMaxPhi / 2)
+        hipLaunchKernelGGL(icephase_compose<true>, grid, dim3(kLanes), 2 * np * sizeof(double), s, m->mass.p, np,
+                           d_groups.p, (uint32_t)groups.size(), d_terms.p, d_out.p);
+    else
+        hipLaunchKernelGGL(icephase_compose<false>, grid, dim3(kLanes), np * sizeof(double), s, m->mass.p, np,
+                           d_groups.p, (uint32_t)groups.size(), d_terms.p, d_out.p);
     IPH_HIP(ctx, hipGetLastError());
     IPH_HIP(ctx, hipMemcpyAsync(out, d_out.p, cells * sizeof(double), hipMemcpyDeviceToHost, s));
     IPH_HIP(ctx, hipStreamSynchronize(s));
