@@ -404,7 +404,8 @@ def run_dataset(args):
     disk): the in-process batch driver against one rainbow_generate process per
     condition, the reference driver's way, at the reference dataset's record
     size (3001^2 rays, 1800 x 3600 query cells coarsened to a 900 x 1800 CDF,
-    r = 1 mm Table I drop). Sample: 16 inclination cells x 3 wavelengths."""
+    r = 1 mm Table I drop). Sample: 16 inclination cells x 10 wavelengths (two
+    launches; the second is traced while the first one's records are written)."""
     import contextlib
     import shutil
     import subprocess
@@ -413,7 +414,7 @@ def run_dataset(args):
     work = Path(tempfile.mkdtemp(prefix="icephase_dataset_"))
     raw = {"output_root": str(work / "data"), "generator": "bin/rainbow_generate", "radius_mm": 1.0,
            "force_sphere": False, "temperature_c": 20.0, "pressure_pa": 101325.0,
-           "wavelengths": {"start_nm": 380, "stop_nm": 830, "step_nm": 225},
+           "wavelengths": {"start_nm": 380, "stop_nm": 830, "step_nm": 50},
            "inclinations": {"min_degrees": -90, "max_degrees": 90, "count": 16,
                             "placement": "linear_angle_cell_centers"},
            "solver": {"grid": 3001, "query_theta": 1800, "query_phi": 3600, "cdf_theta": 900, "cdf_phi": 1800,
@@ -466,8 +467,9 @@ def run_dataset(args):
         "value": n / wall, "unit": "records/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 drop trace, f64 bin/CDF", "data": "synthetic",
-        "config": {"workload": f"water drop r=1mm (Table I), {n} conditions (16 inclination cells x 3 wavelengths "
-                               "380/605/830 nm), 3001^2 rays, 1800x3600 query -> 900x1800 CDF, incoherent stage"},
+        "config": {"workload": f"water drop r=1mm (Table I), {n} conditions (16 inclination cells x 10 wavelengths "
+                               "380..830 nm step 50), 3001^2 rays, 1800x3600 query -> 900x1800 CDF, incoherent "
+                               "stage"},
         "device_launch_s": sum(traced_s), "launches": len(traced_s),
         "per_process_records_per_s": 1.0 / per_process,
         "per_process_sample": f"{k} conditions, one bin/rainbow_generate process each",

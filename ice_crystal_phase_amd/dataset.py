@@ -39,6 +39,7 @@ import shutil
 import struct
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from contextlib import contextmanager
 from dataclasses import dataclass
 from decimal import Decimal, InvalidOperation
@@ -800,18 +801,35 @@ def run(argv: list[str] | None = None) -> int:
         start = time.monotonic()
         done = 0
         ctx = open_context(s["device"]) if selected else None
+        chunks = [selected[k:k + per_launch] for k in range(0, len(selected), per_launch)]
+
+        def launch(index: int):
+            conds = [resolve(c, ii, ll, root) for ii, ll in chunks[index]]
+            t0 = time.monotonic()
+            traced = trace_conditions(ctx, conds, s)
+            return conds, traced, (time.monotonic() - t0) / len(conds)
+
+        # the next launch is traced in a worker thread while this one's records
+        # are written (the device calls release the interpreter lock)
+        pool = ThreadPoolExecutor(max_workers=1) if len(chunks) > 1 else None
+        ahead = None
         try:
-            for first in range(0, len(selected), per_launch):
-                chunk = [resolve(c, ii, ll, root) for ii, ll in selected[first:first + per_launch]]
-                if shutil.disk_usage(root).free < len(chunk) * per_record + reserve:
+            for index in range(len(chunks)):
+                conds, traced, share = ahead.result() if ahead is not None else launch(index)
+                ahead = pool.submit(launch, index + 1) if pool is not None and index + 1 < len(chunks) else None
+                if shutil.disk_usage(root).free < len(conds) * per_record + reserve:
                     raise RuntimeError("Free-space reserve reached; stopping without lowering precision")
-                t0 = time.monotonic()
-                traced = trace_conditions(ctx, chunk, s)
-                share = (time.monotonic() - t0) / len(chunk)
-                for cond, t in zip(chunk, traced):
+                for cond, t in zip(conds, traced):
                     done = _finish(cond, t, c, plan, signature, root, share, done, verified, ni * nl, start,
                                    len(selected))
         finally:
+            if ahead is not None:
+                try:
+                    ahead.result()  # the device finishes before the context goes away; nothing is written
+                except Exception:
+                    pass
+            if pool is not None:
+                pool.shutdown(wait=True)
             if ctx is not None:
                 ctx.close()
         complete = verified + len(selected) == ni * nl

@@ -400,3 +400,22 @@ def test_frame_is_right_handed_and_follows_inclination():
         assert np.allclose(m.T @ m, np.eye(3), atol=2e-15) and abs(np.linalg.det(m) - 1) < 2e-15
         assert e == [0.0, 0.0, 1.0]  # +z is orthogonal to every incident direction in the x-y plane
         assert abs(k[0] - math.cos(a)) < 2e-7 and abs(k[1] + math.sin(a)) < 2e-7
+
+
+def test_pipelined_launches_write_in_plan_order_and_stop_cleanly(h):
+    """With one condition per launch the next launch is traced while a record
+    is written; records still appear in plan order, and a failure leaves no
+    record of the launch already in flight."""
+    assert h.run_some(None, "--batch-records", "1") == 0
+    assert h.jobs() == ["i0000_w0000", "i0000_w0001", "i0001_w0000", "i0001_w0001"]
+    stamps = [(h.out / f"receipts/{j}.json").stat().st_mtime_ns for j in h.jobs()]
+    assert stamps == sorted(stamps)
+
+
+def test_pipelined_failure_discards_the_launch_in_flight(h):
+    h.fail_jobs = {"i0000_w0001"}
+    with pytest.raises(RuntimeError, match="stopped at i0000_w0001"):
+        h.invoke("--execute", "--batch-records", "1")
+    assert sorted(p.name for p in (h.out / "receipts").iterdir()) == ["i0000_w0000.json"]
+    assert not (h.out / "records/i0000/w0001").exists() and not (h.out / "records/i0001").exists()
+    assert not (h.out / ".batch.lock").exists()
