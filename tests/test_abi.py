@@ -1,6 +1,7 @@
 """C-ABI checks that need no GPU: the library loads, exports every function
 include/icephase.h declares, agrees with the oracle on host-side arithmetic,
 and writes record directories with the reference's on-disk contract."""
+import ctypes
 import json
 import re
 from pathlib import Path
@@ -114,3 +115,39 @@ def test_npy_special_bit_patterns_roundtrip(tmp_path):
     ip.write_record(tmp_path / "bits", "{}", phi, theta)
     back = np.load(tmp_path / "bits" / "theta_given_phi_cdf.npy", allow_pickle=False)
     assert np.array_equal(back.view(np.uint64), theta.view(np.uint64))
+
+
+def test_atlas_entry_points_reject_bad_arguments_without_a_device():
+    L = icephase.lib()
+    s = icephase.atlas_settings(0.1, 0.2, 1.31, 3, 2, 12, 24, 2)
+    mass = np.zeros(12 * 24)
+    nodes = (icephase.AtlasNode * 1)()
+    assert L.iph_atlas_trace(None, ctypes.byref(s), 0, 1, mass.ctypes.data, nodes) == -1
+    out = ctypes.c_void_p()
+    assert L.iph_atlas_mass_create(None, mass.ctypes.data, 1, 1, 12, 24, ctypes.byref(out)) == -1
+    assert L.iph_atlas_compose(None, mass.ctypes.data, mass.ctypes.data) == -1
+    L.iph_atlas_mass_destroy(None)
+    assert L.iph_ice_index(550.0, None) == -1
+    count = ctypes.c_uint64()
+    node = icephase.AtlasNode()
+    assert L.iph_atlas_node_rays(None, 0, 0, None, 0, ctypes.byref(count), ctypes.byref(node)) == -1
+    # a too small ray buffer is refused, the count is still reported
+    assert L.iph_atlas_node_rays(ctypes.byref(s), 1, 1, None, 0, ctypes.byref(count), ctypes.byref(node)) == 0
+    small = np.zeros(5)
+    assert L.iph_atlas_node_rays(ctypes.byref(s), 1, 1, small.ctypes.data, 1, ctypes.byref(count),
+                                 ctypes.byref(node)) == -1
+    assert count.value > 1
+
+
+def test_write_npy_matches_numpy(tmp_path):
+    data = np.arange(2 * 3 * 4, dtype=np.float64).reshape(2, 3, 4) / 7
+    shape = (ctypes.c_uint64 * 3)(2, 3, 4)
+    path = tmp_path / "a.npy"
+    assert icephase.lib().iph_write_npy(str(path).encode(), shape, 3, data.ctypes.data) == 0
+    raw = path.read_bytes()
+    hlen = raw[8] | (raw[9] << 8)
+    assert raw[:8] == b"\x93NUMPY\x01\x00" and (10 + hlen) % 64 == 0
+    back = np.load(path, allow_pickle=False)
+    assert back.shape == (2, 3, 4) and back.dtype == np.float64 and np.array_equal(back, data)
+    assert len(raw) == 10 + hlen + data.nbytes
+    assert icephase.lib().iph_write_npy(str(path).encode(), shape, 0, data.ctypes.data) == -1
