@@ -26,8 +26,15 @@
  *                      the compiled reference (bitwise)
  *   orc_drop_*         sphere closed-form deflection D_p and Fresnel powers
  *                      (orc_sphere_analytic_*), FP32 vs FP64 twin agreement
- *   orc_water_*        not present: the water index is host material code
- *                      (parity unpinned in this library)
+ *   orc_atlas_rule     closed form: the ray areas add to the projected area;
+ *                      ray origins on the lit faces; node frames and ray
+ *                      counts equal the compiled reference tool's metadata
+ *   orc_atlas_node     index-matched n = 1 delta, energy balance per node,
+ *                      mirror symmetry of the phi = 0 node, masses of the
+ *                      compiled reference tool (relative L1 < 1e-14)
+ *   orc_water_*        not present: the water index is host material code,
+ *                      checked directly against the compiled reference and
+ *                      golden values (tests/test_host_services.py)
  */
 #ifndef ICEPHASE_ORACLE_H
 #define ICEPHASE_ORACLE_H
