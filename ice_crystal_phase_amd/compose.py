@@ -134,9 +134,15 @@ def validate_rotations(r: np.ndarray, w: np.ndarray) -> None:
         raise ValueError("Malformed orientation quadrature")
     if not np.isfinite(r).all() or not np.isfinite(w).all() or np.any(w <= 0) or abs(math.fsum(w) - 1) > 2e-12:
         raise ValueError("Nonfinite / nonpositive / unnormalized orientation weights")
-    proper = (np.allclose(np.swapaxes(r, 1, 2) @ r, np.eye(3), rtol=0, atol=2e-12)
-              and np.allclose(np.linalg.det(r), 1, rtol=0, atol=2e-12))
-    if not proper:
+    # R^T R = I and det R = 1 to 2e-12 (atlas_math.py:166), with the Gram
+    # matrix and the cofactor determinant written out over all poses at once
+    c = [r[:, :, i] for i in range(3)]
+    gram_ok = all(np.max(np.abs(np.einsum("nk,nk->n", c[i], c[j]) - (i == j))) <= 2e-12
+                  for i in range(3) for j in range(i, 3))
+    det = (r[:, 0, 0] * (r[:, 1, 1] * r[:, 2, 2] - r[:, 1, 2] * r[:, 2, 1])
+           - r[:, 0, 1] * (r[:, 1, 0] * r[:, 2, 2] - r[:, 1, 2] * r[:, 2, 0])
+           + r[:, 0, 2] * (r[:, 1, 0] * r[:, 2, 1] - r[:, 1, 1] * r[:, 2, 0]))
+    if not (gram_ok and np.max(np.abs(det - 1)) <= 2e-12):
         raise ValueError("Orientation nodes must be proper orthogonal rotations")
 
 
@@ -181,13 +187,14 @@ def orientation_rule(config: dict[str, Any]) -> tuple[np.ndarray, np.ndarray, di
         if na * nb * ng > 2_000_000:
             raise ValueError("Too many explicit quadrature nodes")
         share = fraction / (na * nb * ng)
+        # the spin factors are the same for every (alpha, beta): one stacked
+        # product per tilted frame gives the same matrices as one product per pose
+        spins = np.stack([ry((PI / 3) * (ig + 0.5) / ng) for ig in range(ng)])
         for ia in range(na):
             azimuth = ry(2 * PI * (ia + 0.5) / na)
             for beta in betas:
-                tilted = azimuth @ rz(float(beta))
-                for ig in range(ng):
-                    mats.append(tilted @ ry((PI / 3) * (ig + 0.5) / ng))
-                    weights.append(share)
+                mats.extend(azimuth @ rz(float(beta)) @ spins)
+        weights.extend([share] * (na * nb * ng))
         counts.append(na * nb * ng)
     r, w = np.asarray(mats), np.asarray(weights)
     validate_rotations(r, w)
@@ -395,29 +402,23 @@ def coupling(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
     j0 = np.minimum(atlas.ip - 1, y.astype(np.int64))
     ax, by = x - i0, y - j0
     w = np.zeros((atlas.it + 1, atlas.ip + 1, 2, atlas.np), dtype=np.float64)
-    # terms in the scalar order: pose, (di, dj) in (0,0), (0,1), (1,0), (1,1), then lower, lower + 1
-    idx_i, idx_j, idx_p, idx_s, vals = [], [], [], [], []
-    for di, u_ in ((0, 1 - ax), (1, ax)):
-        for dj, v_ in ((0, 1 - by), (1, by)):
-            mix = u_ * v_
-            ii, jj = i0 + di, j0 + dj
-            cval = nw * areas * mix / atlas.area[ii, jj]
-            keep = mix > 0
-            idx_i.append(np.where(keep, ii, -1))
-            idx_j.append(jj)
-            idx_p.append(parity)
-            idx_s.append(np.stack([lower % atlas.np, (lower + 1) % atlas.np], axis=1))
-            vals.append(np.stack([cval * (1 - frac), cval * frac], axis=1))
-    I = np.stack(idx_i, axis=1)          # [n, 4]
-    J = np.stack(idx_j, axis=1)
-    Pp = np.stack(idx_p, axis=1)
-    S = np.stack(idx_s, axis=1)          # [n, 4, 2]
-    V = np.stack(vals, axis=1)           # [n, 4, 2]
-    keep = (I >= 0)[:, :, None] & np.ones((1, 1, 2), bool)
-    Ib = np.broadcast_to(I[:, :, None], S.shape)[keep]
-    Jb = np.broadcast_to(J[:, :, None], S.shape)[keep]
-    Pb = np.broadcast_to(Pp[:, :, None], S.shape)[keep]
-    np.add.at(w, (Ib, Jb, Pb, S[keep]), V[keep])
+    # terms in the scalar order: pose, (di, dj) in (0,0), (0,1), (1,0), (1,1), then lower, lower + 1,
+    # as flat indices into w; np.add.at applies them one after another in that order. A corner the
+    # scalar rule skips (zero bilinear weight) adds an exact +0.0 here, which leaves every sum as it is.
+    index = np.empty((n, 4, 2), dtype=np.int64)
+    value = np.empty((n, 4, 2), dtype=np.float64)
+    shifts = (lower % atlas.np, (lower + 1) % atlas.np)
+    for corner, ((di, u_), (dj, v_)) in enumerate((a, b) for a in ((0, 1 - ax), (1, ax))
+                                                  for b in ((0, 1 - by), (1, by))):
+        mix = u_ * v_
+        ii, jj = i0 + di, j0 + dj
+        cval = np.where(mix > 0, nw * areas * mix / atlas.area[ii, jj], 0.0)
+        block = ((ii * (atlas.ip + 1) + jj) * 2 + parity) * atlas.np
+        index[:, corner, 0] = block + shifts[0]
+        index[:, corner, 1] = block + shifts[1]
+        value[:, corner, 0] = cval * (1 - frac)
+        value[:, corner, 1] = cval * frac
+    np.add.at(w.reshape(-1), index.reshape(-1), value.reshape(-1))
     represented = float(np.sum(np.sum(w, axis=(2, 3)) * atlas.area))
     if abs(represented / expected_area - 1) > 1e-11 or np.any(w < 0) or not np.isfinite(w).all():
         raise ValueError("Orientation-to-convolution area audit failed")

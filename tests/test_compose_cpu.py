@@ -79,6 +79,47 @@ def test_rule_no_rng_and_weights():
     np.testing.assert_allclose(a[0], cm.ry(math.pi / 4) @ cm.rz(beta0) @ cm.ry(math.pi / 18), atol=1e-15)
 
 
+@pytest.mark.parametrize("q", [(1, 1, 1), (3, 2, 5), (72, 12, 24), (5, 9, 31)])
+def test_rule_equals_per_pose_products(q):
+    """The rule's rotations are the per-pose products of atlas_math.py:147-154
+    (azimuth @ Rz(beta), then @ Ry(gamma) for each pose), bit for bit, with the
+    weights fraction / (na nb ng) in the same order."""
+    na, nb, ng = q
+    comps = [{"kind": "horizontal_gaussian", "sigma_deg": 11, "fraction": .7}, {"kind": "isotropic", "fraction": .3}]
+    r, w, meta = cm.orientation_rule({"quadrature": {"azimuth": na, "tilt": nb, "spin": ng}, "components": comps})
+    mats, weights = [], []
+    for comp in comps:
+        betas = (cm.gaussian_tilt_nodes(nb, comp["sigma_deg"]) if comp["kind"] == "horizontal_gaussian"
+                 else np.arccos(1 - 2 * (np.arange(nb) + 0.5) / nb))
+        for ia in range(na):
+            azimuth = cm.ry(2 * math.pi * (ia + 0.5) / na)
+            for beta in betas:
+                tilted = azimuth @ cm.rz(float(beta))
+                for ig in range(ng):
+                    mats.append(tilted @ cm.ry((math.pi / 3) * (ig + 0.5) / ng))
+                    weights.append(comp["fraction"] / (na * nb * ng))
+    assert np.array_equal(r, np.asarray(mats)) and np.array_equal(w, np.asarray(weights))
+    assert meta["component_nodes"] == [na * nb * ng] * 2
+
+
+def test_rotation_validation():
+    r, w, _ = cm.orientation_rule({"quadrature": {"azimuth": 3, "tilt": 2, "spin": 2},
+                                   "components": [{"kind": "isotropic", "fraction": 1.0}]})
+    cm.validate_rotations(r, w)
+    for bad in (r[3] @ np.diag([1, 1, -1.0]),  # improper: det -1
+                r[3] * (1 + 3e-12),            # not orthogonal beyond the 2e-12 tolerance
+                r[3] + np.triu(np.full((3, 3), 5e-12), 1)):
+        rr = r.copy()
+        rr[3] = bad
+        with pytest.raises(ValueError, match="proper orthogonal"):
+            cm.validate_rotations(rr, w)
+    cm.validate_rotations(r * (1 + 5e-13), w)  # within the tolerance
+    with pytest.raises(ValueError, match="unnormalized"):
+        cm.validate_rotations(r, w * 1.01)
+    with pytest.raises(ValueError, match="Malformed"):
+        cm.validate_rotations(r[:, :2], w)
+
+
 def test_bad_rule():
     for sigma in (-1, float("nan")):
         with pytest.raises(ValueError):
