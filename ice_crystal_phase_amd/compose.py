@@ -101,7 +101,7 @@ def gaussian_tilt_nodes(count: int, sigma_deg: float) -> np.ndarray:
     """Tilt nodes of a normal density N(pi/2, sigma^2) truncated to [0, pi]
     (measure d beta, no sin(beta) factor): the midpoints in probability of
     `count` equal strata, found by 64 bisection steps on the erf CDF. sigma = 0
-    is a delta at pi/2 (atlas_math.py:73-106)."""
+    is a delta at pi/2 (atlas_math.py:70-104)."""
     positive_int(count, "tilt count")
     sigma = math.radians(float(sigma_deg))
     if not math.isfinite(sigma) or sigma < 0:
@@ -148,7 +148,7 @@ def validate_rotations(r: np.ndarray, w: np.ndarray) -> None:
 
 def orientation_rule(config: dict[str, Any]) -> tuple[np.ndarray, np.ndarray, dict[str, Any]]:
     """Body-to-ensemble rotations R = Ry(alpha) Rz(beta) Ry(gamma) (body axis
-    +y) and NUMBER weights of every component (atlas_math.py:109-160): alpha at
+    +y) and NUMBER weights of every component (atlas_math.py:105-160): alpha at
     the midpoints of [0, 2 pi), beta isotropic (arccos midpoints) or Gaussian
     tilt nodes, gamma at the midpoints of [0, pi/3) (uniform spin of a regular
     hexagonal prism); a fixed component is one pose. Each component keeps its
@@ -217,7 +217,7 @@ def canonical_frame(theta: float, phi: float) -> np.ndarray:
 def fold_direction(direction: Any) -> tuple[float, float, np.ndarray]:
     """Canonical (theta, phi) in [0, pi/2] x [0, pi/6] and the D6h operation Q
     (possibly improper) with Q k(theta, phi) = direction: the sixfold sector,
-    the mirror inside it, and the basal mirror (atlas_math.py:174-198)."""
+    the mirror inside it, and the basal mirror (atlas_math.py:177-200)."""
     k = unit(direction)
     theta = math.atan2(math.hypot(k[0], k[2]), abs(k[1]))
     az = math.atan2(k[2], k[0]) % (2 * PI)
@@ -343,7 +343,7 @@ def coupling(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
     folded into the fundamental domain, the A-normalized node masses are
     interpolated and scaled by the exact projected area A(s), and the output
     azimuth offset of the folded frame is split between the two neighbouring
-    whole-cell shifts (atlas_math.py:273-306).
+    whole-cell shifts (atlas_math.py:272-307).
 
     All poses are processed as arrays; the weights are accumulated pose by
     pose in the scalar rule's order (np.add.at applies its terms in sequence),
@@ -428,7 +428,7 @@ def coupling(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
 def coupling_scalar(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray,
                     ki: Any) -> tuple[np.ndarray, float, np.ndarray]:
     """coupling() one pose at a time with the scalar operations of
-    atlas_math.py:273-306 (the reference form the array version is tested against)."""
+    atlas_math.py:272-307 (the reference form the array version is tested against)."""
     validate_rotations(rotations, number_weights)
     out_frame = frame(ki)
     k = out_frame[:, 2]
@@ -463,7 +463,7 @@ def coupling_for_rule(atlas: Atlas, rotations: np.ndarray, number_weights: np.nd
     """coupling(), except at exactly vertical incidence (k = +-y), where each
     isotropic or Gaussian component's uniform world azimuth only rotates the
     output azimuth: its orbit is integrated exactly by spreading every
-    tilt/spin contribution evenly over all shifts (atlas_math.py:310-368).
+    tilt/spin contribution evenly over all shifts (atlas_math.py:308-393).
     Fixed poses and non-vertical directions use the plain rule."""
     k = unit(ki)
     vertical = bool(k[0] == 0.0 and k[2] == 0.0)
@@ -523,7 +523,7 @@ def make_cdf(ctx: ip.Context, mass: np.ndarray, edges: np.ndarray, max_l1: float
              max_lost: float = 1e-12) -> tuple[np.ndarray, np.ndarray, dict[str, Any]]:
     """phi marginal [np+1] and theta|phi CDF [np, nt+1] of a cell-mass array
     [nt, np] (iph_build_cdf: ordered FP64 scans of mass/max, audited), with the
-    report of atlas_math.py:389-431."""
+    report of atlas_math.py:412-454."""
     if mass.ndim != 2 or not np.isfinite(mass).all() or np.any(mass < 0) or not float(np.max(mass)) > 0:
         raise ValueError("CDF requires finite nonnegative nonempty mass")
     nt, np_ = mass.shape
@@ -569,7 +569,7 @@ def incidents(config: dict[str, Any]) -> list[tuple[np.ndarray, dict[str, Any]]]
 def metadata(a: Atlas, expected: float, residual: float, report: dict[str, Any], f: np.ndarray, plan_hash: str,
              rule_meta: dict[str, Any], condition: dict[str, Any], source_hash: str,
              azimuth_integration: dict[str, Any], output_contract: str = "rainbow") -> dict[str, Any]:
-    """metadata.json of a composed record: the keys of compose_atlas.py:62-120."""
+    """metadata.json of a composed record: the keys of compose_atlas.py:63-122."""
     g = report["g"]
     material = dict(a.meta["material"])
     geom = dict(a.meta["geometry"])

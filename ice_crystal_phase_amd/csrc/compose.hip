@@ -7,7 +7,7 @@
 //               w[i][j][parity][s] > 0, of  w * cell_parity(i, j)[t][(p - s) mod np],
 //   cell_0 = mass[i][j],  cell_1[t][q] = mass[i][j][t][np - 1 - q],
 //
-// which is compose_mass of hex_atlas/python/atlas_math.py:371-386 evaluated in
+// which is compose_mass of hex_atlas/python/atlas_math.py:394-411 evaluated in
 // the same order with the same roundings (product rounded, then added; no
 // fused multiply-add), so the result is bitwise the reference's.
 //

@@ -365,7 +365,7 @@ typedef struct iph_atlas_mass iph_atlas_mass;
 int iph_atlas_mass_create(iph_ctx* ctx, const double* mass, uint32_t ni, uint32_t nj, uint32_t nt, uint32_t np,
                           iph_atlas_mass** out);
 void iph_atlas_mass_destroy(iph_atlas_mass* m);
-/* Replaces compose_mass (hex_atlas/python/atlas_math.py:371-386): out [nt][np]
+/* Replaces compose_mass (hex_atlas/python/atlas_math.py:394-411): out [nt][np]
  * = sum over nodes (i, j) ascending, parity 0 then 1, shift s ascending of
  * weights[i][j][parity][s] (> 0 only) times the node's cell block shifted by s
  * azimuth cells (parity 1: azimuth-reflected first, cell q -> np-1-q), in that

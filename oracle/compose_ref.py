@@ -1,6 +1,6 @@
 """Test infrastructure only (never imported by the product): NumPy
 restatement of the atlas composition sum, compose_mass of
-hex_atlas/python/atlas_math.py:371-386, the oracle of the device
+hex_atlas/python/atlas_math.py:394-411, the oracle of the device
 composition (iph_atlas_compose).
 
     out = 0; for i, j ascending, parity 0 then 1, shift s ascending with w > 0:

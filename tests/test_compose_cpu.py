@@ -81,7 +81,7 @@ def test_rule_no_rng_and_weights():
 
 @pytest.mark.parametrize("q", [(1, 1, 1), (3, 2, 5), (72, 12, 24), (5, 9, 31)])
 def test_rule_equals_per_pose_products(q):
-    """The rule's rotations are the per-pose products of atlas_math.py:147-154
+    """The rule's rotations are the per-pose products of atlas_math.py:143-149
     (azimuth @ Rz(beta), then @ Ry(gamma) for each pose), bit for bit, with the
     weights fraction / (na nb ng) in the same order."""
     na, nb, ng = q
@@ -290,7 +290,7 @@ def test_atlas_reader_audits(tmp_path):
 
 def test_compose_cli_failure_keeps_part_with_reason(tmp_path, monkeypatch):
     """A failure after staging leaves <out>.part with failure.json and exit 2
-    (compose_atlas.py:221-226); here the device context is refused."""
+    (compose_atlas.py:209-212, 225-229); here the device context is refused."""
     atlas = _fake_atlas(tmp_path / "atlas")
     cfg = tmp_path / "c.json"
     cfg.write_text(json.dumps({"inclinations_deg": [0], "orientation": {
