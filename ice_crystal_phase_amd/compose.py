@@ -30,8 +30,10 @@ import json
 import math
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
+from contextlib import closing
 from pathlib import Path
-from typing import Any
+from typing import Any, Iterator
 
 import numpy as np
 
@@ -508,6 +510,27 @@ def coupling_for_rule(atlas: Atlas, rotations: np.ndarray, number_weights: np.nd
         "tilt_spin_and_atlas_interpolation_remain_numerical": True}
 
 
+def coupling_stream(atlas: Atlas, rotations: np.ndarray, number_weights: np.ndarray, kis: list[np.ndarray],
+                    rule_meta: dict[str, Any]) -> Iterator[tuple[tuple[np.ndarray, float, np.ndarray, dict[str, Any]],
+                                                               float]]:
+    """coupling_for_rule() of each incident direction in order, each with the
+    seconds it took, the next one computed in a worker thread while the caller
+    works on the current one (the device sum, the CDF and the record files).
+    Closing the stream waits for the coupling in flight; an error of a coupling
+    is raised when its result is taken."""
+    def timed(k: np.ndarray) -> tuple[tuple[np.ndarray, float, np.ndarray, dict[str, Any]], float]:
+        start = time.perf_counter()
+        result = coupling_for_rule(atlas, rotations, number_weights, k, rule_meta)
+        return result, time.perf_counter() - start
+
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        ahead = pool.submit(timed, kis[0]) if kis else None
+        for index in range(len(kis)):
+            current = ahead.result()
+            ahead = pool.submit(timed, kis[index + 1]) if index + 1 < len(kis) else None
+            yield current
+
+
 # ------------------------------------------------------------------ device operations
 def compose_mass(ctx: ip.Context, atlas: Atlas, weights: np.ndarray) -> tuple[np.ndarray, float]:
     """Composed cell mass [nt, np] (iph_atlas_compose) and the composed
@@ -719,33 +742,37 @@ def run(atlas_paths: list[Path], config_path: Path, out: Path, output_contract: 
             computation="deterministic_offline_mass_convolution_no_raytracing", records=[])
         _write_json(part / "generation_plan.json", plan)
         ctx = ip.Context(device)
-        for i, (ki, condition) in enumerate(incident_list):
-            start = time.perf_counter()
-            c, expected, fr, azimuth = coupling_for_rule(first, rotations, weights, ki, rule_meta)
-            for w, a in enumerate(atlases):
-                mass, residual = compose_mass(ctx, a, c)
-                total = math.fsum(mass.flat)
-                if (abs((total + residual) / expected - 1) > 2e-10
-                        or residual / expected > a.meta["transport"]["max_unresolved_fraction"] + 2e-15):
-                    raise ValueError("Composed mass and unresolved area fail the transport audit")
-                phi, theta, report = make_cdf(ctx, mass, a.edges)
-                relative = Path("records") / f"i{i:04d}" / f"w{w:04d}"
-                record = part / relative
-                record.mkdir(parents=True)
-                _write_npy(record / "phi_cdf.npy", phi)
-                _write_npy(record / "theta_given_phi_cdf.npy", theta)
-                _write_npy(record / "u_edges.npy", a.edges)
-                _write_json(record / "metadata.json", metadata(a, expected, residual, report, fr, plan_hash,
-                                                              rule_meta, condition, source_hash, azimuth,
-                                                              output_contract))
-                files = {n: sha256_file(record / n)
-                         for n in ("metadata.json", "phi_cdf.npy", "theta_given_phi_cdf.npy", "u_edges.npy")}
-                plan["records"].append(dict(incident_index=i, wavelength_index=w, path=relative.as_posix(),
-                                            condition=[wavelengths[w], -float(ki[1])], ki=ki.tolist(),
-                                            wavelength_nm=wavelengths[w], g=report["g"], files_sha256=files,
-                                            azimuth_integration=azimuth))
-                print(f"record i={i} w={w} mass=1 g={report['g']:.17g} A={expected:.17g}", flush=True)
-            print(f"incident_complete i={i} offline_seconds={time.perf_counter() - start:.6f}", flush=True)
+        # the coupling of the next incident is computed in a worker thread while
+        # this one's records are composed on the device and written
+        with closing(coupling_stream(first, rotations, weights, [k for k, _ in incident_list],
+                                     rule_meta)) as couplings:
+            for i, ((ki, condition), ((c, expected, fr, azimuth), seconds)) in enumerate(zip(incident_list,
+                                                                                           couplings)):
+                start = time.perf_counter() - seconds
+                for w, a in enumerate(atlases):
+                    mass, residual = compose_mass(ctx, a, c)
+                    total = math.fsum(mass.flat)
+                    if (abs((total + residual) / expected - 1) > 2e-10
+                            or residual / expected > a.meta["transport"]["max_unresolved_fraction"] + 2e-15):
+                        raise ValueError("Composed mass and unresolved area fail the transport audit")
+                    phi, theta, report = make_cdf(ctx, mass, a.edges)
+                    relative = Path("records") / f"i{i:04d}" / f"w{w:04d}"
+                    record = part / relative
+                    record.mkdir(parents=True)
+                    _write_npy(record / "phi_cdf.npy", phi)
+                    _write_npy(record / "theta_given_phi_cdf.npy", theta)
+                    _write_npy(record / "u_edges.npy", a.edges)
+                    _write_json(record / "metadata.json", metadata(a, expected, residual, report, fr, plan_hash,
+                                                                  rule_meta, condition, source_hash, azimuth,
+                                                                  output_contract))
+                    files = {n: sha256_file(record / n)
+                             for n in ("metadata.json", "phi_cdf.npy", "theta_given_phi_cdf.npy", "u_edges.npy")}
+                    plan["records"].append(dict(incident_index=i, wavelength_index=w, path=relative.as_posix(),
+                                                condition=[wavelengths[w], -float(ki[1])], ki=ki.tolist(),
+                                                wavelength_nm=wavelengths[w], g=report["g"], files_sha256=files,
+                                                azimuth_integration=azimuth))
+                    print(f"record i={i} w={w} mass=1 g={report['g']:.17g} A={expected:.17g}", flush=True)
+                print(f"incident_complete i={i} offline_seconds={time.perf_counter() - start:.6f}", flush=True)
         plan["complete"] = True
         _write_json(part / "batch_plan.json", plan)
         if out.exists():

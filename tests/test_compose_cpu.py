@@ -230,6 +230,31 @@ def test_incidents():
         cm.incidents({"inclinations_deg": [0], "directions": [[1, 0, 0]]})
 
 
+def test_coupling_stream_is_the_sequence_of_couplings():
+    """The composer's stream (next coupling computed in a worker thread) yields
+    exactly coupling_for_rule() of each direction, in order; a failing
+    direction raises when its turn comes; closing early waits for the worker."""
+    a = fixture()
+    config = {"quadrature": {"azimuth": 5, "tilt": 4, "spin": 3}, "components": [
+        {"kind": "horizontal_gaussian", "sigma_deg": 9, "fraction": .8}, {"kind": "isotropic", "fraction": .2}]}
+    r, w, meta = cm.orientation_rule(config)
+    kis = [np.array(k, dtype=float) for k in ([0, -1, 0], [1, 0, 0], [.5, -.8, 0], [.2, .9, 0])]
+    got = list(cm.coupling_stream(a, r, w, kis, meta))
+    assert len(got) == len(kis)
+    for k, ((c, area, f, info), seconds) in zip(kis, got):
+        c2, area2, f2, info2 = cm.coupling_for_rule(a, r, w, k, meta)
+        assert np.array_equal(c, c2) and area == area2 and np.array_equal(f, f2) and info == info2
+        assert seconds > 0
+    stream = cm.coupling_stream(a, r, w, [kis[1], np.array([np.nan, 1.0, 0.0]), kis[2]], meta)
+    next(stream)
+    with pytest.raises(ValueError):
+        next(stream)
+    stream = cm.coupling_stream(a, r, w, kis, meta)
+    next(stream)
+    stream.close()
+    assert list(cm.coupling_stream(a, r, w, [], meta)) == []
+
+
 @pytest.mark.parametrize("k", [[1, 0, 0], [0.5, -0.8, 0], [1, -.4, .3], [0, -1, 0]])
 def test_array_coupling_equals_scalar_rule(k):
     """The array coupling (all poses at once, weights accumulated in the scalar
