@@ -184,6 +184,14 @@ def _check(rc: int, ctx_ptr=None) -> None:
         raise IcePhaseError(rc, msg)
 
 
+def _live(handle: C.c_void_p, what: str) -> C.c_void_p:
+    """The handle of an open library object; a closed one raises here, not in
+    the library (whose message for a NULL handle is not the context's)."""
+    if not handle:
+        raise IcePhaseError(-1, f"{what} is closed")
+    return handle
+
+
 EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_destroy", "iph_last_error",
             "iph_u_edges", "iph_bin_directions", "iph_batch_create", "iph_batch_destroy", "iph_batch_run",
             "iph_batch_set_schedule", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
@@ -414,6 +422,7 @@ class AtlasMass:
 
     def compose(self, weights: np.ndarray) -> np.ndarray:
         """Composed cell mass [nt, np] of coupling weights [ni, nj, 2, np]."""
+        _live(self._p, "atlas mass")
         ni, nj, nt, np_ = self.shape
         w = np.ascontiguousarray(weights, np.float64)
         if w.shape != (ni, nj, 2, np_):
@@ -474,25 +483,25 @@ class Batch:
             pass
 
     def run(self, flags: int = RUN_TRACE | RUN_CDF) -> None:
-        _check(lib().iph_batch_run(self._p, flags), self.ctx.ptr)
+        _check(lib().iph_batch_run(_live(self._p, "batch"), flags), self.ctx.ptr)
 
     def set_schedule(self, tile_rays: int = 0, blocks: int = 0) -> None:
-        _check(lib().iph_batch_set_schedule(self._p, tile_rays, blocks), self.ctx.ptr)
+        _check(lib().iph_batch_set_schedule(_live(self._p, "batch"), tile_rays, blocks), self.ctx.ptr)
 
     def timing(self) -> tuple[float, float, float]:
         ms = (f64 * 3)()
-        _check(lib().iph_batch_timing(self._p, ms))
+        _check(lib().iph_batch_timing(_live(self._p, "batch"), ms))
         return ms[0], ms[1], ms[2]
 
     def stats(self, rec: int = 0) -> HistStats:
         st = HistStats()
-        _check(lib().iph_batch_stats(self._p, rec, C.byref(st)), self.ctx.ptr)
+        _check(lib().iph_batch_stats(_live(self._p, "batch"), rec, C.byref(st)), self.ctx.ptr)
         return st
 
     def bins(self, rec: int = 0) -> np.ndarray:
         r = self.records[rec]
         out = np.zeros(int(r.theta_count) * int(r.phi_count), np.uint64)
-        _check(lib().iph_batch_bins(self._p, rec, out.ctypes.data), self.ctx.ptr)
+        _check(lib().iph_batch_bins(_live(self._p, "batch"), rec, out.ctypes.data), self.ctx.ptr)
         return out
 
     @property
@@ -506,7 +515,8 @@ class Batch:
         phi = np.zeros(onp + 1, np.float64)
         theta = np.zeros(onp * (ont + 1), np.float64)
         rep = CdfReport()
-        _check(lib().iph_batch_cdf(self._p, rec, phi.ctypes.data, theta.ctypes.data, C.byref(rep)), self.ctx.ptr)
+        _check(lib().iph_batch_cdf(_live(self._p, "batch"), rec, phi.ctypes.data, theta.ctypes.data, C.byref(rep)),
+               self.ctx.ptr)
         return phi, theta.reshape(onp, ont + 1), rep
 
 

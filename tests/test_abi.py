@@ -151,3 +151,18 @@ def test_write_npy_matches_numpy(tmp_path):
     assert back.shape == (2, 3, 4) and back.dtype == np.float64 and np.array_equal(back, data)
     assert len(raw) == 10 + hlen + data.nbytes
     assert icephase.lib().iph_write_npy(str(path).encode(), shape, 0, data.ctypes.data) == -1
+
+
+def test_closed_handles_raise_before_the_library():
+    """A closed batch or atlas mass raises its own message (the library's
+    message for a NULL handle is the thread's, not the context's)."""
+    from types import SimpleNamespace
+    b = icephase.Batch.__new__(icephase.Batch)
+    b._p, b.ctx, b.records, b.policy = ctypes.c_void_p(), SimpleNamespace(ptr=None), [icephase.Record()], None
+    for call in (b.run, b.timing, b.stats, b.bins, b.cdf, lambda: b.set_schedule(256, 8)):
+        with pytest.raises(ip.IcePhaseError, match="batch is closed"):
+            call()
+    a = icephase.AtlasMass.__new__(icephase.AtlasMass)
+    a._p, a.ctx, a.shape = ctypes.c_void_p(), SimpleNamespace(ptr=None), (1, 1, 2, 3)
+    with pytest.raises(ip.IcePhaseError, match="atlas mass is closed"):
+        a.compose(np.ones((1, 1, 2, 3)))
