@@ -56,8 +56,9 @@ def sha256(path: Path) -> str:
 
 
 def total(values: np.ndarray) -> float:
-    """Correctly rounded sum of all entries."""
-    return math.fsum(map(float, np.ravel(values)))
+    """Correctly rounded sum of all entries (one sum whatever the order, so the
+    entries go to fsum as one Python list)."""
+    return math.fsum(np.ravel(np.asarray(values, dtype=np.float64)).tolist())
 
 
 def theta_of_u(u: np.ndarray) -> np.ndarray:
@@ -96,9 +97,9 @@ class Table:
 
     @cached_property
     def theta_mass(self) -> np.ndarray:
-        return np.array([math.fsum(map(float, col)) for col in self.mass.T])
+        return np.array([math.fsum(col) for col in self.mass.T.tolist()])
 
-    @property
+    @cached_property
     def g(self) -> float:
         return total(self.mass * (1 - self.u[:-1] - self.u[1:])) / total(self.mass)
 
