@@ -210,6 +210,118 @@ def test_vertical_incidence_integrates_the_azimuth_orbit():
     assert info["method"] == "original_fixed_orientation_quadrature"
 
 
+# ---- polar incidence: the analytic azimuth orbit (after hex_atlas/tests/test_polar_azimuth.py:13-126)
+def _polar_config(na=16):
+    return {"quadrature": {"azimuth": na, "tilt": 4, "spin": 3}, "components": [
+        {"kind": "horizontal_gaussian", "sigma_deg": 10, "fraction": .9}, {"kind": "isotropic", "fraction": .1}]}
+
+
+def _evaluate(a, config, ki, analytic=True):
+    """Composed mass (oracle sum), unresolved tail, area, frame and method info."""
+    r, w, meta = cm.orientation_rule(config)
+    if analytic:
+        c, area, f, info = cm.coupling_for_rule(a, r, w, ki, meta)
+    else:
+        (c, area, f), info = cm.coupling(a, r, w, ki), {}
+    return compose_mass_ref(a.mass, c), float(np.sum(np.sum(c, axis=(2, 3)) * a.residual)), area, f, info
+
+
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_polar_orbit_is_the_row_mean_of_the_discrete_rule(sign):
+    """At k = -y or +y the orbit spreads each row's mass evenly over azimuth:
+    each row of the composed mass is the row mean of the discrete quadrature's,
+    with the same row totals, tail and projected area."""
+    a = fixture()
+    a.mass *= .97
+    a.residual = .03 * a.area
+    old, old_tail, old_area, _, _ = _evaluate(a, _polar_config(), [0, sign, 0], analytic=False)
+    new, tail, area, f, info = _evaluate(a, _polar_config(), [0, sign, 0])
+    np.testing.assert_allclose(new, np.repeat(old.mean(axis=1, keepdims=True), a.np, axis=1), rtol=1e-12,
+                               atol=2e-17)
+    np.testing.assert_allclose(new.sum(axis=1), old.sum(axis=1), rtol=1e-12, atol=2e-17)
+    assert np.array_equal(f, cm.frame([0, sign, 0]))
+    assert abs(tail - old_tail) <= 1e-16 and abs(area - old_area) <= 1e-15
+    assert abs(new.sum() + tail - area) <= 1e-15
+    assert info["analytic_uniform_azimuth_components"] == [0, 1]
+    assert info["effective_pose_evaluations"] == 4 * 3 * 2 and info["output_post_filter"] is False
+
+
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_polar_orbit_equals_a_complete_discrete_cycle(sign):
+    """With one azimuth node per output cell the discrete rule is itself a
+    complete cyclic average, and agrees with the analytic orbit."""
+    a = fixture()
+    a.mass[:] = 0
+    a.mass[:, :, 1, 2] = a.area
+    new = _evaluate(a, _polar_config(3), [0, sign, 0])[0]
+    discrete = _evaluate(a, _polar_config(a.np), [0, sign, 0], analytic=False)[0]
+    np.testing.assert_allclose(new, discrete, rtol=2e-12, atol=3e-17)
+
+
+def test_polar_orbit_ignores_the_azimuth_node_count():
+    a = fixture()
+    first = _evaluate(a, _polar_config(1), [0, -1, 0])[0]
+    for na in (2, 7, 16, 23):
+        assert np.array_equal(_evaluate(a, _polar_config(na), [0, -1, 0])[0], first)
+
+
+@pytest.mark.parametrize("ki", [[1, 0, 0], [1, -.7, .2], [1e-13, -1, 0]])
+def test_off_pole_directions_use_the_discrete_rule(ki):
+    a = fixture()
+    new, tail, area, f, info = _evaluate(a, _polar_config(), ki)
+    old, old_tail, old_area, old_f, _ = _evaluate(a, _polar_config(), ki, analytic=False)
+    assert np.array_equal(new, old) and np.array_equal(f, old_f)
+    assert tail == old_tail and area == old_area
+    assert info["method"] == "original_fixed_orientation_quadrature"
+
+
+def test_fixed_pose_at_the_pole_is_not_averaged():
+    a = fixture()
+    config = dict(_polar_config(), components=[{"kind": "fixed", "fraction": 1., "quaternion_wxyz": [1, 0, 0, 0]}])
+    new, *_, info = _evaluate(a, config, [0, -1, 0])
+    assert np.array_equal(new, _evaluate(a, config, [0, -1, 0], analytic=False)[0])
+    assert float(np.max(np.ptp(new, axis=1))) > 0
+    assert info["analytic_uniform_azimuth_components"] == []
+
+
+def test_mixture_with_a_fixed_pose_averages_only_the_uniform_part():
+    a = fixture()
+    fixed = {"kind": "fixed", "fraction": .4, "quaternion_wxyz": [1, 0, 0, 0]}
+    iso = {"kind": "isotropic", "fraction": .6}
+    quadrature = _polar_config()["quadrature"]
+    mass, _, area, _, info = _evaluate(a, {"quadrature": quadrature, "components": [fixed, iso]}, [0, -1, 0])
+    pieces = [_evaluate(a, {"quadrature": quadrature, "components": [dict(c, fraction=1.)]}, [0, -1, 0])
+              for c in (fixed, iso)]
+    np.testing.assert_allclose(mass, .4 * pieces[0][0] + .6 * pieces[1][0], rtol=2e-12, atol=2e-17)
+    assert abs(area - (.4 * pieces[0][2] + .6 * pieces[1][2])) < 1e-15
+    assert info["analytic_uniform_azimuth_components"] == [1]
+    assert float(np.max(np.ptp(mass, axis=1))) > 0
+
+
+def test_polar_orbit_keeps_theta_sparsity(oracle):
+    """One populated theta row stays the only populated row (every azimuth of it
+    now positive), and its CDF (oracle restatement of the CDF build) gives the
+    cell probabilities back."""
+    a = fixture()
+    a.mass[:] = 0
+    a.mass[:, :, 1, 2] = a.area
+    mass = _evaluate(a, _polar_config(), [0, -1, 0])[0]
+    assert np.all(mass[1] > 0)
+    assert np.array_equal(mass[[0, 2, 3]], np.zeros((3, a.np)))
+    rc, phi, theta, rep = oracle.build_cdf(mass, a.nt, a.np)
+    assert rc == 0
+    recovered = (np.diff(phi)[:, None] * np.diff(theta, axis=1)).T
+    np.testing.assert_allclose(recovered, mass / mass.sum(), rtol=1e-13, atol=2e-16)
+    assert rep.l1_error < 1e-12 and rep.lost_mass == 0
+
+
+def test_polar_orbit_needs_the_declared_rule():
+    a = fixture()
+    r, w, meta = cm.orientation_rule(_polar_config())
+    with pytest.raises(ValueError, match="declared"):
+        cm.coupling_for_rule(a, r, w[::-1].copy(), [0, -1, 0], meta)
+
+
 def test_residual_not_renormalized_away():
     a = fixture()
     a.mass *= .99
