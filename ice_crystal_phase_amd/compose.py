@@ -12,7 +12,8 @@ tree (records/iNNNN/wNNNN with the record contract, orientation model files,
 generation_plan.json and batch_plan.json, staged in <out>.part).
 
 The coupling weights are assembled on the host, all poses at once as arrays
-(the pose-by-pose rule is kept as coupling_scalar, the reference form it is tested against). The two
+(the pose-by-pose rule is kept as coupling_scalar, the reference form it is tested against), the
+next incident's in a worker thread while the current one is on the device. The two
 array operations run on the device through libicephase.so: the fixed-order
 shifted-block sum over every atlas node (iph_atlas_compose, bitwise the
 reference's order and roundings) with the atlas resident in HBM across
