@@ -37,7 +37,7 @@ struct Command {
     uint32_t grid = 513, theta = 360, phi = 720, cdf_theta = 0, cdf_phi = 0;
     double inclination = 20.0, max_tv = 1.0, max_l1 = 1e-10, max_lost = 1e-12, transition_contrast = 0;
     int device = 0;
-    std::string stage = "incoherent";
+    std::string stage = "diffraction";  // the reference default (apps/generate_phase.cpp:54)
     unsigned focal[4] = {0, 0, 0, 0};
     bool coefficients_given = false;
     double coefficients[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -54,7 +54,7 @@ void usage()
     std::cout << "rainbow_generate --out DIRECTORY [--modules DIRECTORY] [--sphere]\n"
                  "  --radius-mm R --wavelength-nm L (required, vacuum nm) --grid RAYS_PER_AXIS\n"
                  "  --inclination-deg A --query-theta T --query-phi P\n"
-                 "  --stage incoherent (path|focal|diffraction are not built)\n"
+                 "  --stage incoherent|path|focal|diffraction (default diffraction; only incoherent is built)\n"
                  "  --material water (default) --temperature-c T --pressure-pa P\n"
                  "  --material constant --ior N (explicit nondispersive reference)\n"
                  "  --cdf-max-l1 E --cdf-max-lost-mass E --device ORDINAL\n"
@@ -182,10 +182,6 @@ bool parse(int argc, char** argv, Command& c)
     if (c.out.empty()) throw std::invalid_argument("--out is required.");
     if (!c.wavelength_given)
         throw std::invalid_argument("--wavelength-nm is required; there is no implicit wavelength.");
-    if (c.stage != "incoherent")
-        throw std::invalid_argument("Stage '" + c.stage +
-                                    "' (coherent path/focal phases, Table II diffraction) is not built in this tool; "
-                                    "use --stage incoherent.");
     if (c.water) {
         if (c.index_given)
             throw std::invalid_argument(
@@ -217,6 +213,13 @@ bool parse(int argc, char** argv, Command& c)
     if (fs::exists(c.out)) throw std::runtime_error("Output already exists.");
     if (fs::exists(c.out + ".part"))
         throw std::runtime_error("Previous .part directory exists; inspect it before recomputing.");
+    // Every other argument is checked first, as the reference does before it
+    // runs the stage; the coherent stages (its default is diffraction) are then
+    // refused by name rather than replaced silently by another estimand.
+    if (c.stage != "incoherent")
+        throw std::invalid_argument("Stage '" + c.stage +
+                                    "' (coherent path/focal phases, Table II diffraction) is not built in this tool; "
+                                    "use --stage incoherent.");
     return true;
 }
 

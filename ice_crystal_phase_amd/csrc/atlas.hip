@@ -356,6 +356,8 @@ int iph_atlas_trace(iph_ctx* ctx, const iph_atlas_settings* s, uint32_t first, u
     map.n = count;
     map.total = tile_start[count];
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->cus * 8, map.total));
+    if (map.total + blocks > kMaxQueueTickets)
+        return fail(ctx, IPH_E_INVALID, "too many area rays for one call's 32-bit tile queue; trace fewer nodes per call.");
     IPH_HIP(ctx, launch_zero(d_recs.p, count, d_queue.p, st));
     IPH_HIP(ctx, launch_trace_area(nt + 1 > 2048, d_recs.p, map, d_queue.p + 1, blocks, 256, st));
     // FP64 masses and exact quanta totals on the device, straight into the caller's array

@@ -439,6 +439,23 @@ int iph_batch_set_schedule(iph_batch* b, uint64_t tile_rays, uint32_t blocks)
 {
     if (!b) return fail(nullptr, IPH_E_INVALID, "batch must not be NULL.");
     iph_ctx* ctx = b->ctx;
+    // The tile queue and a tile's ray counter are 32-bit device counters: a
+    // workgroup takes one more queue ticket than there are tiles before it sees
+    // the end, and a tile's counter runs past its ray count by at most one
+    // ticket per wave. Schedules that could wrap either are refused here.
+    if (tile_rays > kMaxTileRays)
+        return fail(ctx, IPH_E_INVALID, "tile_rays must be at most 2^31.");
+    {
+        uint64_t tiles = 0;
+        for (uint32_t r = 0; r < b->nrec; ++r) {
+            uint64_t t = tile_rays ? tile_rays : default_tile_rays(b->host[r].rays);
+            tiles += (b->host[r].rays + t - 1) / t;
+        }
+        const uint64_t want = blocks ? blocks : (uint64_t)ctx->cus * 8;
+        if (tiles + want > kMaxQueueTickets)
+            return fail(ctx, IPH_E_INVALID, "schedule has more tiles than the 32-bit tile queue holds; "
+                                            "use larger tiles or fewer rays per batch.");
+    }
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     b->tile_rays = tile_rays;
     std::vector<uint32_t> kind_recs[2];
@@ -446,12 +463,7 @@ int iph_batch_set_schedule(iph_batch* b, uint64_t tile_rays, uint32_t blocks)
     for (int k = 0; k < 2; ++k) starts[k].push_back(0);
     for (uint32_t r = 0; r < b->nrec; ++r) {
         DevRecord& R = b->host[r];
-        uint64_t t = tile_rays;
-        if (!t) {
-            // default: ~64 tiles per record, 2048..65536 rays per tile
-            t = R.rays / 64;
-            t = t < 2048 ? 2048 : (t > 65536 ? 65536 : t);
-        }
+        const uint64_t t = tile_rays ? tile_rays : default_tile_rays(R.rays);
         R.tile_rays = t;
         kind_recs[R.kind].push_back(r);
         starts[R.kind].push_back(starts[R.kind].back() + (R.rays + t - 1) / t);

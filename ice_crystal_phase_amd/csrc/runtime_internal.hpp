@@ -58,6 +58,18 @@ struct DevBuf {
     }
 };
 
+// Limits of the persistent tile queue (a 32-bit ticket counter per launch) and
+// of a tile's 32-bit ray counter.
+constexpr uint64_t kMaxTileRays = 1ull << 31;
+constexpr uint64_t kMaxQueueTickets = 0xffffffffull;
+
+// Library default tile: about 64 tiles per record, 2048..65536 rays per tile.
+inline uint64_t default_tile_rays(uint64_t rays)
+{
+    const uint64_t t = rays / 64;
+    return t < 2048 ? 2048 : (t > 65536 ? 65536 : t);
+}
+
 // Grid (theta_count, phi_count) of record rec of a batch; false for an invalid
 // record index.
 bool batch_record_grid(const iph_batch* b, uint32_t rec, uint32_t& nt, uint32_t& np);

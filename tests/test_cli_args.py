@@ -54,6 +54,19 @@ def test_rainbow_generate_refuses_existing_part(tmp_path):
     assert r.returncode == 1 and "Previous .part directory exists" in r.stderr
 
 
+def test_rainbow_generate_default_stage_is_the_references(tmp_path):
+    """Without --stage the reference runs its diffraction stage
+    (apps/generate_phase.cpp:54); the tool takes the same default and refuses it
+    by name after every other argument has been checked, instead of silently
+    producing the incoherent estimand."""
+    r = _run(BIN / "rainbow_generate", "--out", "x", "--wavelength-nm", "550", cwd=tmp_path)
+    assert r.returncode == 1 and "Stage 'diffraction'" in r.stderr, r.stderr
+    assert not (tmp_path / "x").exists() and not (tmp_path / "x.part").exists()
+    # an argument error is still reported first, as in the reference's order
+    r = _run(BIN / "rainbow_generate", "--out", "x", "--wavelength-nm", "550", "--ior", "1.3", cwd=tmp_path)
+    assert r.returncode == 1 and "Do not specify --ior for water" in r.stderr
+
+
 def test_rainbow_generate_help():
     r = _run(BIN / "rainbow_generate", "--help")
     assert r.returncode == 0 and "--wavelength-nm" in r.stdout

@@ -26,3 +26,30 @@ def test_schedule_invariance_and_autotune(ctx):
     choice = autotune.tune(ctx, recs, shrink=4, repeats=1,
                            cands=[(0, 1024), (2048, 1024), (8192, 2048)])
     assert choice.ms > 0 and choice.blocks in (1024, 2048)
+
+
+def test_schedules_that_wrap_the_tile_queue_are_refused(ctx):
+    """The persistent launch hands out tiles from a 32-bit queue counter and
+    counts a tile's rays in 32 bits: schedules whose tiles (plus one exit ticket
+    per workgroup) exceed 2^32 - 1, or whose tiles exceed 2^31 rays, are refused
+    before anything is launched, and the batch keeps its previous schedule."""
+    print("case: tile-queue limits, 64 hex records of 2^26 rays (not run)", flush=True)
+    big = [ip.hex_record(0.05, 0.2, 1.31, 1 << 26, seed=k, theta=8, phi=16) for k in range(64)]
+    with ip.Batch(ctx, big) as b:
+        with pytest.raises(ip.IcePhaseError, match="32-bit tile queue"):
+            b.set_schedule(1, 0)            # 2^32 tiles
+        with pytest.raises(ip.IcePhaseError, match="32-bit tile queue"):
+            b.set_schedule(2, 1 << 31)      # 2^31 tiles + 2^31 exit tickets
+        b.set_schedule(4, 1024)             # 2^30 tiles + 1024: fits
+        with pytest.raises(ip.IcePhaseError, match="at most 2\\^31"):
+            b.set_schedule((1 << 31) + 1, 0)
+    print("case: tile-queue limits, small batch runs after a refusal", flush=True)
+    small = [ip.hex_record(0.05, 0.2, 1.31, 4096, seed=k, theta=18, phi=36) for k in range(2)]
+    with ip.Batch(ctx, small) as b:
+        b.run(ip.RUN_TRACE)
+        ref = [b.bins(i).copy() for i in range(2)]
+        with pytest.raises(ip.IcePhaseError):
+            b.set_schedule((1 << 31) + 1, 0)
+        b.run(ip.RUN_TRACE)
+        for i in range(2):
+            assert np.array_equal(b.bins(i), ref[i])

@@ -189,7 +189,7 @@ DROP_KEYS = {
 def test_rainbow_generate_record(tmp_path, extra):
     args = [BIN / "rainbow_generate", "--out", "rec", "--radius-mm", "2.0", "--wavelength-nm", "550",
             "--inclination-deg", "-30", "--grid", "256", "--query-theta", "180", "--query-phi", "360",
-            "--cdf-theta", "90", "--cdf-phi", "180"] + extra
+            "--cdf-theta", "90", "--cdf-phi", "180", "--stage", "incoherent"] + extra
     res = _run(args, tmp_path)
     assert res.returncode == 0, res.stderr
     m = record_check.check_record(tmp_path / "rec")
