@@ -20,8 +20,17 @@ bool make_u_edges(uint32_t n, std::vector<double>& e, std::string& err)
         return false;
     }
     e.assign((size_t)n + 1, 0.0);
-    e.front() = 0;
-    e.back() = 1;
+    return make_u_edges(n, e.data(), err);
+}
+
+bool make_u_edges(uint32_t n, double* e, std::string& err)
+{
+    if (n == 0) {
+        err = "theta_count must be positive.";
+        return false;
+    }
+    e[0] = 0;
+    e[n] = 1;
     for (uint32_t i = 1; i < n; ++i) {
         const uint32_t k = i < n - i ? i : n - i;
         const double s = std::sin((0.5 * kPi) * double(k) / double(n));

@@ -16,6 +16,8 @@ namespace icephase {
 // rainbow::make_phase_u_edges (include/rainbow/phase_cdf_math.hpp:16-32):
 // u_i = sin^2(pi min(i, n-i) / 2n), mirrored about 1/2, exact 0 and 1 ends.
 bool make_u_edges(uint32_t n, std::vector<double>& out, std::string& err);
+// The same edges into a caller array of n + 1 doubles (no allocation).
+bool make_u_edges(uint32_t n, double* out, std::string& err);
 
 // Drop contract frame: e0 = normalize(axis - k (k.axis)), axis = +z unless
 // |k.z| >= 0.9 (then +x), e1 = k x e0 (include/rainbow/vec3.hpp:56-65, FP64).

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -104,6 +105,72 @@ const char* report_reason(int status)
     case 6: return "CDF mass / HG audit failed.";
     default: return "ok";
     }
+}
+
+}  // namespace
+
+// Fixed device scratch of iph_build_cdf_noalloc: one copy per GPU, held until
+// the process ends. In host code `scratch` only names the symbol; its device
+// address is looked up once per device (hipGetSymbolAddress, with the device
+// current) when a context is created, kept in g_scratch and in the context, and
+// only that address is passed to kernels and copies.
+constexpr size_t kNoallocScratchBytes = 9u << 20;
+constexpr uint32_t kNoallocMaxTheta = 4096;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch;  // per device ordinal
+
+// Device address of the current device's scratch (looked up on first use).
+int device_scratch(int device, int count, unsigned char** out, std::string& err)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch.size() < (size_t)count) g_scratch.resize(count, nullptr);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return IPH_E_DEVICE;
+        }
+        g_scratch[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[device];
+    return IPH_OK;
+}
+
+// Byte offsets of one allocation-free CDF inside the scratch (256-byte aligned).
+struct NoallocLayout {
+    size_t job, edges, cedges, cols, scalars, partials, coarse, bytes;
+};
+
+NoallocLayout noalloc_layout(uint32_t nt, const CdfSetup& s, uint32_t pblocks, bool coarse)
+{
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    NoallocLayout L{};
+    size_t o = 0;
+    L.job = o; o = up(o + sizeof(CdfJob));
+    L.edges = o; o = up(o + (size_t)(nt + 1) * sizeof(double));
+    L.cedges = o; o = up(o + (size_t)(s.ont + 1) * sizeof(double));
+    L.cols = o; o = up(o + (size_t)s.onp * sizeof(double));
+    L.scalars = o; o = up(o + kCdfScalars * sizeof(double));
+    L.partials = o; o = up(o + (size_t)pblocks * 8 * sizeof(double));
+    L.coarse = o; o = up(o + (coarse ? (size_t)s.ont * s.onp * sizeof(double) : 0));
+    L.bytes = o;
+    return L;
+}
+
+// True when p is device (or managed) memory of the given device.
+bool device_memory_of(const void* p, int device)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unregistered host pointer: clear the query's error
+        return false;
+    }
+    return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) && a.device == device;
 }
 
 }  // namespace
@@ -203,6 +270,11 @@ int iph_ctx_create(int device, iph_ctx** out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->cus = prop.multiProcessorCount;
+    std::string err;
+    if (device_scratch(device, count, &c->cdf_scratch, err) != IPH_OK) {
+        (void)hipStreamDestroy(c->stream);
+        return fail(nullptr, IPH_E_DEVICE, err);
+    }
     *out = c.release();
     return IPH_OK;
 }
@@ -718,6 +790,64 @@ int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, co
     IPH_HIP(ctx, hipMemcpyAsync(theta, J.theta, sz_theta * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     IPH_HIP(ctx, hipMemcpyAsync(sc, J.scalars, sizeof sc, hipMemcpyDeviceToHost, ctx->stream));
     IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const iph_cdf_report rep = make_report(sc, s, coarse);
+    if (report) *report = rep;
+    if (rep.status) return fail(ctx, IPH_E_AUDIT, report_reason(rep.status));
+    return IPH_OK;
+}
+
+int iph_build_cdf_noalloc(iph_ctx* ctx, const double* d_mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
+                          double* d_phi, double* d_theta, iph_cdf_report* report)
+{
+    if (!ctx) return fail(nullptr, IPH_E_INVALID, "ctx must not be NULL.");
+    if (!d_mass || !d_phi || !d_theta || !nt || !np || (uint64_t)nt * np > 0xffffffffull)
+        return fail(ctx, IPH_E_INVALID, "Invalid CDF arguments.");
+    if (nt > kNoallocMaxTheta)
+        return fail(ctx, IPH_E_INVALID, "theta_count above 4096, the allocation-free CDF's limit.");
+    if (!ctx->cdf_scratch) return fail(ctx, IPH_E_DEVICE, "The context has no device scratch.");
+    CdfSetup s;
+    std::string err;
+    if (!resolve_policy(policy, nt, np, s, err)) return fail(ctx, IPH_E_INVALID, err);
+    const bool coarse = s.ont != nt || s.onp != np;
+    const uint32_t red = 512;
+    const uint32_t pblocks = cdf_partial_blocks(red, s.onp);
+    const NoallocLayout L = noalloc_layout(nt, s, pblocks, coarse);
+    if (L.bytes > kNoallocScratchBytes)
+        return fail(ctx, IPH_E_INVALID,
+                    "This CDF needs " + std::to_string(L.bytes) + " bytes of scratch; the allocation-free entry point "
+                    "holds " + std::to_string(kNoallocScratchBytes) + " (a coarsened grid of at most about 1.1 M cells). "
+                    "Use iph_build_cdf.");
+    IPH_HIP(ctx, hipSetDevice(ctx->device));
+    if (!device_memory_of(d_mass, ctx->device) || !device_memory_of(d_phi, ctx->device) ||
+        !device_memory_of(d_theta, ctx->device))
+        return fail(ctx, IPH_E_INVALID, "mass, phi and theta must be device memory of the context's device.");
+    double e[kNoallocMaxTheta + 1], ce[kNoallocMaxTheta + 1];
+    if (!make_u_edges(nt, e, err)) return fail(ctx, IPH_E_INVALID, err);
+    const uint32_t dt = nt / s.ont;
+    for (uint32_t i = 0; i <= s.ont; ++i) ce[i] = e[(size_t)i * dt];
+    unsigned char* base = ctx->cdf_scratch;
+    CdfJob J;
+    std::memset(&J, 0, sizeof J);
+    J.nt = nt; J.np = np; J.ont = s.ont; J.onp = s.onp;
+    J.mass = d_mass;
+    J.edges = reinterpret_cast<double*>(base + L.edges);
+    J.cedges = reinterpret_cast<double*>(base + L.cedges);
+    J.phi = d_phi;
+    J.theta = d_theta;
+    J.cols = reinterpret_cast<double*>(base + L.cols);
+    J.scalars = reinterpret_cast<double*>(base + L.scalars);  // cleared by the first CDF kernel
+    J.partials = reinterpret_cast<double*>(base + L.partials);
+    J.coarse = coarse ? reinterpret_cast<double*>(base + L.coarse) : nullptr;
+    CdfJob* d_job = reinterpret_cast<CdfJob*>(base + L.job);
+    hipStream_t st = ctx->stream;
+    IPH_HIP(ctx, hipMemcpyAsync(base + L.edges, e, (size_t)(nt + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    IPH_HIP(ctx, hipMemcpyAsync(base + L.cedges, ce, (size_t)(s.ont + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    IPH_HIP(ctx, hipMemcpyAsync(d_job, &J, sizeof J, hipMemcpyHostToDevice, st));
+    IPH_HIP(ctx, launch_cdf(d_job, 1, s.onp, coarse, red, st));
+    double sc[kCdfScalars];
+    IPH_HIP(ctx, hipMemcpyAsync(sc, J.scalars, sizeof sc, hipMemcpyDeviceToHost, st));
+    // the host arrays above stay alive until the copies have run
+    IPH_HIP(ctx, hipStreamSynchronize(st));
     const iph_cdf_report rep = make_report(sc, s, coarse);
     if (report) *report = rep;
     if (rep.status) return fail(ctx, IPH_E_AUDIT, report_reason(rep.status));

@@ -17,6 +17,7 @@ struct iph_ctx {
     std::vector<iph_batch*> batches;  // live batches; destroyed with the context
     std::vector<iph_sur*> surrogates; // live surrogates; destroyed with the context
     std::vector<iph_atlas_mass*> atlases;  // live device atlas masses; destroyed with the context
+    unsigned char* cdf_scratch = nullptr;  // device address of this device's fixed CDF scratch (runtime.hip)
 };
 
 namespace icephase {

@@ -135,6 +135,8 @@ def lib() -> C.CDLL:
     L.iph_batch_cdf.argtypes = [C.c_void_p, u32, C.c_void_p, C.c_void_p, P(CdfReport)]
     L.iph_build_cdf.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(CdfPolicy), C.c_void_p, C.c_void_p,
                                 P(CdfReport)]
+    L.iph_build_cdf_noalloc.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(CdfPolicy), C.c_void_p, C.c_void_p,
+                                        P(CdfReport)]
     L.iph_write_record.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, u32, u32]
     L.iph_water_optics.argtypes = [f64, f64, f64, P(WaterState)]
     L.iph_water_index.argtypes = [f64, f64, f64, P(f64)]
@@ -195,7 +197,7 @@ def _live(handle: C.c_void_p, what: str) -> C.c_void_p:
 EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_destroy", "iph_last_error",
             "iph_u_edges", "iph_bin_directions", "iph_batch_create", "iph_batch_destroy", "iph_batch_run",
             "iph_batch_set_schedule", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
-            "iph_build_cdf", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
+            "iph_build_cdf", "iph_build_cdf_noalloc", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
             "iph_drop_projected_area", "iph_hex_projected_area", "iph_batch_probabilities", "iph_sur_create",
             "iph_sur_destroy", "iph_sur_param_count", "iph_sur_get_params", "iph_sur_set_params",
             "iph_sur_set_record", "iph_sur_set_target", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",

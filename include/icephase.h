@@ -205,6 +205,20 @@ int iph_batch_cdf(iph_batch* b, uint32_t rec, double* phi /* onp+1 */, double* t
 int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
                   double* phi, double* theta, iph_cdf_report* report);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the same CDF as iph_build_cdf, bitwise, from caller DEVICE arrays
+ * (d_mass nt*np, d_phi onp+1, d_theta onp*(ont+1), all on the context's device)
+ * with no allocation. Its scratch (u edges, column sums, reduction partials,
+ * report scalars, the coarsened grid) is a fixed 9 MiB device array per GPU,
+ * held until the process ends: theta_count <= 4096, and a coarsened stored grid
+ * of at most about 1.1 M cells; larger requests are refused with IPH_E_INVALID.
+ * Calls on one GPU share that array, so calls that overlap on one GPU (two
+ * contexts' streams, two threads) race; each call leaves its scratch contents
+ * behind and clears what must start at zero itself. Synchronous. */
+int iph_build_cdf_noalloc(iph_ctx* ctx, const double* d_mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
+                          double* d_phi, double* d_theta, iph_cdf_report* report);
+
 /* Writes <out_dir>.part/{phi_cdf,theta_given_phi_cdf,u_edges}.npy and
  * metadata.json, then renames to out_dir: replaces DatasetDirectory +
  * NpyFloat64Writer (src/npy_writer.cpp:18-121) and PhaseCdf::write_arrays
