@@ -25,7 +25,14 @@
  *   orc_build_cdf      uniform-sphere KAT, sparse HDR KAT, golden arrays from
  *                      the compiled reference (bitwise)
  *   orc_drop_*         sphere closed-form deflection D_p and Fresnel powers
- *                      (orc_sphere_analytic_*), FP32 vs FP64 twin agreement
+ *                      (orc_sphere_analytic_*), FP32 vs FP64 twin agreement;
+ *                      non-spherical drops against the reference's own drop
+ *                      code compiled into oracle/_ref: surface and gradient
+ *                      (RaindropShape, FP64 to 1e-14), entry/exit points and
+ *                      normals (RaindropIntersector), per-family directions,
+ *                      statuses and flux powers of R/TT/TRT/TRRT
+ *                      (RaindropPathTracer), and central differences of the
+ *                      gradient (tests/test_drop_geometry_pins.py)
  *   orc_atlas_rule     closed form: the ray areas add to the projected area;
  *                      ray origins on the lit faces; node frames and ray
  *                      counts equal the compiled reference tool's metadata
@@ -216,6 +223,25 @@ int64_t orc_drop_trace(const orc_drop_record* r, int precision, uint64_t first, 
 int orc_drop_hist(const orc_drop_record* r, int precision, uint32_t nt, uint32_t np,
                   uint32_t qbits, uint64_t first, uint64_t count,
                   uint64_t* bins, orc_hist_stats* st);
+
+/* Geometry of the drop intersector, for pinning it against the reference's
+ * RaindropShape / RaindropIntersector (include/rainbow/raindrop_shape.hpp:85-100,
+ * include/rainbow/raindrop_intersection.hpp:337-382):
+ *   orc_drop_surface: f = r - R(mu) and the outward (unnormalized) gradient at n
+ *     points (x,y,z triples);
+ *   orc_drop_entry_point: the first surface point of the line e0 u + e1 v + t ki
+ *     and the unit outward normal there (0 = miss);
+ *   orc_drop_exit_point: the exit of the interior ray from surface point p along
+ *     d and the unit outward normal there (0 = unresolved);
+ *   orc_drop_trace_origin: the full trace of the ray through e0 u + e1 v (samples
+ *     as orc_drop_trace, in emission order: R, then T at internal hit 1, 2, ...). */
+int orc_drop_surface(const orc_drop_record* r, int precision, uint64_t n, const double* pts, double* f,
+                     double* grad);
+int orc_drop_entry_point(const orc_drop_record* r, int precision, double u, double v, double p[3], double nrm[3]);
+int orc_drop_exit_point(const orc_drop_record* r, int precision, const double p[3], const double d[3], double q[3],
+                        double nrm[3]);
+int64_t orc_drop_trace_origin(const orc_drop_record* r, int precision, double u, double v, orc_sample* samples,
+                              uint64_t cap, orc_ray_audit* audit);
 
 /* Closed-form sphere: impact parameter b in [0,1), index n. Order p=0 is the
  * external reflection, p>=1 transmits after p-1 internal reflections. Returns

@@ -149,6 +149,97 @@ int64_t orc_drop_trace(const orc_drop_record* r, int precision, uint64_t first, 
     return (int64_t)n;
 }
 
+int orc_drop_surface(const orc_drop_record* r, int precision, uint64_t n, const double* pts, double* f,
+                     double* grad)
+{
+    if (precision == 32) {
+        rec_f R; load_f(r, &R);
+        for (uint64_t i = 0; i < n; ++i) {
+            const v3f p = mk_f((float)pts[3 * i], (float)pts[3 * i + 1], (float)pts[3 * i + 2]);
+            f[i] = (double)surf_f(&R, p);
+            const v3f g = grad_f(&R, p);
+            grad[3 * i] = g.x; grad[3 * i + 1] = g.y; grad[3 * i + 2] = g.z;
+        }
+    } else {
+        rec_d R; load_d(r, &R);
+        for (uint64_t i = 0; i < n; ++i) {
+            const v3d p = mk_d(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+            f[i] = surf_d(&R, p);
+            const v3d g = grad_d(&R, p);
+            grad[3 * i] = g.x; grad[3 * i + 1] = g.y; grad[3 * i + 2] = g.z;
+        }
+    }
+    return 0;
+}
+
+int orc_drop_entry_point(const orc_drop_record* r, int precision, double u, double v, double p[3], double nrm[3])
+{
+    if (precision == 32) {
+        rec_f R; load_f(r, &R);
+        const float uf = (float)u, vf = (float)v;
+        const v3f o = add_f(scl_f(R.e0, uf), scl_f(R.e1, vf));
+        float t;
+        if (!entry_f(&R, o, uf, vf, &t)) return 0;
+        const v3f q = add_f(o, scl_f(R.ki, t));
+        const v3f n = outward_normal_f(&R, q);
+        p[0] = q.x; p[1] = q.y; p[2] = q.z;
+        nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
+    } else {
+        rec_d R; load_d(r, &R);
+        const v3d o = add_d(scl_d(R.e0, u), scl_d(R.e1, v));
+        double t;
+        if (!entry_d(&R, o, u, v, &t)) return 0;
+        const v3d q = add_d(o, scl_d(R.ki, t));
+        const v3d n = outward_normal_d(&R, q);
+        p[0] = q.x; p[1] = q.y; p[2] = q.z;
+        nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
+    }
+    return 1;
+}
+
+int orc_drop_exit_point(const orc_drop_record* r, int precision, const double p[3], const double d[3], double q[3],
+                        double nrm[3])
+{
+    if (precision == 32) {
+        rec_f R; load_f(r, &R);
+        const v3f pf = mk_f((float)p[0], (float)p[1], (float)p[2]);
+        const v3f df = mk_f((float)d[0], (float)d[1], (float)d[2]);
+        float t;
+        if (!exit_f(&R, pf, df, &t)) return 0;
+        const v3f x = add_f(pf, scl_f(df, t));
+        const v3f n = outward_normal_f(&R, x);
+        q[0] = x.x; q[1] = x.y; q[2] = x.z;
+        nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
+    } else {
+        rec_d R; load_d(r, &R);
+        const v3d pd = mk_d(p[0], p[1], p[2]), dd = mk_d(d[0], d[1], d[2]);
+        double t;
+        if (!exit_d(&R, pd, dd, &t)) return 0;
+        const v3d x = add_d(pd, scl_d(dd, t));
+        const v3d n = outward_normal_d(&R, x);
+        q[0] = x.x; q[1] = x.y; q[2] = x.z;
+        nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
+    }
+    return 1;
+}
+
+int64_t orc_drop_trace_origin(const orc_drop_record* r, int precision, double u, double v, orc_sample* samples,
+                              uint64_t cap, orc_ray_audit* audit)
+{
+    uint64_t n = 0;
+    int overflow = 0;
+    orc_ray_audit a;
+    if (precision == 32) {
+        rec_f R; load_f(r, &R);
+        a = trace_origin_f(&R, 0, (float)u, (float)v, samples, cap, &n, &overflow);
+    } else {
+        rec_d R; load_d(r, &R);
+        a = trace_origin_d(&R, 0, u, v, samples, cap, &n, &overflow);
+    }
+    if (audit) *audit = a;
+    return overflow ? -1 : (int64_t)n;
+}
+
 int orc_drop_hist(const orc_drop_record* r, int precision, uint32_t nt, uint32_t np,
                   uint32_t qbits, uint64_t first, uint64_t count, uint64_t* bins, orc_hist_stats* st)
 {

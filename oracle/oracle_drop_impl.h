@@ -250,16 +250,14 @@ static FN(split_) FN(split_fn_)(const FN(ray_)* in, V n, REAL ni, REAL nt)
 
 static inline V FN(outward_normal_)(const FN(rec_)* R, V p) { return FN(unit_)(FN(grad_)(R, p)); }
 
-/* Trace one grid ray. emit(dir, power) for each escaping branch. */
-static orc_ray_audit FN(trace_ray_)(const FN(rec_)* R, uint64_t id, orc_sample* out, uint64_t cap,
-                                      uint64_t* n_out, int* overflow)
+/* Trace the ray through o = e0 u + e1 v along ki; samples carry id.
+ * emit(dir, power) for each escaping branch. */
+static orc_ray_audit FN(trace_origin_)(const FN(rec_)* R, uint64_t id, REAL u, REAL v, orc_sample* out,
+                                         uint64_t cap, uint64_t* n_out, int* overflow)
 {
     orc_ray_audit a;
     memset(&a, 0, sizeof a);
     a.status = 8; /* miss */
-    const uint32_t ix = (uint32_t)(id % R->w), iy = (uint32_t)(id / R->w);
-    const REAL u = ((REAL)(2u * ix + 1u) / (REAL)R->w - (REAL)1) * R->h;
-    const REAL v = ((REAL)(2u * iy + 1u) / (REAL)R->hgt - (REAL)1) * R->h;
     const V o = FN(add_)(FN(scl_)(R->e0, u), FN(scl_)(R->e1, v));
     REAL t;
     if (!FN(entry_)(R, o, u, v, &t)) { a.unresolved = 0; return a; }
@@ -312,6 +310,16 @@ static orc_ray_audit FN(trace_ray_)(const FN(rec_)* R, uint64_t id, orc_sample* 
     a.escaped = escaped;
     a.unresolved = (double)rem;
     return a;
+}
+
+/* Trace grid ray id: the cell centre (ix, iy) of the w x hgt grid over [-h,h]^2. */
+static orc_ray_audit FN(trace_ray_)(const FN(rec_)* R, uint64_t id, orc_sample* out, uint64_t cap,
+                                      uint64_t* n_out, int* overflow)
+{
+    const uint32_t ix = (uint32_t)(id % R->w), iy = (uint32_t)(id / R->w);
+    const REAL u = ((REAL)(2u * ix + 1u) / (REAL)R->w - (REAL)1) * R->h;
+    const REAL v = ((REAL)(2u * iy + 1u) / (REAL)R->hgt - (REAL)1) * R->h;
+    return FN(trace_origin_)(R, id, u, v, out, cap, n_out, overflow);
 }
 
 #undef V

@@ -9,6 +9,8 @@
 #include <rainbow/water_refractive_index.hpp>
 #include <rainbow/projected_area.hpp>
 #include <rainbow/raindrop_shape.hpp>
+#include <rainbow/raindrop_intersection.hpp>
+#include <rainbow/raindrop_paths.hpp>
 #include <ice_crystal/phase_generate.hpp>
 #include <ice_crystal/orientation_plan.hpp>
 
@@ -183,6 +185,113 @@ int ref_orientation_masses(double R, double L, const double k[3], double ni, dou
         *area_out = area;
         return 0;
     } catch (const std::exception&) { return -1; }
+}
+
+// ---- the reference's drop geometry and path trace (header-only), for pinning
+// the oracle's drop intersector against it
+
+static rainbow::RaindropShape drop_shape_of(const float c[8], float inner, float outer)
+{
+    rainbow::RaindropShape s{};
+    for (int i = 0; i < 8; ++i) s.coefficients[i] = c[i];
+    s.inner_radius = inner; s.outer_radius = outer;
+    return s;
+}
+
+// RaindropShape::implicit_value / gradient (include/rainbow/raindrop_shape.hpp:85-100)
+// in double (precision 64) or float (32) at n points.
+int ref_drop_surface(const float c[8], float inner, float outer, int precision, uint64_t n, const double* pts,
+                     double* f, double* grad)
+{
+    const auto s = drop_shape_of(c, inner, outer);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (precision == 64) {
+            const rainbow::Vec3T<double> p{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+            f[i] = s.implicit_value(p);
+            const auto g = s.gradient(p);
+            grad[3 * i] = g.x; grad[3 * i + 1] = g.y; grad[3 * i + 2] = g.z;
+        } else {
+            const rainbow::Vec3 p{float(pts[3 * i]), float(pts[3 * i + 1]), float(pts[3 * i + 2])};
+            f[i] = s.implicit_value(p);
+            const auto g = s.gradient(p);
+            grad[3 * i] = g.x; grad[3 * i + 1] = g.y; grad[3 * i + 2] = g.z;
+        }
+    }
+    return 0;
+}
+
+// RaindropIntersector::intersect (include/rainbow/raindrop_intersection.hpp:337-382):
+// FP32 solve, FP64 re-solve when unresolved or grazing. Returns the BoundaryCode
+// (0 miss, 1 hit, 2 unresolved) and, on a hit, position, unit outward normal,
+// distance and the fallback flag.
+int ref_drop_intersect(const float c[8], float inner, float outer, const float origin[3], const float dir[3],
+                       float start, int inside, float pos[3], float nrm[3], float* dist, int* fallback)
+{
+    const auto s = drop_shape_of(c, inner, outer);
+    const rainbow::RaindropIntersector is{s};
+    rainbow::BoundaryRay ray{};
+    ray.physical_origin = {origin[0], origin[1], origin[2]};
+    ray.direction = {dir[0], dir[1], dir[2]};
+    ray.start_distance = start;
+    ray.starts_inside = inside != 0;
+    const auto h = is.intersect(ray);
+    pos[0] = h.position.x; pos[1] = h.position.y; pos[2] = h.position.z;
+    nrm[0] = h.outward_normal.x; nrm[1] = h.outward_normal.y; nrm[2] = h.outward_normal.z;
+    *dist = h.distance;
+    *fallback = (h.diagnostics & rainbow::IntersectionFallback) != 0;
+    return static_cast<int>(h.code);
+}
+
+// detail::solve_surface<double> alone: the FP64 root distance (0x1p-44 relative
+// bracket width) of the same ray.
+int ref_drop_solve64(const float c[8], float inner, float outer, const float origin[3], const float dir[3],
+                     float start, int inside, double* dist)
+{
+    const auto s = drop_shape_of(c, inner, outer);
+    rainbow::BoundaryRay ray{};
+    ray.physical_origin = {origin[0], origin[1], origin[2]};
+    ray.direction = {dir[0], dir[1], dir[2]};
+    ray.start_distance = start;
+    ray.starts_inside = inside != 0;
+    const auto r = rainbow::detail::solve_surface<double>(s, ray);
+    *dist = r.distance;
+    return static_cast<int>(r.code);
+}
+
+// RaindropPathTracer::trace_vertex_unpolarized over a w x h vertex grid with the
+// reference's RaindropIntersector (include/rainbow/raindrop_paths.hpp:40-107):
+// per family (R, TT, TRT, TRRT; path-major, 4*w*h entries) the status, outgoing
+// direction, surface point and the two unit-input response columns (s/p field
+// components in the ray frame: x.re, x.im, y.re, y.im per column).
+int ref_drop_trace_vertices(const float c[8], float inner, float outer, const float ki[3], const float basis_x[3],
+                            float n_int, float half_extent, float reference_distance, uint32_t w, uint32_t h,
+                            uint32_t* status, float* dir, float* pos, float* col0, float* col1)
+{
+    rainbow::RaindropTraceConfig cfg{};
+    cfg.shape = drop_shape_of(c, inner, outer);
+    cfg.incident_direction = {ki[0], ki[1], ki[2]};
+    cfg.incident_basis_x = {basis_x[0], basis_x[1], basis_x[2]};
+    cfg.radius_mm = 2.0f;
+    cfg.wavelength_nm = 550.0f;
+    cfg.exterior_index = 1.0f;
+    cfg.interior_index = n_int;
+    cfg.grid_half_extent = half_extent;
+    cfg.reference_distance = cfg.outgoing_reference_distance = reference_distance;
+    cfg.grid_width = w; cfg.grid_height = h;
+    const rainbow::RaindropIntersector is{cfg.shape};
+    std::vector<rainbow::OutgoingVertex> v(std::size_t(4) * w * h);
+    std::vector<rainbow::Field32> second(v.size());
+    for (uint32_t i = 0; i < w * h; ++i)
+        rainbow::RaindropPathTracer::trace_vertex_unpolarized(cfg, i, is, v.data(), second.data());
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        status[i] = static_cast<uint32_t>(v[i].status);
+        dir[3 * i] = v[i].direction_drop.x; dir[3 * i + 1] = v[i].direction_drop.y; dir[3 * i + 2] = v[i].direction_drop.z;
+        pos[3 * i] = v[i].position_drop.x; pos[3 * i + 1] = v[i].position_drop.y; pos[3 * i + 2] = v[i].position_drop.z;
+        const rainbow::Field32 f0 = v[i].field, f1 = second[i];
+        col0[4 * i] = f0.x.real; col0[4 * i + 1] = f0.x.imag; col0[4 * i + 2] = f0.y.real; col0[4 * i + 3] = f0.y.imag;
+        col1[4 * i] = f1.x.real; col1[4 * i + 1] = f1.x.imag; col1[4 * i + 2] = f1.y.real; col1[4 * i + 3] = f1.y.imag;
+    }
+    return 0;
 }
 
 } // extern "C"

@@ -128,6 +128,11 @@ def lib() -> C.CDLL:
         L.orc_drop_trace.argtypes = [P(DropRecord), C.c_int, u64, u64, C.c_void_p, u64, C.c_void_p]
         L.orc_drop_trace.restype = i64
         L.orc_drop_hist.argtypes = [P(DropRecord), C.c_int, u32, u32, u32, u64, u64, C.c_void_p, P(HistStats)]
+        L.orc_drop_surface.argtypes = [P(DropRecord), C.c_int, u64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_drop_entry_point.argtypes = [P(DropRecord), C.c_int, f64, f64, P(f64), P(f64)]
+        L.orc_drop_exit_point.argtypes = [P(DropRecord), C.c_int, P(f64), P(f64), P(f64), P(f64)]
+        L.orc_drop_trace_origin.argtypes = [P(DropRecord), C.c_int, f64, f64, C.c_void_p, u64, C.c_void_p]
+        L.orc_drop_trace_origin.restype = i64
         L.orc_sphere_order.argtypes = [f64, f64, u32, P(f64), P(f64), P(f64)]
         L.orc_sphere_order.restype = None
         L.orc_normalize3.argtypes = [P(f64), P(f64)]
@@ -193,6 +198,12 @@ def ref() -> C.CDLL:
         R.ref_water_index.argtypes = [f64, f64, f64, P(f64), P(f64)]
         R.ref_projected_area.argtypes = [P(f64), f64, P(f64), P(f64), P(C.c_int)]
         R.ref_drop_shape.argtypes = [f32, C.c_int, P(f32), P(f32), P(f32)]
+        R.ref_drop_surface.argtypes = [P(f32), f32, f32, C.c_int, u64, C.c_void_p, C.c_void_p, C.c_void_p]
+        R.ref_drop_intersect.argtypes = [P(f32), f32, f32, P(f32), P(f32), f32, C.c_int, P(f32), P(f32), P(f32),
+                                         P(C.c_int)]
+        R.ref_drop_solve64.argtypes = [P(f32), f32, f32, P(f32), P(f32), f32, C.c_int, P(f64)]
+        R.ref_drop_trace_vertices.argtypes = [P(f32), f32, f32, P(f32), P(f32), f32, f32, f32, u32, u32,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         R.ref_orientation_masses.argtypes = [f64, f64, P(f64), f64, f64, u32, f64, C.c_char_p, u32, u32,
                                              C.c_void_p, P(f64)]
         _ref = R
@@ -438,6 +449,83 @@ def ref_orientation_masses(radius_mm, length_mm, k, n_ext, n_int, max_hits, tail
     rc = ref().ref_orientation_masses(radius_mm, length_mm, vec3(k), n_ext, n_int, max_hits, tail, csv.encode(), nt,
                                       np_, mass.ctypes.data, C.byref(area))
     return rc, mass, area.value
+
+
+# ---------------------------------------------------------------- drop geometry (oracle and reference)
+def drop_surface(r: DropRecord, pts: np.ndarray, precision: int = 64):
+    """Oracle f = r - R(mu) and outward gradient at points [n, 3]."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    n = len(pts)
+    f, g = np.zeros(n), np.zeros((n, 3))
+    assert lib().orc_drop_surface(C.byref(r), precision, n, pts.ctypes.data, f.ctypes.data, g.ctypes.data) == 0
+    return f, g
+
+
+def drop_entry_point(r: DropRecord, u: float, v: float, precision: int = 64):
+    p, n = (f64 * 3)(), (f64 * 3)()
+    hit = lib().orc_drop_entry_point(C.byref(r), precision, u, v, p, n)
+    return bool(hit), np.array(p[:]), np.array(n[:])
+
+
+def drop_exit_point(r: DropRecord, p, d, precision: int = 64):
+    q, n = (f64 * 3)(), (f64 * 3)()
+    ok = lib().orc_drop_exit_point(C.byref(r), precision, vec3(p), vec3(d), q, n)
+    return bool(ok), np.array(q[:]), np.array(n[:])
+
+
+def drop_trace_origin(r: DropRecord, u: float, v: float, precision: int = 32, cap: int = 256):
+    samples = np.zeros(cap, SAMPLE_DTYPE)
+    audit = np.zeros(1, AUDIT_DTYPE)
+    n = lib().orc_drop_trace_origin(C.byref(r), precision, u, v, samples.ctypes.data, cap, audit.ctypes.data)
+    assert n >= 0
+    return samples[:n], audit[0]
+
+
+def _ref_shape(r: DropRecord):
+    return (f32 * 8)(*r.coeffs[:]), r.inner, r.outer
+
+
+def ref_drop_surface(r: DropRecord, pts: np.ndarray, precision: int = 64):
+    """The reference's RaindropShape::implicit_value / gradient at points [n, 3]."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    n = len(pts)
+    f, g = np.zeros(n), np.zeros((n, 3))
+    c, inner, outer = _ref_shape(r)
+    assert ref().ref_drop_surface(c, inner, outer, precision, n, pts.ctypes.data, f.ctypes.data, g.ctypes.data) == 0
+    return f, g
+
+
+def ref_drop_intersect(r: DropRecord, origin, direction, start: float = 0.0, inside: bool = False):
+    """The reference's RaindropIntersector::intersect: (code, pos, normal, distance, fp64 fallback)."""
+    c, inner, outer = _ref_shape(r)
+    pos, nrm, dist, fb = (f32 * 3)(), (f32 * 3)(), f32(), C.c_int()
+    code = ref().ref_drop_intersect(c, inner, outer, (f32 * 3)(*origin), (f32 * 3)(*direction), start,
+                                    1 if inside else 0, pos, nrm, C.byref(dist), C.byref(fb))
+    return code, np.array(pos[:], np.float64), np.array(nrm[:], np.float64), dist.value, bool(fb.value)
+
+
+def ref_drop_solve64(r: DropRecord, origin, direction, start: float = 0.0, inside: bool = False):
+    c, inner, outer = _ref_shape(r)
+    d = f64()
+    code = ref().ref_drop_solve64(c, inner, outer, (f32 * 3)(*origin), (f32 * 3)(*direction), start,
+                                  1 if inside else 0, C.byref(d))
+    return code, d.value
+
+
+def ref_drop_trace_vertices(r: DropRecord, w: int, h: int, reference_distance: float | None = None):
+    """The reference's RaindropPathTracer::trace_vertex_unpolarized over a w x h
+    vertex grid (families R, TT, TRT, TRRT, path-major)."""
+    c, inner, outer = _ref_shape(r)
+    n = 4 * w * h
+    status = np.zeros(n, np.uint32)
+    d, p = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+    c0, c1 = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+    dist = 2.0 * outer if reference_distance is None else reference_distance
+    assert ref().ref_drop_trace_vertices(c, inner, outer, (f32 * 3)(*r.ki[:]), (f32 * 3)(*r.e0[:]), r.n_int,
+                                         r.half_extent, dist, w, h, status.ctypes.data, d.ctypes.data, p.ctypes.data,
+                                         c0.ctypes.data, c1.ctypes.data) == 0
+    return (status.reshape(4, h * w), d.reshape(4, h * w, 3), p.reshape(4, h * w, 3), c0.reshape(4, h * w, 4),
+            c1.reshape(4, h * w, 4))
 
 
 def oracle_env_ok() -> bool:
