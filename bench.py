@@ -13,6 +13,10 @@ metric, each measured on the device with inputs resident in HBM:
   hex        hexagonal ice column (n = 1.31, a = 50 um, c/a = 2), 2^28 rays,
              <= 12 internal interactions, 900 x 1800 cells, FP64 tracing;
   sphere     water sphere r = 1 mm at 550 nm, 2048^2 rays, <= 8 internal hits;
+  uneven     the 512-record uneven batch (spheres, Table I drops, hex prisms,
+             2^20-2^26 rays each, 900 x 1800 cells): records/s and rays/s of
+             one timed pass after one warm-up pass, with the schedule the
+             autotuner picks among three on a 1/64-ray proxy;
   surrogate  the learned surrogate (trained weights in
              ice_crystal_phase_amd/data/surrogate_drop_450x900.npy, when
              present) serving 450 x 900 CDF records, with its total variation
@@ -21,8 +25,8 @@ metric, each measured on the device with inputs resident in HBM:
   cpu_baseline_hex  the compiled reference hex tracer + compensated histogram
                     (oracle/_ref) on one host core;
   *_all_cores       the same on every host core (at most 16 processes).
---config uneven runs the 512-record uneven batch instead (records/s, with the
-autotuned schedule); --config surrogate runs the surrogate leg alone;
+--config uneven runs the uneven batch alone (records/s, with the schedule
+tuned over the full candidate grid); --config surrogate runs the surrogate leg alone;
 --config dataset times the in-process dataset driver against one
 rainbow_generate process per condition (records written to disk); --config
 atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
@@ -364,44 +368,61 @@ def cpu_baseline_hex(budget_s: float = 10.0):
                       f"PhaseMassAccumulator, 900x1800) in {dt:.1f} s"}
 
 
-def run_uneven(ctx, args):
-    """512-record uneven batch with the autotuned schedule: records/s."""
+QUICK_CANDIDATES = ((0, 2048), (8192, 2048), (32768, 4096))
+
+
+def run_uneven(ctx, warmup: int = 1, steps: int = 1, quick: bool = True):
+    """512-record uneven batch (seed 4) with the autotuned schedule: records/s
+    and rays/s over `steps` timed passes after `warmup` passes. quick: tune over
+    three schedules on a 1/64-ray proxy (the full grid of candidates otherwise).
+    Every record's quanta are checked for conservation at full size (escaped +
+    unresolved = hitting rays x 2^Q, FP32 drops to 1e-5, FP64 hex to 1e-12)."""
     import ice_crystal_phase_amd as ip
     from ice_crystal_phase_amd import autotune
-    recs = autotune.uneven_records(512, seed=4)
+    specs = autotune.uneven_specs(512, seed=4)
+    recs = [autotune.spec_record(sp) for sp in specs]
     table = []
 
     def log(m):
         table.append(m)
         print("autotune: " + m, file=sys.stderr, flush=True)
 
-    choice = autotune.tune(ctx, recs, shrink=64, log=log)
+    choice = autotune.tune(ctx, recs, shrink=64, log=log, cands=QUICK_CANDIDATES if quick else None)
     with ip.Batch(ctx, recs) as b:
         b.set_schedule(choice.tile_rays, choice.blocks)
         flags = ip.RUN_TRACE | ip.RUN_CDF | ip.RUN_GRAPH
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             b.run(flags)
             print(f"uneven warmup step {b.timing()[2]:.0f} ms", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         ms = []
-        for _ in range(args.steps):
+        for _ in range(steps):
             b.run(flags)
             ms.append(b.timing())
             print(f"uneven step {ms[-1][2]:.0f} ms", file=sys.stderr, flush=True)
-        wall = (time.perf_counter() - t0) / args.steps
+        wall = (time.perf_counter() - t0) / steps
         rays = sum(ip.record_rays(r) for r in recs)
-    step = float(np.mean([m[2] for m in ms]))
+        worst = 0.0
+        for k, sp in enumerate(specs):
+            st = b.stats(k)
+            total = float(ip.record_rays(recs[k]) - st.status[8]) * 2.0 ** st.qbits
+            rel = abs(float(st.q_escaped + st.q_unresolved) - total) / total
+            worst = max(worst, rel / (1e-12 if sp.kind == 2 else 1e-5))
+        if worst > 1:
+            raise RuntimeError(f"uneven batch: a record's quanta are not conserved ({worst:.2f} x tolerance)")
+    kinds = [sp.kind for sp in specs]
+    hex_rays = sum(ip.record_rays(r) for r, sp in zip(recs, specs) if sp.kind == 2)
     return {
-        "metric": "records/s of the uneven 512-record batch on 1 MI355X (autotuned schedule)",
-        "value": 512 / wall, "unit": "records/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 drop trace, f64 hex trace, f64 CDF", "data": "synthetic",
-        "config": {"workload": "512 records seed 4: 1/3 spheres, 1/3 oblate drops, 1/3 hex (aspect 0.2-4), "
-                               "r U[0.1,3] mm, lambda U[400,700] nm, alpha U[-90,90], rays 2^U[20,26], 900x1800"},
-        "device_ms_per_step": step, "trace_ms": float(np.mean([m[0] for m in ms])),
-        "cdf_ms": float(np.mean([m[1] for m in ms])), "rays_per_s": rays / wall, "rays": rays,
+        "records": len(recs), "records_per_s": len(recs) / wall, "rays_per_s": rays / wall, "rays": rays,
+        "hex_rays": hex_rays, "spheres_drops_hex": [kinds.count(0), kinds.count(1), kinds.count(2)],
+        "ms_per_pass": wall * 1e3, "device_ms_per_pass": float(np.mean([m[2] for m in ms])),
+        "trace_ms": float(np.mean([m[0] for m in ms])), "cdf_ms": float(np.mean([m[1] for m in ms])),
+        "warmup": warmup, "steps": steps, "conservation_worst_over_tolerance": worst,
         "schedule": {"tile_rays": choice.tile_rays, "blocks": choice.blocks, "proxy_ms": choice.ms},
         "autotune": table,
+        "config": "512 records seed 4: 1/3 spheres, 1/3 Table I drops, 1/3 hex prisms (aspect 0.2-4, n 1.31), "
+                  "r U[0.1,3] mm, lambda U[400,700] nm, alpha U[-90,90], rays 2^U[20,26], 900x1800, one "
+                  "persistent launch per kind in one hipGraph",
     }
 
 
@@ -666,11 +687,19 @@ def main() -> int:
     import ice_crystal_phase_amd as ip
 
     if args.config == "uneven":
-        # one timed pass of the batch (~9 G rays, a third of them FP64 hex rays) after one warmup pass
+        # timed passes of the batch (~8 G rays, a third of them FP64 hex rays) after warmup passes
         if args.steps == ap.get_default("steps"):
             args.steps = 1
+        if args.warmup == ap.get_default("warmup"):
+            args.warmup = 1
         with ip.Context(0) as ctx:
-            out = run_uneven(ctx, args)
+            u = run_uneven(ctx, args.warmup, args.steps, quick=False)
+        out = {"metric": "records/s of the uneven 512-record batch on 1 MI355X (autotuned schedule)",
+               "value": u["records_per_s"], "unit": "records/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": u["ms_per_pass"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32 drop trace, f64 hex trace, f64 CDF",
+               "data": "synthetic", "config": {"workload": u.pop("config")}}
+        out.update(u)
         print(json.dumps(out))
         return 0
     if args.config == "dataset":
@@ -767,6 +796,7 @@ def main() -> int:
     if not args.no_extras:
         out["hex"] = run_hex(ctx)
         out["sphere"] = run_sphere(ctx)
+        out["uneven"] = run_uneven(ctx)
         out["surrogate"] = run_surrogate(ctx)
     if not args.no_cpu_baseline:
         rps, n, dt = cpu_baseline(recs[len(recs) // 2], nt, np_)

@@ -26,12 +26,23 @@ class Choice:
     ms: float
 
 
-def uneven_records(n: int = 512, seed: int = 4, theta: int = 900, phi: int = 1800, log2_rays=(20.0, 26.0)):
-    """The uneven batch: n seeded records, one third spheres, one third oblate
-    drops (Table I shape of the radius), one third hexagonal prisms (aspect
+@dataclass
+class UnevenSpec:
+    """One record of the uneven batch: kind 0 sphere, 1 Table I drop, 2 hex prism."""
+    kind: int
+    radius_mm: float
+    wavelength_nm: float
+    alpha_deg: float
+    rays: int
+    aspect: float
+    seed: int
+
+
+def uneven_specs(n: int = 512, seed: int = 4, log2_rays=(20.0, 26.0)) -> list[UnevenSpec]:
+    """The uneven batch's draws: n seeded records, one third spheres, one third
+    oblate drops (Table I shape of the radius), one third hexagonal prisms (aspect
     L/(2R) log-uniform in [0.2, 4], n = 1.31); radius U[0.1, 3] mm, wavelength
-    U[400, 700] nm (IAPWS water index for drops), inclination U[-90, 90] deg,
-    rays floor(2^U[20, 26]) (drops: the nearest square grid)."""
+    U[400, 700] nm, inclination U[-90, 90] deg, rays floor(2^U[20, 26])."""
     rng = np.random.default_rng(seed)
     kind = rng.permutation(np.arange(n) % 3)
     r = rng.uniform(0.1, 3.0, n)
@@ -39,26 +50,32 @@ def uneven_records(n: int = 512, seed: int = 4, theta: int = 900, phi: int = 180
     alpha = rng.uniform(-90.0, 90.0, n)
     rays = np.floor(2.0 ** rng.uniform(*log2_rays, n)).astype(np.int64)
     aspect = np.exp(rng.uniform(math.log(0.2), math.log(4.0), n))
-    recs = []
-    for k in range(n):
-        if kind[k] == 2:
-            R = float(r[k])
-            L = 2 * R * float(aspect[k])
-            a = math.radians(alpha[k])
-            recs.append(icp.hex_record(R, L, 1.31, int(rays[k]), incident=(math.cos(a), -math.sin(a), 0.0),
-                                       seed=int(k), theta=theta, phi=phi, max_internal_hits=12, tail=1e-12,
-                                       wavelength_nm=float(lam[k])))
-            continue
-        grid = max(1, int(round(math.sqrt(float(rays[k])))))
-        nw = icp.water_optics(float(np.float32(lam[k]))).refractive_index
-        if kind[k] == 0:
-            recs.append(icp.sphere_record(nw, grid, theta, phi, radius_mm=float(r[k]), inclination_deg=float(alpha[k]),
-                                          max_internal_hits=9, wavelength_nm=float(lam[k])))
-        else:
-            c = icp.drop_shape(float(np.float32(r[k])))
-            recs.append(icp.drop_record(c, nw, grid, theta, phi, radius_mm=float(r[k]), inclination_deg=float(alpha[k]),
-                                        max_internal_hits=9, wavelength_nm=float(lam[k])))
-    return recs
+    return [UnevenSpec(int(kind[k]), float(r[k]), float(lam[k]), float(alpha[k]), int(rays[k]), float(aspect[k]), k)
+            for k in range(n)]
+
+
+def spec_record(s: UnevenSpec, theta: int = 900, phi: int = 1800):
+    """The library record of one spec: hex prisms trace `rays` random entries;
+    drops and spheres the nearest square grid (IAPWS water index at the FP32
+    wavelength), at most 9 internal hits; hex at most 12, tail 1e-12."""
+    if s.kind == 2:
+        a = math.radians(s.alpha_deg)
+        return icp.hex_record(s.radius_mm, 2 * s.radius_mm * s.aspect, 1.31, s.rays,
+                              incident=(math.cos(a), -math.sin(a), 0.0), seed=s.seed, theta=theta, phi=phi,
+                              max_internal_hits=12, tail=1e-12, wavelength_nm=s.wavelength_nm)
+    grid = max(1, int(round(math.sqrt(float(s.rays)))))
+    nw = icp.water_optics(float(np.float32(s.wavelength_nm))).refractive_index
+    if s.kind == 0:
+        return icp.sphere_record(nw, grid, theta, phi, radius_mm=s.radius_mm, inclination_deg=s.alpha_deg,
+                                 max_internal_hits=9, wavelength_nm=s.wavelength_nm)
+    c = icp.drop_shape(float(np.float32(s.radius_mm)))
+    return icp.drop_record(c, nw, grid, theta, phi, radius_mm=s.radius_mm, inclination_deg=s.alpha_deg,
+                           max_internal_hits=9, wavelength_nm=s.wavelength_nm)
+
+
+def uneven_records(n: int = 512, seed: int = 4, theta: int = 900, phi: int = 1800, log2_rays=(20.0, 26.0)):
+    """The uneven batch (see uneven_specs) as library records."""
+    return [spec_record(s, theta, phi) for s in uneven_specs(n, seed, log2_rays)]
 
 
 def shrink_records(recs, shrink: int):
