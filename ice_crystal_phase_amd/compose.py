@@ -48,55 +48,68 @@ GENERATOR_VERSION = "icephase.mi355x.atlas_composition.v1"
 
 # ------------------------------------------------------------------ small helpers
 def positive_int(x: Any, name: str, maximum: int = 1_000_000) -> int:
-    if type(x) is not int or not 1 <= x <= maximum:
-        raise ValueError(f"{name} must be an integer in [1,{maximum}]")
-    return x
+    """x itself when it is a JSON integer in [1, maximum]."""
+    if type(x) is int and 1 <= x <= maximum:
+        return x
+    raise ValueError(f"{name} must be an integer in [1,{maximum}]")
 
 
 def sha256_file(path: Path) -> str:
-    h = hashlib.sha256()
-    with path.open("rb") as f:
-        for chunk in iter(lambda: f.read(1 << 20), b""):
-            h.update(chunk)
-    return h.hexdigest()
+    return hashlib.sha256(path.read_bytes()).hexdigest()
 
 
 def unit(v: Any) -> np.ndarray:
-    """v / |v| for a finite nonzero three-vector (scaled first by its largest entry)."""
-    a = np.asarray(v, dtype=np.float64)
-    if a.shape != (3,) or not np.isfinite(a).all() or np.max(np.abs(a)) == 0:
+    """v / |v| for a finite nonzero three-vector; v is first divided by its
+    largest magnitude, so the norm neither overflows nor underflows."""
+    a = np.array(v, dtype=np.float64)
+    if a.shape != (3,) or not np.isfinite(a).all():
         raise ValueError("Expected finite nonzero three-vector")
-    a = a / np.max(np.abs(a))
-    return a / np.linalg.norm(a)
+    big = np.abs(a).max()
+    if big == 0:
+        raise ValueError("Expected finite nonzero three-vector")
+    a /= big
+    return a / math.sqrt(float(a @ a))
 
 
 def frame(ki: Any) -> np.ndarray:
-    """Right-handed columns [e0 e1 k]: e0 from the axis least aligned with k."""
+    """Right-handed columns [e0 e1 k]: e0 is the world axis least aligned with
+    k, made orthogonal to k; e1 = k x e0."""
     k = unit(ki)
-    axis = np.eye(3)[int(np.argmin(np.abs(k)))]
-    e0 = unit(axis - k * np.dot(k, axis))
-    return np.column_stack((e0, unit(np.cross(k, e0)), k))
+    i = int(np.argmin(np.abs(k)))
+    axis = np.zeros(3)
+    axis[i] = 1.0
+    e0 = unit(axis - k * k[i])
+    return np.stack([e0, unit(np.cross(k, e0)), k], axis=1)
+
+
+def _axis_rotation(axis: int, angle: float) -> np.ndarray:
+    """Active right-handed rotation by `angle` about world axis 0 (x), 1 (y) or 2 (z)."""
+    c, s = math.cos(angle), math.sin(angle)
+    j, k = (axis + 1) % 3, (axis + 2) % 3      # the plane the rotation turns, in cyclic order
+    m = np.eye(3)
+    m[j, j] = m[k, k] = c
+    m[k, j], m[j, k] = s, -s
+    return m
 
 
 def ry(a: float) -> np.ndarray:
-    c, s = math.cos(a), math.sin(a)
-    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+    return _axis_rotation(1, a)
 
 
 def rz(a: float) -> np.ndarray:
-    c, s = math.cos(a), math.sin(a)
-    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+    return _axis_rotation(2, a)
 
 
 def quaternion_matrix(q: Any) -> np.ndarray:
-    """Active rotation of a unit quaternion (w, x, y, z)."""
+    """Active rotation of a unit quaternion (w, x, y, z):
+    R = I + 2 w [v]x + 2 [v]x^2, v = (x, y, z)."""
     q = np.asarray(q, dtype=np.float64)
     if q.shape != (4,) or not np.isfinite(q).all() or abs(float(q @ q) - 1) > 2e-12:
         raise ValueError("Fixed quaternion must be unit wxyz")
-    w, x, y, z = q / np.linalg.norm(q)
-    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
-                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
-                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    q = q / math.sqrt(float(q @ q))
+    w, v = q[0], q[1:]
+    cross = np.array([[0.0, -v[2], v[1]], [v[2], 0.0, -v[0]], [-v[1], v[0], 0.0]])
+    return np.eye(3) + 2.0 * w * cross + 2.0 * (cross @ cross)
 
 
 # ------------------------------------------------------------------ orientation quadrature
@@ -568,79 +581,90 @@ def make_cdf(ctx: ip.Context, mass: np.ndarray, edges: np.ndarray, max_l1: float
 
 
 # ------------------------------------------------------------------ records
+def _inclination_direction(alpha: float) -> np.ndarray:
+    """ki = (cos a, -sin a, 0), with the exact axis at a = +-90 deg."""
+    if not (math.isfinite(alpha) and -90 <= alpha <= 90):
+        raise ValueError("Inclination must be in [-90,90] degrees")
+    if abs(alpha) == 90:
+        return np.array([0.0, -math.copysign(1.0, alpha), 0.0])
+    r = math.radians(alpha)
+    return unit([math.cos(r), -math.sin(r), 0.0])
+
+
 def incidents(config: dict[str, Any]) -> list[tuple[np.ndarray, dict[str, Any]]]:
-    have = set(config) & {"inclinations_deg", "directions"}
-    if len(have) != 1:
+    """The incident conditions of a composition config: `inclinations_deg`
+    (canonical directions in the x-y plane) or explicit `directions`."""
+    given = [k for k in ("inclinations_deg", "directions") if k in config]
+    if len(given) != 1:
         raise ValueError("Specify exactly one of inclinations_deg or directions")
-    out = []
-    if "inclinations_deg" in config:
-        for value in config["inclinations_deg"]:
-            a = float(value)
-            if not math.isfinite(a) or not -90 <= a <= 90:
-                raise ValueError("Inclination must be in [-90,90] degrees")
-            k = np.array([math.cos(math.radians(a)), -math.sin(math.radians(a)), 0.0])
-            if abs(a) == 90:
-                k = np.array([0.0, -math.copysign(1, a), 0.0])
-            out.append((unit(k), {"inclination_deg": a}))
+    if given[0] == "inclinations_deg":
+        out = [(_inclination_direction(float(a)), {"inclination_deg": float(a)}) for a in config["inclinations_deg"]]
     else:
-        for k in config["directions"]:
-            out.append((unit(k), {}))
-    if not out or len(out) > 10000:
+        out = [(unit(k), {}) for k in config["directions"]]
+    if not 1 <= len(out) <= 10000:
         raise ValueError("Require 1..10000 incident conditions")
     return out
 
 
-def metadata(a: Atlas, expected: float, residual: float, report: dict[str, Any], f: np.ndarray, plan_hash: str,
-             rule_meta: dict[str, Any], condition: dict[str, Any], source_hash: str,
-             azimuth_integration: dict[str, Any], output_contract: str = "rainbow") -> dict[str, Any]:
-    """metadata.json of a composed record: the keys of compose_atlas.py:63-122."""
-    g = report["g"]
-    material = dict(a.meta["material"])
-    geom = dict(a.meta["geometry"])
-    geom.update(projected_area_mm2=expected, projected_area_m2=expected * 1e-6,
-                projected_area_method="deterministic_orientation_quadrature_of_exact_facet_projection")
-    total = report["resolved_mass_mm2"]
-    quality = {k: v for k, v in report.items() if k not in ("g", "resolved_mass_mm2")}
-    quality.update(source_coverage_certified=False, angular_convergence_certified=False,
-                   orientation_convergence_certified=False, incident_atlas_convergence_certified=False,
-                   continuous_wave_solution_certified=False, maximum_accepted_l1_mass_error=1e-10,
-                   maximum_accepted_lost_probability_mass=1e-12)
-    material.setdefault("wavelength_convention", "vacuum_nm")
-    common: dict[str, Any] = dict(
-        source_repository="ice_crystal_phase_amd", source_base_commit="unknown", azimuth_periodic=True,
-        particle_frame_handedness="right", particle_up_axis=[0, 1, 0], shape_polar_axis=[0, -1, 0],
-        phi_range_rad=[-math.pi, math.pi],
-        transport_reference_frame="ensemble_reference; not an individual crystal body frame",
-        shape_polar_axis_semantics="oriented transport conditioning axis -y; body c-axis is geometry.body_long_axis")
-    if output_contract == "rainbow":
+def _canonical_incidence(k: np.ndarray) -> bool:
+    """The rainbow record contract's incident directions: (cos a, -sin a, 0), a in [-90, 90]."""
+    return abs(k[2]) <= 2e-14 and k[0] >= -2e-14
+
+
+def _record_contract_keys(a: Atlas) -> dict[str, Any]:
+    return dict(schema="rainbow.phase_cdf.numpy.v2", complete=True, generator_version=GENERATOR_VERSION,
+                backend="hip_fixed_order_atlas_convolution", dtype="<f8", order="C", theta_count=a.nt,
+                phi_count=a.np, density_measure="solid_angle_sr", cell_model="constant_density_per_spherical_cell",
+                coordinates="u=(1-cos(theta))/2; v=(phi+pi)/(2*pi)", cdf_axis_order=["phi_cell", "theta_edge"],
+                normalization="integral_p_domega_equals_one", input_polarization="unpolarized",
+                direction_convention="physical_propagation", theta_zero="forward", theta_pi="backward")
+
+
+def _frame_keys(f: np.ndarray, contract: str) -> dict[str, Any]:
+    rainbow = contract == "rainbow"
+    keys = dict(source_repository="ice_crystal_phase_amd", source_base_commit="unknown", azimuth_periodic=True,
+                particle_frame_handedness="right", particle_up_axis=[0, 1, 0], shape_polar_axis=[0, -1, 0],
+                phi_range_rad=[-math.pi, math.pi],
+                transport_reference_frame="ensemble_reference; not an individual crystal body frame",
+                shape_polar_axis_semantics="oriented transport conditioning axis -y; body c-axis is "
+                                           "geometry.body_long_axis",
+                coordinate_contract_id="rainbow.phase_cdf.coordinates.v1" if rainbow
+                else "ice.phase_cdf.ensemble_coordinates.v1",
+                sampling_frame_layout="rows_xyz_columns_e0_e1_k",
+                sampling_frame_map="local_column_to_particle_column" if rainbow
+                else "local_column_to_ensemble_reference_column",
+                sampling_frame_columns=f.tolist(), query_frame_axis=f[:, 2].tolist(), query_frame_e0=f[:, 0].tolist(),
+                nf_condition_axes=["wavelength_nm", "dot([0,-1,0],incident_propagation)"],
+                geometry_and_orientation_fixed_within_dataset=True)
+    if rainbow:
         k = f[:, 2]
-        if abs(k[2]) > 2e-14 or k[0] < -2e-14:
+        if not _canonical_incidence(k):
             raise ValueError("RainbowReference requires canonical ki=(cos(alpha),-sin(alpha),0); no implicit "
                              "rotation of a non-axisymmetric ensemble")
-        common["incident_inclination_degrees"] = math.degrees(math.atan2(-k[1], max(0.0, k[0])))
-    common["nf_condition_axes"] = ["wavelength_nm", "dot([0,-1,0],incident_propagation)"]
-    common["geometry_and_orientation_fixed_within_dataset"] = True
-    rainbow = output_contract == "rainbow"
+        keys["incident_inclination_degrees"] = math.degrees(math.atan2(-k[1], max(0.0, k[0])))
+    return keys
+
+
+def _physics_keys(a: Atlas, expected: float) -> dict[str, Any]:
+    material = {"wavelength_convention": "vacuum_nm", **a.meta["material"]}
+    geometry = {**a.meta["geometry"], "projected_area_mm2": expected, "projected_area_m2": expected * 1e-6,
+                "projected_area_method": "deterministic_orientation_quadrature_of_exact_facet_projection"}
+    mm2 = {"scattering": expected, "extinction": expected, "absorption": 0}
+    sections = {f"{k}_mm2": v for k, v in mm2.items()}
+    sections.update({f"{k}_m2": (v * 1e-6 if v else 0) for k, v in mm2.items()})
+    return dict(wavelength_nm=material["wavelength_nm"], interior_index=material["interior_index"],
+                exterior_index=material["exterior_index"], geometry=geometry, material=material,
+                cross_sections=dict(model="geometric_nonabsorbing", **sections,
+                                    wave_scattering_cross_section_computed=False),
+                single_scattering_albedo=1, stage="incoherent",
+                diffraction_policy=dict(computed=False, selected_output_includes_diffraction=False),
+                inter_path_interference=False, tir_relative_phase_retained=True)
+
+
+def _provenance_keys(a: Atlas, rule_meta: dict[str, Any], plan_hash: str, source_hash: str,
+                     azimuth_integration: dict[str, Any]) -> dict[str, Any]:
     return dict(
-        **common, schema="rainbow.phase_cdf.numpy.v2", complete=True, generator_version=GENERATOR_VERSION,
-        backend="hip_fixed_order_atlas_convolution", source_sha256=source_hash, dtype="<f8", order="C",
-        theta_count=a.nt, phi_count=a.np, density_measure="solid_angle_sr",
-        cell_model="constant_density_per_spherical_cell", coordinates="u=(1-cos(theta))/2; v=(phi+pi)/(2*pi)",
-        cdf_axis_order=["phi_cell", "theta_edge"], normalization="integral_p_domega_equals_one",
-        input_polarization="unpolarized", direction_convention="physical_propagation", theta_zero="forward",
-        theta_pi="backward",
-        coordinate_contract_id="rainbow.phase_cdf.coordinates.v1" if rainbow else "ice.phase_cdf.ensemble_coordinates.v1",
-        sampling_frame_layout="rows_xyz_columns_e0_e1_k",
-        sampling_frame_map="local_column_to_particle_column" if rainbow else "local_column_to_ensemble_reference_column",
-        sampling_frame_columns=f.tolist(), query_frame_axis=f[:, 2].tolist(), query_frame_e0=f[:, 0].tolist(),
-        wavelength_nm=material["wavelength_nm"], interior_index=material["interior_index"],
-        exterior_index=material["exterior_index"], geometry=geom, material=material,
-        cross_sections=dict(model="geometric_nonabsorbing", scattering_mm2=expected, extinction_mm2=expected,
-                            absorption_mm2=0, scattering_m2=expected * 1e-6, extinction_m2=expected * 1e-6,
-                            absorption_m2=0, wave_scattering_cross_section_computed=False),
-        single_scattering_albedo=1, stage="incoherent",
-        diffraction_policy=dict(computed=False, selected_output_includes_diffraction=False),
-        inter_path_interference=False, tir_relative_phase_retained=True,
+        source_sha256=source_hash,
         orientation=dict(representation="deterministic_SO3_number_quadrature", rule=rule_meta,
                          model_parameter_policy="explicit_user_parameters_not_aerodynamic_prediction",
                          plan_sha256=plan_hash, randomness=False, plan_embedded=False, plan_location="dataset_root",
@@ -649,16 +673,66 @@ def metadata(a: Atlas, expected: float, residual: float, report: dict[str, Any],
                    angular_reconstruction="A_normalized_bilinear_theta_phi_in_incident_spherical_frame",
                    symmetry="D6h_including_reflections", transform="phi_shift_or_phi_reflection_then_shift",
                    azimuth_remap="exact_overlap_of_piecewise_constant_equal_width_cells",
-                   analytical_average_of_continuous_optics=False, no_retracing=True),
-        transport_audit=dict(escaped_mm2=total, unresolved_mm2=residual, unresolved_fraction=residual / expected,
-                             balance_error=(total + residual) / expected - 1,
+                   analytical_average_of_continuous_optics=False, no_retracing=True))
+
+
+def _result_keys(expected: float, residual: float, report: dict[str, Any], f: np.ndarray) -> dict[str, Any]:
+    g, escaped = report["g"], report["resolved_mass_mm2"]
+    quality = {k: v for k, v in report.items() if k not in ("g", "resolved_mass_mm2")}
+    certified = ("source_coverage", "angular_convergence", "orientation_convergence", "incident_atlas_convergence",
+                 "continuous_wave_solution")
+    quality.update({f"{k}_certified": False for k in certified})
+    quality.update(maximum_accepted_l1_mass_error=1e-10, maximum_accepted_lost_probability_mass=1e-12)
+    return dict(
+        transport_audit=dict(escaped_mm2=escaped, unresolved_mm2=residual, unresolved_fraction=residual / expected,
+                             balance_error=(escaped + residual) / expected - 1,
                              cdf_normalized_to_resolved_escaped_mass=True,
                              residual_estimator="same_covariant_incident_interpolation_as_response"),
         storage_processing=dict(gaussian_sigma_degrees=0, coarsening_tv=0, mass_conservative_aggregation=False,
                                 g_source=g, g_stored=g),
         hg=dict(g=g, axis_particle_frame=f[:, 2].tolist(), method="first_moment_of_saved_cell_pdf",
                 target="saved_cdf", cosine_convention="dot(incident_propagation,outgoing_propagation)"),
-        condition=condition, quality=quality)
+        quality=quality)
+
+
+def metadata(a: Atlas, expected: float, residual: float, report: dict[str, Any], f: np.ndarray, plan_hash: str,
+             rule_meta: dict[str, Any], condition: dict[str, Any], source_hash: str,
+             azimuth_integration: dict[str, Any], output_contract: str = "rainbow") -> dict[str, Any]:
+    """metadata.json of a composed record (the key set of the reference
+    composer's records, hex_atlas/python/compose_atlas.py:63-122), assembled
+    from its sections: record contract, frame, physics, provenance, results."""
+    meta = _frame_keys(f, output_contract)
+    meta.update(_record_contract_keys(a))
+    meta.update(_physics_keys(a, expected))
+    meta.update(_provenance_keys(a, rule_meta, plan_hash, source_hash, azimuth_integration))
+    meta.update(_result_keys(expected, residual, report, f))
+    meta["condition"] = condition
+    return meta
+
+
+def _check_batch(atlases: list[Atlas]) -> list[float]:
+    """Atlases composed in one run must differ only in wavelength (and the
+    index it implies): same geometry, grids, areas, exterior medium, material
+    model and transport policy. Returns their wavelengths."""
+    if not atlases:
+        raise ValueError("At least one atlas is required")
+    ref = atlases[0]
+    for a in atlases[1:]:
+        same_grid = (a.it, a.ip, a.nt, a.np) == (ref.it, ref.ip, ref.nt, ref.np)
+        if not (same_grid and a.meta["geometry"] == ref.meta["geometry"] and np.array_equal(a.edges, ref.edges)
+                and np.array_equal(a.area, ref.area)):
+            raise ValueError("Batch atlases must share geometry, angular grids, and areas")
+        if any(a.meta["material"].get(k) != ref.meta["material"].get(k)
+               for k in ("model", "exterior_index", "absorption", "birefringence")):
+            raise ValueError("A spectral batch must use one material model and exterior medium")
+        if a.meta["transport"] != ref.meta["transport"]:
+            raise ValueError("A spectral batch must use one transport/audit policy")
+    waves = [float(a.meta["material"]["wavelength_nm"]) for a in atlases]
+    if len(set(waves)) != len(waves) or not all(math.isfinite(w) and w > 0 for w in waves):
+        raise ValueError("Unique positive wavelength per atlas required")
+    if (ref.it + 1) * (ref.ip + 1) * 2 * ref.np * 8 > 512 * 1024 ** 2:
+        raise ValueError("Coupling buffer would exceed the documented 512 MiB bound")
+    return waves
 
 
 def _write_json(path: Path, value: Any) -> None:
@@ -684,46 +758,27 @@ def _staging(out: Path) -> Path:
 def run(atlas_paths: list[Path], config_path: Path, out: Path, output_contract: str = "rainbow",
         device: int = 0) -> None:
     config = json.loads(config_path.read_text(encoding="utf-8"))
-    if set(config) - {"orientation", "inclinations_deg", "directions"} or "orientation" not in config:
+    if "orientation" not in config or not set(config) <= {"orientation", "inclinations_deg", "directions"}:
         raise ValueError("Unknown/missing composition configuration keys")
     incident_list = incidents(config)
     atlases: list[Atlas] = []
     part = None
     ctx = None
     try:
-        for path in atlas_paths:
-            atlases.append(Atlas(path))
-        if not atlases:
-            raise ValueError("At least one atlas is required")
-        first = atlases[0]
-        geometry = first.meta["geometry"]
+        atlases = [Atlas(path) for path in atlas_paths]
+        first = atlases[0] if atlases else None
         rotations, weights, rule_meta = orientation_rule(config["orientation"])
-        rule_meta = dict(rule_meta, vertical_uniform_azimuth_policy={
+        rule_meta = {**rule_meta, "vertical_uniform_azimuth_policy": {
             "version": "analytic_uniform_alpha_at_exact_vertical_v1",
             "reference_arrays": "discrete_reference_not_literal_nodes_at_analytic_conditions",
             "eligible_components": ["horizontal_gaussian", "isotropic"], "fixed_components_unchanged": True,
-            "nonvertical_conditions_unchanged": True})
+            "nonvertical_conditions_unchanged": True}}
         if output_contract not in ("rainbow", "ensemble"):
             raise ValueError("Unknown output contract")
-        if output_contract == "rainbow":
-            for k, _ in incident_list:
-                if abs(k[2]) > 2e-14 or k[0] < -2e-14:
-                    raise ValueError("Noncanonical incident direction is not RainbowReference-compatible; select "
-                                     "--output-contract ensemble explicitly")
-        for a in atlases[1:]:
-            if ((a.it, a.ip, a.nt, a.np) != (first.it, first.ip, first.nt, first.np) or a.meta["geometry"] != geometry
-                    or not np.array_equal(a.edges, first.edges) or not np.array_equal(a.area, first.area)):
-                raise ValueError("Batch atlases must share geometry, angular grids, and areas")
-            if any(a.meta["material"].get(k) != first.meta["material"].get(k)
-                   for k in ("model", "exterior_index", "absorption", "birefringence")):
-                raise ValueError("A spectral batch must use one material model and exterior medium")
-            if a.meta["transport"] != first.meta["transport"]:
-                raise ValueError("A spectral batch must use one transport/audit policy")
-        wavelengths = [float(a.meta["material"]["wavelength_nm"]) for a in atlases]
-        if len(set(wavelengths)) != len(wavelengths) or any(not math.isfinite(w) or w <= 0 for w in wavelengths):
-            raise ValueError("Unique positive wavelength per atlas required")
-        if (first.it + 1) * (first.ip + 1) * 2 * first.np * 8 > 512 * 1024 ** 2:
-            raise ValueError("Coupling buffer would exceed the documented 512 MiB bound")
+        if output_contract == "rainbow" and not all(_canonical_incidence(k) for k, _ in incident_list):
+            raise ValueError("Noncanonical incident direction is not RainbowReference-compatible; select "
+                             "--output-contract ensemble explicitly")
+        wavelengths = _check_batch(atlases)
         part = _staging(out)
         _write_json(part / "orientation_model.json", rule_meta)
         _write_npy(part / "orientation_rotations.npy", rotations)

@@ -29,7 +29,6 @@ Run as `python -m ice_crystal_phase_amd.dataset --config CONFIG [--execute]`.
 from __future__ import annotations
 
 import argparse
-import ast
 import ctypes as C
 import hashlib
 import json
@@ -38,6 +37,7 @@ import os
 import shutil
 import struct
 import sys
+import tempfile
 import time
 from concurrent.futures import ThreadPoolExecutor
 from contextlib import contextmanager
@@ -68,16 +68,28 @@ MAX_BATCH_RECORDS = 256
 
 
 # ------------------------------------------------------------------ config values
-def _reject_constant(token: str) -> None:
-    raise ValueError(f"Nonfinite JSON constant: {token}")
+# Numbers in a config are read as the exact decimal they spell (JSON numbers and
+# numeric strings alike), so axis endpoints and steps are exact and a plan
+# written from them is reproducible. The limits below are the dataset
+# contract's (tools/generate_phase_dataset.py).
+_SOLVER_INTEGERS = {"grid": (2, 32767), "query_theta": (1, 32767), "query_phi": (1, 32767),
+                    "cdf_theta": (1, 32767), "cdf_phi": (1, 32767), "device": (0, 128)}
+_SOLVER_FRACTIONS = ("maximum_coarsening_tv", "cdf_max_l1", "cdf_max_lost_mass")   # each in [0, 1]
+_CONDITION_SCALARS = {"radius_mm": (1e-9, 3), "temperature_c": (0, 60), "pressure_pa": (50000, 100000000)}
+WAVELENGTH_KEYS = frozenset({"start_nm", "stop_nm", "step_nm"})
+INCLINATION_KEYS = frozenset({"min_degrees", "max_degrees", "count", "placement"})
 
 
 def load_json(path: Path, limit: int = 8 << 20) -> dict[str, Any]:
-    """A JSON object from a file of bounded size; NaN/Infinity are rejected."""
+    """A JSON object from a file of at most `limit` bytes (a UTF-8 BOM is
+    allowed); NaN and Infinity are refused."""
     if path.stat().st_size > limit:
         raise ValueError(f"JSON too large: {path}")
-    with path.open("r", encoding="utf-8-sig") as f:
-        value = json.load(f, parse_constant=_reject_constant)
+
+    def refuse(token: str) -> Any:
+        raise ValueError(f"Nonfinite JSON constant: {token}")
+
+    value = json.loads(path.read_text(encoding="utf-8-sig"), parse_constant=refuse)
     if not isinstance(value, dict):
         raise ValueError(f"Expected JSON object: {path}")
     return value
@@ -98,135 +110,150 @@ def exact(value: Any) -> Decimal:
 
 def scalar(value: Any, low: float, high: float) -> float:
     d = exact(value)
-    if not Decimal(str(low)) <= d <= Decimal(str(high)):
+    if d < Decimal(str(low)) or d > Decimal(str(high)):
         raise ValueError(f"Parameter {value} is outside [{low}, {high}]")
     return float(d)
 
 
 def integer(value: Any, low: int, high: int) -> int:
     d = exact(value)
-    if d != d.to_integral_value() or not low <= d <= high:
+    whole = d == d.to_integral_value()
+    if not whole or d < low or d > high:
         raise ValueError(f"Expected integer in [{low}, {high}]")
     return int(d)
 
 
 def flag(value: Any) -> bool:
-    if type(value) is not bool:
-        raise ValueError("Expected JSON true or false")
-    return value
+    if value is True or value is False:
+        return value
+    raise ValueError("Expected JSON true or false")
 
 
 def fp32(x: float) -> float:
     """x rounded to the nearest single-precision value (the solver's precision)."""
-    return struct.unpack("<f", struct.pack("<f", x))[0]
+    return float(np.float32(x))
+
+
+def _text(d: Decimal) -> str:
+    return format(d, "f")
 
 
 def wavelengths(spec: dict[str, Any]) -> list[str]:
     """Inclusive exact-step axis start, start + step, ..., stop, as decimal strings."""
-    if set(spec) != {"start_nm", "stop_nm", "step_nm"}:
+    if set(spec) != WAVELENGTH_KEYS:
         raise ValueError("wavelengths needs start_nm, stop_nm, step_nm")
-    start, stop, step = (exact(spec[k]) for k in ("start_nm", "stop_nm", "step_nm"))
-    if not (380 <= start <= stop <= 830 and step > 0):
+    start, stop, step = exact(spec["start_nm"]), exact(spec["stop_nm"]), exact(spec["step_nm"])
+    if step <= 0 or not (380 <= start <= stop <= 830):
         raise ValueError("Require 380 <= start <= stop <= 830 and positive step")
-    steps = (stop - start) / step
-    if steps != steps.to_integral_value():
+    n_steps, remainder = divmod(stop - start, step)
+    if remainder != 0:
         raise ValueError("The wavelength range must contain an integer number of steps; "
                          "no silent endpoint truncation")
-    count = int(steps) + 1
-    if count > 100000:
+    if n_steps + 1 > 100000:
         raise ValueError("Too many wavelengths")
-    values = [format(start + step * k, "f") for k in range(count)]
-    if len({fp32(float(v)) for v in values}) != count:
+    axis = [_text(start + step * k) for k in range(int(n_steps) + 1)]
+    solver_values = np.array([float(v) for v in axis], dtype=np.float32)
+    if np.unique(solver_values).size != len(axis):
         raise ValueError("Distinct wavelengths collapse to the same FP32 solver value")
-    return values
+    return axis
 
 
 def inclinations(spec: dict[str, Any]) -> list[str]:
-    """Cell centres of `count` equal angle cells over [min, max], as decimal strings."""
-    if set(spec) != {"min_degrees", "max_degrees", "count", "placement"}:
+    """Cell centres of `count` equal angle cells over [min, max], as decimal strings:
+    min + (2k + 1) (max - min) / (2 count)."""
+    if set(spec) != INCLINATION_KEYS:
         raise ValueError("inclinations needs min_degrees, max_degrees, count, placement")
     if spec["placement"] != "linear_angle_cell_centers":
         raise ValueError("Only explicit linear_angle_cell_centers is supported")
     low, high = exact(spec["min_degrees"]), exact(spec["max_degrees"])
     count = integer(spec["count"], 1, 100000)
-    if not -90 <= low < high <= 90:
+    if not (-90 <= low < high <= 90):
         raise ValueError("Canonical inclination domain is [-90,90], NOT [0,180]")
-    width = high - low
-    return [format(low + width * (Decimal(k) + Decimal("0.5")) / count, "f") for k in range(count)]
+    half_cell = (high - low) / (2 * count)
+    return [_text(low + half_cell * (2 * k + 1)) for k in range(count)]
 
 
-def normalized_config(raw: dict[str, Any]) -> dict[str, Any]:
-    """Validated conditions of a dataset config (the plan's `conditions`)."""
-    if set(raw) != CONFIG_KEYS:
-        raise ValueError(f"Unexpected/missing config keys: {set(raw) ^ CONFIG_KEYS}")
-    s = raw["solver"]
+def _solver(s: Any) -> dict[str, Any]:
     if not isinstance(s, dict) or set(s) != SOLVER_KEYS:
         raise ValueError("Invalid solver keys")
-    out: dict[str, Any] = {}
-    for k in ("grid", "query_theta", "query_phi", "cdf_theta", "cdf_phi"):
-        out[k] = integer(s[k], 2 if k == "grid" else 1, 32767)
-    if out["grid"] ** 2 * 4 > 0xFFFFFFFF:
+    out: dict[str, Any] = {k: integer(s[k], lo, hi) for k, (lo, hi) in _SOLVER_INTEGERS.items() if k != "device"}
+    if 4 * out["grid"] * out["grid"] > 0xFFFFFFFF:
         raise ValueError("Source index range exceeds the existing solver contract")
-    for axis in ("theta", "phi"):
-        q, c = out["query_" + axis], out["cdf_" + axis]
-        if c > q or q % c:
-            raise ValueError("CDF grids must divide query grids")
+    if any(out["cdf_" + a] > out["query_" + a] or out["query_" + a] % out["cdf_" + a] for a in ("theta", "phi")):
+        raise ValueError("CDF grids must divide query grids")
     if s["stage"] not in STAGES:
         raise ValueError("Unknown stage")
     if s["stage"] != "incoherent":
         raise ValueError(f"Stage '{s['stage']}' (coherent path/focal phases, Table II diffraction) is not built; "
                          "set solver.stage to incoherent.")
     out["stage"] = s["stage"]
-    out["device"] = integer(s["device"], 0, 128)
+    out["device"] = integer(s["device"], *_SOLVER_INTEGERS["device"])
     out["allow_underresolved"] = flag(s["allow_underresolved"])
-    out["maximum_coarsening_tv"] = scalar(s["maximum_coarsening_tv"], 0, 1)
-    out["cdf_max_l1"] = scalar(s["cdf_max_l1"], 0, 1)
-    out["cdf_max_lost_mass"] = scalar(s["cdf_max_lost_mass"], 0, 1)
+    for k in _SOLVER_FRACTIONS:
+        out[k] = scalar(s[k], 0, 1)
     offsets = s["focal_offsets"]
-    if not isinstance(offsets, list) or len(offsets) != 4:
+    if not (isinstance(offsets, list) and len(offsets) == 4):
         raise ValueError("Four focal offsets required")
     out["focal_offsets"] = [integer(v, 0, 3) for v in offsets]
-    return {"radius_mm": scalar(raw["radius_mm"], 1e-9, 3), "force_sphere": flag(raw["force_sphere"]),
-            "temperature_c": scalar(raw["temperature_c"], 0, 60),
-            "pressure_pa": scalar(raw["pressure_pa"], 50000, 100000000),
-            "wavelength_nm": wavelengths(raw["wavelengths"]),
-            "inclination_degrees": inclinations(raw["inclinations"]),
-            "inclination_placement": "linear_angle_cell_centers", "material": "water", "solver": out}
+    return out
+
+
+def normalized_config(raw: dict[str, Any]) -> dict[str, Any]:
+    """Validated conditions of a dataset config (the plan's `conditions`)."""
+    if set(raw) != CONFIG_KEYS:
+        raise ValueError(f"Unexpected/missing config keys: {set(raw) ^ CONFIG_KEYS}")
+    solver = _solver(raw["solver"])
+    values = {k: scalar(raw[k], lo, hi) for k, (lo, hi) in _CONDITION_SCALARS.items()}
+    return {"radius_mm": values["radius_mm"], "force_sphere": flag(raw["force_sphere"]),
+            "temperature_c": values["temperature_c"], "pressure_pa": values["pressure_pa"],
+            "wavelength_nm": wavelengths(raw["wavelengths"]), "inclination_degrees": inclinations(raw["inclinations"]),
+            "inclination_placement": "linear_angle_cell_centers", "material": "water", "solver": solver}
 
 
 # ------------------------------------------------------------------ files
 def sha256(path: Path) -> str:
-    h = hashlib.sha256()
+    """SHA-256 of a file's bytes, read in 1 MiB pieces."""
+    digest = hashlib.sha256()
     with path.open("rb") as f:
-        for chunk in iter(lambda: f.read(1 << 20), b""):
-            h.update(chunk)
-    return h.hexdigest()
+        while piece := f.read(1 << 20):
+            digest.update(piece)
+    return digest.hexdigest()
 
 
 def object_hash(value: Any) -> str:
-    text = json.dumps(value, sort_keys=True, separators=(",", ":"), allow_nan=False)
-    return hashlib.sha256(text.encode("utf-8")).hexdigest()
+    """SHA-256 of the canonical JSON text of `value` (sorted keys, no spaces)."""
+    canonical = json.dumps(value, sort_keys=True, separators=(",", ":"), allow_nan=False)
+    return hashlib.sha256(canonical.encode("utf-8")).hexdigest()
 
 
 def write_json(path: Path, value: Any) -> None:
-    """Write through a temporary file and rename, so a reader never sees half a file."""
+    """Write through a temporary file in the same directory and rename it over
+    the target, so a reader sees the old file or the new one, never part of one."""
     path.parent.mkdir(parents=True, exist_ok=True)
-    tmp = path.with_suffix(path.suffix + ".tmp")
-    with tmp.open("w", encoding="utf-8", newline="\n") as f:
-        json.dump(value, f, indent=2, ensure_ascii=False, allow_nan=False)
-        f.write("\n")
-    tmp.replace(path)
+    text = json.dumps(value, indent=2, ensure_ascii=False, allow_nan=False) + "\n"
+    fd, tmp = tempfile.mkstemp(prefix=path.name + ".", suffix=".tmp", dir=path.parent)
+    try:
+        with os.fdopen(fd, "w", encoding="utf-8", newline="\n") as f:
+            f.write(text)
+        os.replace(tmp, path)
+    except BaseException:
+        Path(tmp).unlink(missing_ok=True)
+        raise
 
 
 @contextmanager
 def batch_lock(root: Path) -> Iterator[None]:
+    """Exclusive writer lock `root/.batch.lock`, created atomically and removed
+    on exit; an existing lock (another writer, or a crashed one) is never taken
+    over."""
     lock = root / ".batch.lock"
     try:
-        with lock.open("x", encoding="ascii") as f:
-            f.write(f"pid={os.getpid()}\n")
+        fd = os.open(lock, os.O_CREAT | os.O_EXCL | os.O_WRONLY, 0o644)
     except FileExistsError as exc:
         raise RuntimeError("Batch lock exists. Do not run two writers. After verifying that no batch is running, "
                            "remove the stale .batch.lock manually.") from exc
+    with os.fdopen(fd, "w", encoding="ascii") as f:
+        f.write(f"pid={os.getpid()}\n")
     try:
         yield
     finally:
@@ -234,30 +261,22 @@ def batch_lock(root: Path) -> Iterator[None]:
 
 
 def npy_size(path: Path, shape: tuple[int, ...]) -> int:
-    """Size of a '<f8' C-order NPY file of the given shape, from its header and
-    length only (no data read)."""
+    """Byte size of a '<f8' C-order NPY file of the given shape, checked from its
+    header and the file length only (the data is not read)."""
+    from numpy.lib import format as npy
     with path.open("rb") as f:
-        if f.read(6) != b"\x93NUMPY":
-            raise ValueError(f"Not NPY: {path}")
-        version = f.read(2)
-        if version == b"\x01\x00":
-            size_field, fmt = 2, "<H"
-        elif version == b"\x02\x00":
-            size_field, fmt = 4, "<I"
-        else:
+        try:
+            version = npy.read_magic(f)
+        except ValueError as exc:
+            raise ValueError(f"Not NPY: {path}") from exc
+        if version not in ((1, 0), (2, 0)):
             raise ValueError("Unsupported NPY version")
-        raw = f.read(size_field)
-        if len(raw) != size_field:
-            raise ValueError("Truncated NPY")
-        hlen = struct.unpack(fmt, raw)[0]
-        if not 0 < hlen <= 65536:
-            raise ValueError("Invalid NPY header size")
-        text = f.read(hlen)
-        if len(text) != hlen:
-            raise ValueError("Truncated NPY header")
-        header = ast.literal_eval(text.decode("latin1"))
-        if (not isinstance(header, dict) or header.get("descr") != "<f8"
-                or header.get("fortran_order") is not False or header.get("shape") != shape):
+        reader = npy.read_array_header_1_0 if version == (1, 0) else npy.read_array_header_2_0
+        try:
+            stored_shape, fortran, dtype = reader(f, max_header_size=65536)
+        except ValueError as exc:
+            raise ValueError(f"Wrong NPY layout: {path}") from exc
+        if dtype != np.dtype("<f8") or fortran or tuple(stored_shape) != tuple(shape):
             raise ValueError(f"Wrong NPY layout: {path}")
         expected = f.tell() + 8 * math.prod(shape)
     if path.stat().st_size != expected:
@@ -273,82 +292,98 @@ def engine_files() -> dict[str, Path]:
 
 
 # ------------------------------------------------------------------ record checks
-def _close(a: Any, b: float, tol: float = 1e-12) -> bool:
-    return type(a) in (int, float) and math.isfinite(a) and abs(a - b) <= tol * max(1.0, abs(b))
+def _number(x: Any) -> bool:
+    return type(x) in (int, float) and math.isfinite(x)
+
+
+def _near(x: Any, want: float, tol: float = 1e-12) -> bool:
+    return _number(x) and abs(x - want) <= tol * max(1.0, abs(want))
+
+
+def _record_checks(m: dict[str, Any], c: dict[str, Any], ii: int, ll: int) -> list[tuple[str, bool]]:
+    """(label, passed) for every rule a record of condition (ii, ll) obeys."""
+    s = c["solver"]
+    get = lambda dotted: _dig(m, dotted)   # noqa: E731
+    expected = {"complete": True, "schema": "rainbow.phase_cdf.numpy.v2", "generator_version": GENERATOR_VERSION,
+                "coordinate_contract_id": "rainbow.phase_cdf.coordinates.v1",
+                "direction_convention": "physical_propagation", "dtype": "<f8", "order": "C",
+                "density_measure": "solid_angle_sr", "cdf_axis_order": ["phi_cell", "theta_edge"],
+                "theta_count": s["cdf_theta"], "phi_count": s["cdf_phi"],
+                "query_grid": [s["query_theta"], s["query_phi"]], "incident_grid": [s["grid"], s["grid"]],
+                "stage": s["stage"], "force_sphere": c["force_sphere"],
+                "focal_quarter_turn_offsets": s["focal_offsets"]}
+    checks = [(f"mismatched {k}", m.get(k) == v) for k, v in expected.items()]
+    alpha = float(c["inclination_degrees"][ii])
+    checks += [
+        ("radius", _near(m.get("radius_mm"), fp32(c["radius_mm"]))),
+        ("wavelength", _near(m.get("wavelength_nm"), fp32(float(c["wavelength_nm"][ll])))),
+        ("inclination", _near(m.get("incident_inclination_degrees"), alpha)),
+        ("material", get("material.model") == "water_iapws_r9_97"),
+        ("temperature", _near(get("material.temperature_kelvin"), c["temperature_c"] + 273.15)),
+        ("pressure", _near(get("material.pressure_pascal"), c["pressure_pa"])),
+    ]
+    frame = m.get("sampling_frame_columns")
+    shaped = isinstance(frame, list) and len(frame) == 3 and all(isinstance(r, list) and len(r) == 3 for r in frame)
+    checks.append(("frame shape", shaped))
+    if shaped:
+        finite = all(_number(v) for row in frame for v in row)
+        checks.append(("frame values", finite))
+        if finite:
+            F = np.array(frame, dtype=np.float64)          # columns e0, e1, ki
+            checks.append(("frame orthogonality", bool(np.all(np.abs(F.T @ F - np.eye(3)) <= 2e-12))))
+            checks.append(("frame handedness", float(np.linalg.det(F)) > 1 - 2e-12))
+            a = math.radians(alpha)
+            ki = np.array([math.cos(a), -math.sin(a), 0.0])
+            checks.append(("incident direction/sign", bool(np.all(np.abs(F[:, 2] - ki) < 2e-6))))
+    area = get("geometry.projected_area_mm2")
+    checks += [
+        ("projected area", _number(area) and area > 0),
+        ("area validation", get("geometry.projected_area_converged") is True
+         and get("geometry.convexity_verified") is True),
+        ("cross-section model", get("cross_sections.model") == "geometric_nonabsorbing"
+         and get("cross_sections.wave_scattering_cross_section_computed") is False),
+        ("cross sections", get("cross_sections.scattering_mm2") == area and get("cross_sections.extinction_mm2") == area
+         and get("cross_sections.absorption_mm2") == 0),
+        ("invalid optical data", get("quality.invalid_values") == 0 and get("quality.incomplete_directions") == 0),
+        ("underresolution policy", get("quality.allow_underresolved") is s["allow_underresolved"]),
+        ("underresolution not authorized", s["allow_underresolved"] or get("quality.underresolved_directions") == 0),
+    ]
+    for key, limit in (("cdf_l1_mass_error", "cdf_max_l1"), ("cdf_lost_probability_mass", "cdf_max_lost_mass")):
+        v = get("quality." + key)
+        checks.append((key, _number(v) and 0 <= v <= s[limit]))
+    tv = get("storage_processing.coarsening_tv")
+    g = get("hg.g")
+    checks += [
+        ("unexpected global smoothing", get("storage_processing.additional_filter") == "none"),
+        ("coarsening error", _number(tv) and 0 <= tv <= s["maximum_coarsening_tv"]),
+        ("HG target", get("hg.target") == "saved_cdf" and _number(g) and -1 < g < 1),
+    ]
+    return checks
+
+
+def _dig(tree: Any, dotted: str) -> Any:
+    for part in dotted.split("."):
+        if not isinstance(tree, dict):
+            return None
+        tree = tree.get(part)
+    return tree
 
 
 def verify_record(path: Path, plan: dict[str, Any], ii: int, ll: int) -> dict[str, Any]:
     """Check a written record against its planned condition (metadata contract,
     frame, policy limits, NPY headers and sizes) and return its receipt body."""
     c = plan["conditions"]
-    s = c["solver"]
     m = load_json(path / "metadata.json", 1 << 20)
-
-    def require(ok: bool, what: str) -> None:
+    for label, ok in _record_checks(m, c, ii, ll):
         if not ok:
-            raise ValueError(f"{path}: {what}")
-
-    fixed = {"complete": True, "schema": "rainbow.phase_cdf.numpy.v2", "generator_version": GENERATOR_VERSION,
-             "coordinate_contract_id": "rainbow.phase_cdf.coordinates.v1",
-             "direction_convention": "physical_propagation", "dtype": "<f8", "order": "C",
-             "density_measure": "solid_angle_sr", "cdf_axis_order": ["phi_cell", "theta_edge"],
-             "theta_count": s["cdf_theta"], "phi_count": s["cdf_phi"],
-             "query_grid": [s["query_theta"], s["query_phi"]], "incident_grid": [s["grid"], s["grid"]],
-             "stage": s["stage"], "force_sphere": c["force_sphere"], "focal_quarter_turn_offsets": s["focal_offsets"]}
-    for key, want in fixed.items():
-        require(m.get(key) == want, f"mismatched {key}")
-    require(_close(m.get("radius_mm"), fp32(c["radius_mm"])), "radius")
-    require(_close(m.get("wavelength_nm"), fp32(float(c["wavelength_nm"][ll]))), "wavelength")
-    alpha = float(c["inclination_degrees"][ii])
-    require(_close(m.get("incident_inclination_degrees"), alpha), "inclination")
-    mat = m.get("material", {})
-    require(mat.get("model") == "water_iapws_r9_97", "material")
-    require(_close(mat.get("temperature_kelvin"), c["temperature_c"] + 273.15), "temperature")
-    require(_close(mat.get("pressure_pascal"), c["pressure_pa"]), "pressure")
-    frame = m.get("sampling_frame_columns", [])
-    require(isinstance(frame, list) and len(frame) == 3
-            and all(isinstance(row, list) and len(row) == 3 for row in frame), "frame shape")
-    require(all(type(v) in (int, float) and math.isfinite(v) for row in frame for v in row), "frame values")
-    cols = [[frame[r][j] for r in range(3)] for j in range(3)]
-    for j in range(3):
-        for k in range(3):
-            dot = sum(cols[j][r] * cols[k][r] for r in range(3))
-            require(_close(dot, float(j == k), 2e-12), "frame orthogonality")
-    e0, e1, kk = cols
-    cross = [e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]]
-    require(sum(cross[r] * kk[r] for r in range(3)) > 1 - 2e-12, "frame handedness")
-    a = math.radians(alpha)
-    ki = (math.cos(a), -math.sin(a), 0.0)
-    require(all(abs(kk[r] - ki[r]) < 2e-6 for r in range(3)), "incident direction/sign")
-    geometry, cs = m.get("geometry", {}), m.get("cross_sections", {})
-    area = geometry.get("projected_area_mm2")
-    require(type(area) in (int, float) and math.isfinite(area) and area > 0, "projected area")
-    require(geometry.get("projected_area_converged") is True and geometry.get("convexity_verified") is True,
-            "area validation")
-    require(cs.get("model") == "geometric_nonabsorbing" and cs.get("wave_scattering_cross_section_computed") is False,
-            "cross-section model")
-    require(cs.get("scattering_mm2") == area and cs.get("extinction_mm2") == area and cs.get("absorption_mm2") == 0,
-            "cross sections")
-    q = m.get("quality", {})
-    require(q.get("invalid_values") == 0 and q.get("incomplete_directions") == 0, "invalid optical data")
-    require(q.get("allow_underresolved") is s["allow_underresolved"], "underresolution policy")
-    require(s["allow_underresolved"] or q.get("underresolved_directions") == 0, "underresolution not authorized")
-    for key, limit in (("cdf_l1_mass_error", "cdf_max_l1"), ("cdf_lost_probability_mass", "cdf_max_lost_mass")):
-        v = q.get(key)
-        require(type(v) in (int, float) and math.isfinite(v) and 0 <= v <= s[limit], key)
-    sp = m.get("storage_processing", {})
-    require(sp.get("additional_filter") == "none", "unexpected global smoothing")
-    tv = sp.get("coarsening_tv")
-    require(type(tv) in (int, float) and math.isfinite(tv) and 0 <= tv <= s["maximum_coarsening_tv"],
-            "coarsening error")
-    hg = m.get("hg", {})
-    require(hg.get("target") == "saved_cdf" and type(hg.get("g")) in (int, float) and -1 < hg["g"] < 1, "HG target")
-    nt, np_ = s["cdf_theta"], s["cdf_phi"]
-    shapes = ((np_ + 1,), (np_, nt + 1), (nt + 1,))
-    sizes = {name: npy_size(path / name, shape) for name, shape in zip(ARRAYS, shapes)}
-    return {"metadata_sha256": sha256(path / "metadata.json"), "npy_sizes": sizes, "hg_g": hg["g"],
-            "projected_area_mm2": area, "underresolved_directions": q.get("underresolved_directions"),
-            "coarsening_tv": tv}
+            raise ValueError(f"{path}: {label}")
+    nt, nphi = c["solver"]["cdf_theta"], c["solver"]["cdf_phi"]
+    shapes = dict(zip(ARRAYS, ((nphi + 1,), (nphi, nt + 1), (nt + 1,))))
+    return {"metadata_sha256": sha256(path / "metadata.json"),
+            "npy_sizes": {name: npy_size(path / name, shape) for name, shape in shapes.items()},
+            "hg_g": m["hg"]["g"], "projected_area_mm2": m["geometry"]["projected_area_mm2"],
+            "underresolved_directions": _dig(m, "quality.underresolved_directions"),
+            "coarsening_tv": m["storage_processing"]["coarsening_tv"]}
 
 
 # ------------------------------------------------------------------ one condition
