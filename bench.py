@@ -55,13 +55,18 @@ sys.path.insert(0, str(ROOT))
 # +-5 % around it with seed 2 and checks convexity in the library.
 BASE_COEFFS = (-0.0458, -0.0335, -0.1211, -0.0227)
 
-# Operation census of the drop kernel per event (flops, one per IEEE
-# operation including div/sqrt, counted on the FP32 mirror of the kernel and
-# on the binning): entry of a drop ray, one internal interaction, one emission
-# (power + FP64 binning). With FMA contraction off each is one instruction.
+# Operation census of the drop kernel per event: mean operations (+, -, *, /,
+# sqrt and libm calls, one each) of the FP32 mirror of the kernel and of the
+# FP64 phase_bin rule, counted by the counting build of the oracle
+# (oracle/census.cpp) over 8 grid rows of every record of the workload; derived
+# by `python oracle/census.py` and checked by tests/test_census_cpu.py. miss: a
+# ray that misses; entry: a hitting ray's set-up, entry solve, first split and
+# its reflection's power; hit: one internal encounter with its transmitted
+# branch's power; emit: the binning of one emitted direction. With FMA
+# contraction off each operation is one instruction.
 CENSUS = {
-    "drop": {"miss": 10, "entry": 1230, "hit": 1150, "emit": 102},
-    "sphere": {"miss": 10, "entry": 250, "hit": 270, "emit": 102},
+    "drop": {"miss": 210.2, "entry": 1202.0, "hit": 1154.2, "emit": 46.5},
+    "sphere": {"miss": 12.0, "entry": 269.0, "hit": 273.0, "emit": 46.5},
 }
 PEAK_TFLOPS_FP32 = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
@@ -139,6 +144,27 @@ def drop_traffic(args):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def atomic_calibration(ctx, count: int, trace_ms: float, batch_cells: int, record_cells: int) -> dict:
+    """The trace kernel's u64 atomicAdd rate (one per binned emission) beside
+    what the device sustains for the same operation alone (iph_atomic_probe):
+    random cells over the batch's whole histogram slab (the peak the fraction
+    is taken against), random cells over one record's slab, and one hot cell."""
+    per_s = count / (trace_ms / 1e3) if trace_ms == trace_ms else None
+    big = 1 << 30
+    rates = {}
+    for name, cells, n, hot in (("random_batch_slab", batch_cells, big, False),
+                                ("random_record_slab", record_cells, big, False),
+                                ("hot_cell", 1, 1 << 26, True)):
+        ctx.atomic_probe(cells, 1 << 20, hot)            # warm the code and the allocation path
+        ms = min(ctx.atomic_probe(cells, n, hot) for _ in range(3))
+        rates[name] = {"cells": cells, "atomics": n, "ms": ms, "per_s": n / (ms / 1e3)}
+    peak = rates["random_batch_slab"]["per_s"]
+    return {"count_per_launch": count, "per_s": per_s, "peak": peak,
+            "peak_definition": "u64 atomicAdd to uniformly random cells of a slab the size of the batch's "
+                               "histograms, launched like the trace kernels (iph_atomic_probe)",
+            "frac": per_s / peak if per_s else None, "calibration": rates}
 
 
 def run_hex(ctx, rays_log2: int = 28):
@@ -789,8 +815,7 @@ def main() -> int:
                      "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
                      "frac": (achieved / PEAK_TFLOPS_FP32) if achieved else None,
                      "traffic": drop_traffic(args), "flops_per_launch": flops},
-        "atomics": {"count_per_launch": atomics,
-                    "per_s": atomics / (trace_avg / 1e3) if trace_avg == trace_avg else None},
+        "atomics": atomic_calibration(ctx, atomics, trace_avg, len(recs) * nt * np_, nt * np_),
     }
     batch.close()
     if not args.no_extras:

@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
                                 P(CdfReport)]
     L.iph_build_cdf_noalloc.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(CdfPolicy), C.c_void_p, C.c_void_p,
                                         P(CdfReport)]
+    L.iph_atomic_probe.argtypes = [C.c_void_p, u32, u64, u64, P(f64)]
     L.iph_write_record.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, u32, u32]
     L.iph_water_optics.argtypes = [f64, f64, f64, P(WaterState)]
     L.iph_water_index.argtypes = [f64, f64, f64, P(f64)]
@@ -197,7 +198,7 @@ def _live(handle: C.c_void_p, what: str) -> C.c_void_p:
 EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_destroy", "iph_last_error",
             "iph_u_edges", "iph_bin_directions", "iph_batch_create", "iph_batch_destroy", "iph_batch_run",
             "iph_batch_set_schedule", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
-            "iph_build_cdf", "iph_build_cdf_noalloc", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
+            "iph_build_cdf", "iph_build_cdf_noalloc", "iph_atomic_probe", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
             "iph_drop_projected_area", "iph_hex_projected_area", "iph_batch_probabilities", "iph_sur_create",
             "iph_sur_destroy", "iph_sur_param_count", "iph_sur_get_params", "iph_sur_set_params",
             "iph_sur_set_record", "iph_sur_set_target", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",
@@ -382,6 +383,13 @@ class Context:
         _check(lib().iph_bin_directions(self._p, nt, np_, frame.ctypes.data_as(P(f64)), n, dirs.ctypes.data,
                                         index.ctypes.data, kind.ctypes.data), self._p)
         return index, kind
+
+    def atomic_probe(self, cells: int, atomics: int, hot: bool = False) -> float:
+        """Device ms of `atomics` u64 atomicAdds over `cells` random cells (or one
+        hot cell): iph_atomic_probe."""
+        ms = f64()
+        _check(lib().iph_atomic_probe(self._p, 1 if hot else 0, cells, atomics, C.byref(ms)), self._p)
+        return ms.value
 
     def build_cdf(self, mass: np.ndarray, nt: int, np_: int, out_theta: int = 0, out_phi: int = 0,
                   max_l1: float = 1e-10, max_lost: float = 1e-12, max_tv: float = 1.0, check: bool = True):

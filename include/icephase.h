@@ -205,6 +205,15 @@ int iph_batch_cdf(iph_batch* b, uint32_t rec, double* phi /* onp+1 */, double* t
 int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
                   double* phi, double* theta, iph_cdf_report* report);
 
+/* Measurement only, no counterpart in the reference: the device time (ms) of
+ * `atomics` u64 atomicAdd(+1) operations on a zeroed slab of `cells` u64, to
+ * cells drawn by a counter hash (mode 0) or all to one cell (mode 1), launched
+ * like the trace kernels (8 workgroups of 256 per CU). The slab is summed
+ * afterwards and IPH_E_AUDIT returned unless the sum equals `atomics`. It
+ * calibrates the histogram atomics' throughput against what the device
+ * sustains for the same footprint. 1 <= cells < 2^32, 1 <= atomics <= 2^40. */
+int iph_atomic_probe(iph_ctx* ctx, uint32_t mode, uint64_t cells, uint64_t atomics, double* ms);
+
 /* No counterpart in the reference. For callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid
  * hipMalloc): the same CDF as iph_build_cdf, bitwise, from caller DEVICE arrays

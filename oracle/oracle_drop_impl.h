@@ -12,6 +12,13 @@
  * in single precision, and the emission rule (R at entry, T at every internal
  * encounter, continue R) follows hex_trace/include/ice_crystal/hex_trace_core.hpp:101-163.
  */
+/* Event marks for the operation census (census.cpp); empty in the oracle build:
+ * 0 ray start, 1 miss, 2 entry done (first split and the external reflection),
+ * 3 one internal encounter done, 4 ray end. */
+#ifndef CENSUS_MARK
+#define CENSUS_MARK(event)
+#endif
+
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 #define V CAT(v3, SFX)
@@ -258,9 +265,10 @@ static orc_ray_audit FN(trace_origin_)(const FN(rec_)* R, uint64_t id, REAL u, R
     orc_ray_audit a;
     memset(&a, 0, sizeof a);
     a.status = 8; /* miss */
+    CENSUS_MARK(0);
     const V o = FN(add_)(FN(scl_)(R->e0, u), FN(scl_)(R->e1, v));
     REAL t;
-    if (!FN(entry_)(R, o, u, v, &t)) { a.unresolved = 0; return a; }
+    if (!FN(entry_)(R, o, u, v, &t)) { a.unresolved = 0; CENSUS_MARK(1); return a; }
     V p = FN(add_)(o, FN(scl_)(R->ki, t));
     FN(ray_) ray;
     ray.dir = R->ki; ray.e0 = R->e0; ray.e1 = R->e1;
@@ -285,7 +293,8 @@ static orc_ray_audit FN(trace_origin_)(const FN(rec_)* R, uint64_t id, REAL u, R
         }                                                                         \
     } while (0)
     EMIT(sp.r, 0u);
-    if (!sp.has_t) { a.tir_count = 1; a.status = 0; a.escaped = escaped; a.unresolved = 0; return a; }
+    CENSUS_MARK(2);
+    if (!sp.has_t) { a.tir_count = 1; a.status = 0; a.escaped = escaped; a.unresolved = 0; CENSUS_MARK(4); return a; }
     ray = sp.t;
     a.status = 2; /* interaction limit unless something ends it first */
     for (uint32_t hit = 0; hit < R->max_hits; ++hit) {
@@ -303,12 +312,14 @@ static orc_ray_audit FN(trace_origin_)(const FN(rec_)* R, uint64_t id, REAL u, R
         if (sp.has_t) EMIT(sp.t, 1u);
         ray = sp.r;
         p = q;
+        CENSUS_MARK(3);
     }
 #undef EMIT
     const REAL rem = FN(pow_)(&ray.j);
     if (a.status == 2 && rem == (REAL)0) a.status = 0;
     a.escaped = escaped;
     a.unresolved = (double)rem;
+    CENSUS_MARK(4);
     return a;
 }
 

@@ -483,42 +483,28 @@ int orc_drop_frame(const double k_in[3], orc_frame* f)
 }
 
 /* phase_math.hpp:37-81: everything after the unit-length checks. */
-static orc_bin bin_core(const double* ue, uint32_t nt, uint32_t np, double x, double y, double z, double n)
-{
-    orc_bin b; memset(&b, 0, sizeof b);
-    const double mu = z / n, s = hypot(x, y) / n;
-    b.mu = mu;
-    if (s <= 256 * DBL_EPSILON) { b.pole = mu >= 0 ? 1 : -1; b.valid = 1; return b; }
-    const double u = mu >= 0 ? (0.5 * s * s) / (1 + mu) : 1 - (0.5 * s * s) / (1 - mu);
-    if (!(u >= 0 && u <= 1)) return b;
-    uint32_t lo = 0, hi = nt;
-    while (lo + 1 < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (u >= ue[mid]) lo = mid; else hi = mid;
-    }
-    for (unsigned cand = 0; cand < 2; ++cand) {
-        const uint32_t e = lo + cand;
-        if (e == 0 || e >= nt) continue;
-        const double width = fmin(ue[e] - ue[e - 1], ue[e + 1] - ue[e]);
-        const double tol = fmin(64 * DBL_EPSILON * fmax(fabs(u), fabs(ue[e])), 1e-7 * width);
-        if (fabs(u - ue[e]) <= tol) { b.snapped = (u != ue[e]); lo = e; break; }
-    }
-    double phi = atan2(y, x);
-    if (phi >= PI_D) phi -= 2 * PI_D;
-    const double v = (phi + PI_D) / (2 * PI_D);
-    double index = v * np;
-    const double nearest = floor(index + 0.5);
-    if (fabs(index - nearest) <= fmin(64 * DBL_EPSILON * np, 1e-7)) {
-        b.snapped = b.snapped || (index != nearest);
-        index = nearest;
-    }
-    uint32_t j = (uint32_t)index;
-    if (j == np) j = 0;
-    if (j >= np) return b;
-    b.index = (uint64_t)lo * np + j;
-    b.valid = 1;
-    return b;
-}
+/* the phase_bin rule (phase_math.hpp:37-81) in double: oracle_bin_rule.h */
+#define BREAL double
+#define BFN(name) name##d
+#define BVAL(x) (x)
+#define BHYPOT hypot
+#define BATAN2 atan2
+#define BFLOOR floor
+#define BSQRT sqrt
+#define BFABS fabs
+#define BFMIN fmin
+#define BFMAX fmax
+#include "oracle_bin_rule.h"
+#undef BREAL
+#undef BFN
+#undef BVAL
+#undef BHYPOT
+#undef BATAN2
+#undef BFLOOR
+#undef BSQRT
+#undef BFABS
+#undef BFMIN
+#undef BFMAX
 
 orc_bin orc_phase_bin(const double* ue, uint32_t nt, uint32_t np, const orc_frame* g, const double dir[3])
 {
@@ -530,20 +516,12 @@ orc_bin orc_phase_bin(const double* ue, uint32_t nt, uint32_t np, const orc_fram
     const double x = dot(e0, d), y = dot(e1, d), z = dot(ki, d);
     const double n = sqrt(x * x + y * y + z * z);
     if (!(n > 0) || !fin(n) || fabs(n - 1) > 512 * DBL_EPSILON) return b;
-    return bin_core(ue, nt, np, x, y, z, n);
+    return bin_core_d(ue, nt, np, x, y, z, n);
 }
 
 orc_bin orc_phase_bin_raw(const double* ue, uint32_t nt, uint32_t np, const orc_frame* g, const double dir[3])
 {
-    orc_bin b; memset(&b, 0, sizeof b);
-    const v3 d = mk(dir[0], dir[1], dir[2]);
-    if (!ue || !nt || !np || !vfin(d)) return b;
-    const v3 e0 = mk(g->e0[0], g->e0[1], g->e0[2]), e1 = mk(g->e1[0], g->e1[1], g->e1[2]);
-    const v3 ki = mk(g->ki[0], g->ki[1], g->ki[2]);
-    const double x = dot(e0, d), y = dot(e1, d), z = dot(ki, d);
-    const double n = sqrt(x * x + y * y + z * z);
-    if (!(n > 0) || !fin(n)) return b;
-    return bin_core(ue, nt, np, x, y, z, n);
+    return bin_raw_d(ue, nt, np, g, dir);
 }
 
 uint32_t orc_quantum_bits(uint64_t rays)

@@ -528,6 +528,33 @@ def ref_drop_trace_vertices(r: DropRecord, w: int, h: int, reference_distance: f
             c1.reshape(4, h * w, 4))
 
 
+# ---------------------------------------------------------------- operation census
+CENSUS_SO = ROOT / "oracle" / "libcensus.so"
+
+
+class Census(C.Structure):
+    _fields_ = [("misses", u64), ("entries", u64), ("hits", u64), ("emissions", u64), ("ops_miss", u64),
+                ("ops_entry", u64), ("ops_hit", u64), ("ops_emit", u64)]
+
+
+_census = None
+
+
+def drop_census(r: DropRecord, nt: int, np_: int, first: int, count: int) -> Census:
+    """Counted operations and events of rays [first, first + count) of a drop
+    record (oracle/census.cpp)."""
+    global _census
+    if _census is None:
+        if not CENSUS_SO.exists():
+            build_oracle()
+        lib()
+        _census = C.CDLL(str(CENSUS_SO))
+        _census.orc_drop_census.argtypes = [P(DropRecord), u32, u32, u64, u64, P(Census)]
+    out = Census()
+    assert _census.orc_drop_census(C.byref(r), nt, np_, first, count, C.byref(out)) == 0
+    return out
+
+
 def oracle_env_ok() -> bool:
     return ORACLE_SO.exists() or (ROOT / "oracle" / "Makefile").exists()
 

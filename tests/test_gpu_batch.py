@@ -53,3 +53,15 @@ def test_schedules_that_wrap_the_tile_queue_are_refused(ctx):
         b.run(ip.RUN_TRACE)
         for i in range(2):
             assert np.array_equal(b.bins(i), ref[i])
+
+
+def test_atomic_probe_counts_every_increment(ctx):
+    """The atomic calibration lands every increment (the library checks the slab
+    sum against the count and fails otherwise) and refuses bad arguments."""
+    print("case: atomic probe random 2^20 cells, hot cell, refusals", flush=True)
+    assert ctx.atomic_probe(1 << 20, 1 << 24) > 0
+    assert ctx.atomic_probe(1, 1 << 20, hot=True) > 0
+    assert ctx.atomic_probe(3, 1000) > 0
+    for cells, n in ((0, 10), (1 << 32, 10), (10, 0), (10, (1 << 40) + 1)):
+        with pytest.raises(ip.IcePhaseError):
+            ctx.atomic_probe(cells, n)
