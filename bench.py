@@ -65,7 +65,7 @@ BASE_COEFFS = (-0.0458, -0.0335, -0.1211, -0.0227)
 # branch's power; emit: the binning of one emitted direction. With FMA
 # contraction off each operation is one instruction.
 CENSUS = {
-    "drop": {"miss": 210.2, "entry": 1202.0, "hit": 1154.2, "emit": 46.5},
+    "drop": {"miss": 79.7, "entry": 646.9, "hit": 607.8, "emit": 46.5},
     "sphere": {"miss": 12.0, "entry": 269.0, "hit": 273.0, "emit": 46.5},
 }
 PEAK_TFLOPS_FP32 = 157.3   # MI355X published FP32 vector peak
@@ -402,7 +402,8 @@ def run_uneven(ctx, warmup: int = 1, steps: int = 1, quick: bool = True):
     and rays/s over `steps` timed passes after `warmup` passes. quick: tune over
     three schedules on a 1/64-ray proxy (the full grid of candidates otherwise).
     Every record's quanta are checked for conservation at full size (escaped +
-    unresolved = hitting rays x 2^Q, FP32 drops to 1e-5, FP64 hex to 1e-12)."""
+    unresolved = hitting rays x 2^Q, up to half a quantum per emission and per
+    remainder plus 1e-5 of the total for FP32 drops, 1e-12 for FP64 hex)."""
     import ice_crystal_phase_amd as ip
     from ice_crystal_phase_amd import autotune
     specs = autotune.uneven_specs(512, seed=4)
@@ -431,9 +432,12 @@ def run_uneven(ctx, warmup: int = 1, steps: int = 1, quick: bool = True):
         worst = 0.0
         for k, sp in enumerate(specs):
             st = b.stats(k)
-            total = float(ip.record_rays(recs[k]) - st.status[8]) * 2.0 ** st.qbits
-            rel = abs(float(st.q_escaped + st.q_unresolved) - total) / total
-            worst = max(worst, rel / (1e-12 if sp.kind == 2 else 1e-5))
+            hit = ip.record_rays(recs[k]) - st.status[8]
+            total = float(hit) * 2.0 ** st.qbits
+            # rounding: half a quantum per emission and per unresolved remainder;
+            # power balance: ~1e-7 per FP32 drop ray, 2048 eps per FP64 hex interface
+            allowed = 0.5 * float(st.samples + hit) + (1e-12 if sp.kind == 2 else 1e-5) * total
+            worst = max(worst, abs(float(st.q_escaped + st.q_unresolved) - total) / allowed)
         if worst > 1:
             raise RuntimeError(f"uneven batch: a record's quanta are not conserved ({worst:.2f} x tolerance)")
     kinds = [sp.kind for sp in specs]

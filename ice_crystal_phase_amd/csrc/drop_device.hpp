@@ -49,15 +49,21 @@ struct Shape {
     float inner, outer;
     float rmax;  // bounding radius of the intersector (max r(chi), rounded up to 2^-20)
     uint32_t sphere;
+    uint32_t terms;  // 1 + index of the last nonzero coefficient, at least 2
 };
 
-__device__ inline float cheb_radius(const float* c, float mu, float& der)
+// r(mu) and r'(mu) by the Chebyshev recurrence, stopped after the last nonzero
+// coefficient: the skipped terms would add c_n T_n = +-0 to finite sums, which
+// leaves every value unchanged. `terms` is the same for every lane of a wave
+// (one record per tile), so the early exit is a scalar branch.
+__device__ inline float cheb_radius(const float* c, uint32_t terms, float mu, float& der)
 {
     float prev = 1.0f, cur = mu, dprev = 0.0f, dcur = 1.0f;
     float rad = (1.0f + c[0]) + c[1] * mu;
     float d = c[1];
 #pragma unroll
     for (int n = 2; n < 8; ++n) {
+        if ((uint32_t)n >= terms) break;
         const float next = (2.0f * mu) * cur - prev;
         const float dnext = (2.0f * cur + (2.0f * mu) * dcur) - dprev;
         rad = rad + c[n] * next;
@@ -78,7 +84,7 @@ __device__ inline float surface(const Shape& s, f3 p)
     if (r < s.inner * 0.25f) return r - s.inner;
     const float mu = -(p.y * (1.0f / r));
     float der;
-    return r - cheb_radius(s.c, mu, der);
+    return r - cheb_radius(s.c, s.terms, mu, der);
 }
 
 __device__ inline f3 gradient(const Shape& s, f3 p)
@@ -89,7 +95,7 @@ __device__ inline f3 gradient(const Shape& s, f3 p)
     if (s.sphere) return u;
     const float mu = -u.y;
     float der;
-    (void)cheb_radius(s.c, mu, der);
+    (void)cheb_radius(s.c, s.terms, mu, der);
     const float k = der * ir;
     const f3 a = F3(0.0f - u.x * mu, -1.0f - u.y * mu, 0.0f - u.z * mu);
     return u - a * k;
@@ -108,7 +114,7 @@ __device__ inline float surface_slope(const Shape& s, f3 p, f3 d, float& slope)
     }
     const float mu = -u.y;
     float der;
-    const float rad = cheb_radius(s.c, mu, der);
+    const float rad = cheb_radius(s.c, s.terms, mu, der);
     const float k = der * ir;
     const f3 a = F3(0.0f - u.x * mu, -1.0f - u.y * mu, 0.0f - u.z * mu);
     slope = dot(u - a * k, d);

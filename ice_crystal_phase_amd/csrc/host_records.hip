@@ -345,6 +345,9 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
             if (rr > rm) rm = rr;
         }
         c.shape.rmax = sphere ? 1.0f : (float)(std::ceil((rm + 1e-6) * 1048576.0) / 1048576.0);
+        c.shape.terms = 2;
+        for (uint32_t m = 2; m < 8; ++m)
+            if (c.shape.c[m] != 0.0f) c.shape.terms = m + 1;
         BinFrame f;
         if (!drop_frame(k_in, f)) {
             err = "Incident direction must be finite and nonzero.";

@@ -12,13 +12,15 @@
 
 namespace {
 
-__device__ inline uint64_t mix64(uint64_t x)
+// 32-bit integer hash (multiply-xorshift): a few VALU instructions per
+// increment, so the probe measures the atomics rather than the address arithmetic.
+__device__ inline uint32_t mix32(uint32_t x)
 {
-    x ^= x >> 31;
-    x *= 0x7fb5d329728ea185ull;
-    x ^= x >> 27;
-    x *= 0x81dadef4bc2dd44dull;
-    return x ^ (x >> 33);
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    return x ^ (x >> 16);
 }
 
 __global__ void __launch_bounds__(256) probe_zero(unsigned long long* __restrict__ slab, uint64_t cells)
@@ -27,14 +29,16 @@ __global__ void __launch_bounds__(256) probe_zero(unsigned long long* __restrict
         slab[i] = 0ull;
 }
 
-// mode 0: random cells (the high 32 bits of a hash times the cell count, so no
-// 64-bit modulo in the loop); mode 1: cell 0 only.
+// mode 0: random cells (a 32-bit hash of the increment's index scaled to the
+// cell count by a 32 x 32 -> 64 multiply, no modulo); mode 1: cell 0 only.
 __global__ void __launch_bounds__(256) probe_atomics(unsigned long long* __restrict__ slab, uint64_t cells,
                                                      uint64_t atomics, uint32_t mode)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t n = (uint32_t)cells;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < atomics; i += stride) {
-        const uint64_t cell = mode == 0 ? ((mix64(i) >> 32) * cells) >> 32 : 0ull;
+        const uint32_t h = mix32((uint32_t)i ^ (uint32_t)(i >> 32) * 0x9e3779b9u);
+        const uint64_t cell = mode == 0 ? ((uint64_t)h * n) >> 32 : 0ull;
         atomicAdd(&slab[cell], 1ull);
     }
 }

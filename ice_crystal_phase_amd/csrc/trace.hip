@@ -382,7 +382,15 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
     }
 }
 
-extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+// Waves per SIMD the drop kernel is compiled for. It is latency bound (serial
+// Newton chains): uncapped it needs 201 VGPRs and runs 2 waves per SIMD without
+// spills; capped for 3 waves (168 VGPRs, 100 B/lane of spills) it is fastest
+// (4.31 G rays/s on 24 records of the oblate workload, against 4.26 at 4 waves
+// and 3.65 at 2).
+#ifndef ICEPHASE_DROP_WAVES
+#define ICEPHASE_DROP_WAVES 3
+#endif
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_DROP_WAVES, 8)))
     icephase_trace_drop(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindDrop, true>(recs, map, queue);
@@ -403,7 +411,7 @@ extern "C" __global__ void __launch_bounds__(256)
 {
     trace_loop<kKindHex, false, true>(recs, map, queue);
 }
-extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_DROP_WAVES, 8)))
     icephase_trace_drop_wide(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindDrop, false>(recs, map, queue);

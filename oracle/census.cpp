@@ -137,6 +137,9 @@ int orc_drop_census(const orc_drop_record* r, uint32_t nt, uint32_t np, uint64_t
     R.e1 = mk_c(r->e1[0], r->e1[1], r->e1[2]);
     R.n_int = r->n_int; R.n_ext = r->n_ext; R.h = r->half_extent; R.tail = r->tail;
     R.w = r->grid_w; R.hgt = r->grid_h; R.max_hits = r->max_hits; R.sphere = r->sphere;
+    R.terms = 2;
+    for (uint32_t m = 2; m < 8; ++m)
+        if (r->coeffs[m] != 0.0f) R.terms = m + 1;
     orc_sample buf[64];
     for (uint64_t i = 0; i < count; ++i) {
         census::marks.clear();

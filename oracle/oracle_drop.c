@@ -98,6 +98,15 @@ int orc_drop_make(const double coeffs[8], int sphere, double alpha_deg, double n
     return 0;
 }
 
+/* 1 + the index of the last nonzero shape coefficient, at least 2 */
+static uint32_t orc_drop_terms(const orc_drop_record* r)
+{
+    uint32_t t = 2;
+    for (uint32_t m = 2; m < 8; ++m)
+        if (r->coeffs[m] != 0.0f) t = m + 1;
+    return t;
+}
+
 static void load_f(const orc_drop_record* r, rec_f* R)
 {
     for (int n = 0; n < 8; ++n) R->c[n] = r->coeffs[n];
@@ -107,6 +116,7 @@ static void load_f(const orc_drop_record* r, rec_f* R)
     R->e1 = mk_f(r->e1[0], r->e1[1], r->e1[2]);
     R->n_int = r->n_int; R->n_ext = r->n_ext; R->h = r->half_extent; R->tail = r->tail;
     R->w = r->grid_w; R->hgt = r->grid_h; R->max_hits = r->max_hits; R->sphere = r->sphere;
+    R->terms = orc_drop_terms(r);
 }
 static void load_d(const orc_drop_record* r, rec_d* R)
 {
@@ -124,6 +134,7 @@ static void load_d(const orc_drop_record* r, rec_d* R)
     R->e1 = unit_d(crs_d(R->ki, R->e0));
     R->n_int = r->n_int; R->n_ext = r->n_ext; R->h = r->half_extent; R->tail = r->tail;
     R->w = r->grid_w; R->hgt = r->grid_h; R->max_hits = r->max_hits; R->sphere = r->sphere;
+    R->terms = orc_drop_terms(r);
 }
 
 int64_t orc_drop_trace(const orc_drop_record* r, int precision, uint64_t first, uint64_t count,

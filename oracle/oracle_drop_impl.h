@@ -42,13 +42,15 @@ static inline V FN(crs_)(V a, V b)
 { return FN(mk_)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 static inline V FN(unit_)(V a) { return FN(scl_)(a, (REAL)1 / SQRT(FN(dot_)(a, a))); }
 
-/* Chebyshev radius and slope, raindrop_shape.hpp:71-84 recurrence order. */
-static inline REAL FN(radius_)(const REAL* c, REAL mu, REAL* der)
+/* Chebyshev radius and slope, raindrop_shape.hpp:71-84 recurrence order,
+ * stopped after the last nonzero coefficient (terms = its index + 1, at least
+ * 2): the omitted terms add c_n T_n = +-0 to finite sums, which changes no value. */
+static inline REAL FN(radius_)(const REAL* c, uint32_t terms, REAL mu, REAL* der)
 {
     REAL prev = (REAL)1, cur = mu, dprev = (REAL)0, dcur = (REAL)1;
     REAL rad = ((REAL)1 + c[0]) + c[1] * mu;
     REAL d = c[1];
-    for (int n = 2; n < 8; ++n) {
+    for (int n = 2; n < 8 && (uint32_t)n < terms; ++n) {
         const REAL next = ((REAL)2 * mu) * cur - prev;
         const REAL dnext = ((REAL)2 * cur + ((REAL)2 * mu) * dcur) - dprev;
         rad = rad + c[n] * next;
@@ -64,7 +66,7 @@ typedef struct {
     REAL inner, outer, rmax;
     V ki, e0, e1;
     REAL n_int, n_ext, h, tail;
-    uint32_t w, hgt, max_hits, sphere;
+    uint32_t w, hgt, max_hits, sphere, terms;
 } FN(rec_);
 
 static inline REAL FN(surf_)(const FN(rec_)* R, V p)
@@ -74,7 +76,7 @@ static inline REAL FN(surf_)(const FN(rec_)* R, V p)
     if (r < R->inner * (REAL)0.25) return r - R->inner;
     const REAL mu = -(p.y * ((REAL)1 / r));
     REAL der;
-    return r - FN(radius_)(R->c, mu, &der);
+    return r - FN(radius_)(R->c, R->terms, mu, &der);
 }
 /* outward (unnormalized) gradient: u - (g - u mu) R'(mu)/r with g=(0,-1,0) */
 static inline V FN(grad_)(const FN(rec_)* R, V p)
@@ -85,10 +87,31 @@ static inline V FN(grad_)(const FN(rec_)* R, V p)
     if (R->sphere) return u;
     const REAL mu = -u.y;
     REAL der;
-    (void)FN(radius_)(R->c, mu, &der);
+    (void)FN(radius_)(R->c, R->terms, mu, &der);
     const REAL k = der * ir;
     const V a = FN(mk_)((REAL)0 - u.x * mu, (REAL)-1 - u.y * mu, (REAL)0 - u.z * mu);
     return FN(sub_)(u, FN(scl_)(a, k));
+}
+
+/* f(p) and grad f(p) . d from one evaluation of r, mu and the Chebyshev
+ * recurrence (the kernel's surface_slope): the same operations as surf_ and
+ * grad_ above, so the same values. */
+static inline REAL FN(surf_slope_)(const FN(rec_)* R, V p, V d, REAL* slope)
+{
+    const REAL r = SQRT(FN(dot_)(p, p));
+    const REAL ir = (REAL)1 / r;
+    const V u = FN(scl_)(p, ir);
+    if (R->sphere) {
+        *slope = FN(dot_)(u, d);
+        return r - (REAL)1;
+    }
+    const REAL mu = -u.y;
+    REAL der;
+    const REAL rad = FN(radius_)(R->c, R->terms, mu, &der);
+    const REAL k = der * ir;
+    const V a = FN(mk_)((REAL)0 - u.x * mu, (REAL)-1 - u.y * mu, (REAL)0 - u.z * mu);
+    *slope = FN(dot_)(FN(sub_)(u, FN(scl_)(a, k)), d);
+    return r < R->inner * (REAL)0.25 ? r - R->inner : r - rad;
 }
 
 /* Safeguarded Newton on f(o + t d) over [a,b] where f(a) has sign sa (+1/-1)
@@ -99,12 +122,12 @@ static REAL FN(newton_)(const FN(rec_)* R, V o, V d, REAL a, REAL b, int a_pos, 
     if (!(x > a && x < b)) x = a + (b - a) * (REAL)0.5;
     for (int it = 0; it < 48; ++it) {
         const V p = FN(add_)(o, FN(scl_)(d, x));
-        const REAL fx = FN(surf_)(R, p);
+        REAL slope;
+        const REAL fx = FN(surf_slope_)(R, p, d, &slope);
         if (fx == (REAL)0) return x;
         if ((fx > (REAL)0) == (a_pos != 0)) a = x; else b = x;
         const REAL scale = FABS(x) > (REAL)1 ? FABS(x) : (REAL)1;
         if (b - a <= tol * scale) return x;
-        const REAL slope = FN(dot_)(FN(grad_)(R, p), d);
         const REAL cand = x - fx / slope;
         if (cand > a && cand < b) {
             const REAL step = FABS(cand - x);
@@ -131,13 +154,13 @@ static int FN(entry_)(const FN(rec_)* R, V o, REAL u, REAL v, REAL* tout)
     if (!(rho2 < out2)) return 0;
     const REAL ta = -SQRT(out2 - rho2);
     REAL tm = (REAL)0;
-    REAL fm = FN(surf_)(R, o);
+    REAL g0;
+    const REAL fm = FN(surf_slope_)(R, o, R->ki, &g0);
     if (!(fm < (REAL)0)) {
         /* foot of the perpendicular is outside: minimize f along the line by the
          * secant method on f'(t) = grad f . ki from the foot (the inside of a convex
          * body along a line is the single interval around that minimum); any
          * iterate with f < 0 is an interior point, convergence with f >= 0 is a miss */
-        REAL g0 = FN(dot_)(FN(grad_)(R, o), R->ki);
         REAL t0 = (REAL)0;
         REAL t1 = -g0 * SQRT(rho2);
         int found = 0;
@@ -145,9 +168,9 @@ static int FN(entry_)(const FN(rec_)* R, V o, REAL u, REAL v, REAL* tout)
             if (t1 < ta) t1 = ta;
             if (t1 > -ta) t1 = -ta;
             const V p1 = FN(add_)(o, FN(scl_)(R->ki, t1));
-            const REAL f1 = FN(surf_)(R, p1);
+            REAL g1;
+            const REAL f1 = FN(surf_slope_)(R, p1, R->ki, &g1);
             if (f1 < (REAL)0) { tm = t1; found = 1; break; }
-            const REAL g1 = FN(dot_)(FN(grad_)(R, p1), R->ki);
             if (g1 == g0) break;
             const REAL t2 = t1 - g1 * (t1 - t0) / (g1 - g0);
             t0 = t1; g0 = g1;

@@ -81,7 +81,8 @@ def test_uneven_mixed_batch_equals_oracle(ctx, oracle):
             hit = ip.record_rays(rec) - st.status[8]
             total = float(hit) * 2.0 ** st.qbits
             tol = 1e-12 if s.kind == 2 else 1e-5
-            assert abs(float(st.q_escaped + st.q_unresolved) - total) <= tol * total, (i, s)
+            rounding = 0.5 * float(st.samples + hit)   # half a quantum per emission and per remainder
+            assert abs(float(st.q_escaped + st.q_unresolved) - total) <= tol * total + rounding, (i, s)
             # the device CDF of this record equals the oracle CDF of the same masses
             phi, theta, rep = b.cdf(i)
             rc, ophi, otheta, _ = oracle.build_cdf(oracle.bins_to_mass(got, st.q_forward, st.q_backward, NT, NP),
