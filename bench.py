@@ -31,7 +31,8 @@ tuned over the full candidate grid); --config surrogate runs the surrogate leg a
 rainbow_generate process per condition (records written to disk); --config
 atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
 atlas tool; --config compose times the orientation composition of that atlas
-(records/s) beside the oracle's NumPy sum.
+(records/s) beside the oracle's NumPy sum; --config atomics runs the u64
+atomicAdd calibration alone (the histogram footprint of --records records).
 
 Prints one JSON line (rank 0). --gpus above 1 exits: the benchmark runs on one GPU.
 """
@@ -149,21 +150,24 @@ def drop_traffic(args):
 def atomic_calibration(ctx, count: int, trace_ms: float, batch_cells: int, record_cells: int) -> dict:
     """The trace kernel's u64 atomicAdd rate (one per binned emission) beside
     what the device sustains for the same operation alone (iph_atomic_probe):
-    random cells over the batch's whole histogram slab (the peak the fraction
-    is taken against), random cells over one record's slab, and one hot cell."""
+    consecutive cells of one record's slab (a wave updates 64 adjacent cells:
+    the ceiling the fraction is taken against), random cells over the batch's
+    whole histogram slab and over one record's slab, and one hot cell."""
     per_s = count / (trace_ms / 1e3) if trace_ms == trace_ms else None
     big = 1 << 30
     rates = {}
-    for name, cells, n, hot in (("random_batch_slab", batch_cells, big, False),
-                                ("random_record_slab", record_cells, big, False),
-                                ("hot_cell", 1, 1 << 26, True)):
-        ctx.atomic_probe(cells, 1 << 20, hot)            # warm the code and the allocation path
-        ms = min(ctx.atomic_probe(cells, n, hot) for _ in range(3))
+    for name, cells, n, pattern in (("coalesced_record_slab", record_cells, big, "coalesced"),
+                                    ("random_batch_slab", batch_cells, big, "random"),
+                                    ("random_record_slab", record_cells, big, "random"),
+                                    ("hot_cell", 1, 1 << 26, "hot")):
+        ctx.atomic_probe(cells, 1 << 20, pattern)        # warm the code and the allocation path
+        ms = min(ctx.atomic_probe(cells, n, pattern) for _ in range(3))
         rates[name] = {"cells": cells, "atomics": n, "ms": ms, "per_s": n / (ms / 1e3)}
-    peak = rates["random_batch_slab"]["per_s"]
+    peak = rates["coalesced_record_slab"]["per_s"]
     return {"count_per_launch": count, "per_s": per_s, "peak": peak,
-            "peak_definition": "u64 atomicAdd to uniformly random cells of a slab the size of the batch's "
-                               "histograms, launched like the trace kernels (iph_atomic_probe)",
+            "peak_definition": "u64 atomicAdd(+1) to consecutive cells of one record's histogram slab (each "
+                               "wave's 64 lanes on 64 adjacent cells), launched like the trace kernels "
+                               "(iph_atomic_probe mode 2)",
             "frac": per_s / peak if per_s else None, "calibration": rates}
 
 
@@ -692,7 +696,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
-    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas", "compose"),
+    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas", "compose", "atomics"),
                     default="drops")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -731,6 +735,14 @@ def main() -> int:
                "data": "synthetic", "config": {"workload": u.pop("config")}}
         out.update(u)
         print(json.dumps(out))
+        return 0
+    if args.config == "atomics":
+        # the calibration alone, at the headline batch's histogram footprint
+        with ip.Context(0) as ctx:
+            cells = args.theta * args.phi
+            cal = atomic_calibration(ctx, 0, float("nan"), args.records * cells, cells)
+        print(json.dumps({"atomic_calibration": cal["calibration"], "peak": cal["peak"],
+                          "peak_definition": cal["peak_definition"]}))
         return 0
     if args.config == "dataset":
         print(json.dumps(run_dataset(args)))

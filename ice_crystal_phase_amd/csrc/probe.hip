@@ -4,8 +4,10 @@
 // fixed-point histogram slab. This kernel measures what the hardware sustains
 // for the same operation on its own: `atomics` increments of 1 to cells drawn
 // by a counter hash from a slab of `cells` u64 (the histogram footprint of a
-// batch), or all to one cell (the worst contention). The slab is summed
-// afterwards; the sum must equal the number of increments exactly.
+// batch), to consecutive cells in the order of the increments (a wave's 64
+// lanes update 64 adjacent cells: the device's ceiling for the operation), or
+// all to one cell (the worst contention). The slab is summed afterwards; the
+// sum must equal the number of increments exactly.
 #include "runtime_internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -30,15 +32,26 @@ __global__ void __launch_bounds__(256) probe_zero(unsigned long long* __restrict
 }
 
 // mode 0: random cells (a 32-bit hash of the increment's index scaled to the
-// cell count by a 32 x 32 -> 64 multiply, no modulo); mode 1: cell 0 only.
+// cell count by a 32 x 32 -> 64 multiply, no modulo); mode 1: cell 0 only;
+// mode 2: cell = index mod cells, kept by a wrapping counter (stride mod cells
+// added per step, so no 64-bit modulo inside the loop).
 __global__ void __launch_bounds__(256) probe_atomics(unsigned long long* __restrict__ slab, uint64_t cells,
                                                      uint64_t atomics, uint32_t mode)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint32_t n = (uint32_t)cells;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < atomics; i += stride) {
-        const uint32_t h = mix32((uint32_t)i ^ (uint32_t)(i >> 32) * 0x9e3779b9u);
-        const uint64_t cell = mode == 0 ? ((uint64_t)h * n) >> 32 : 0ull;
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t step = (uint32_t)(stride % cells);
+    uint32_t next = (uint32_t)(first % cells);
+    for (uint64_t i = first; i < atomics; i += stride) {
+        uint64_t cell = 0;
+        if (mode == 0) {
+            const uint32_t h = mix32((uint32_t)i ^ (uint32_t)(i >> 32) * 0x9e3779b9u);
+            cell = ((uint64_t)h * n) >> 32;
+        } else if (mode == 2) {
+            cell = next;
+            next = next >= n - step ? next - (n - step) : next + step;
+        }
         atomicAdd(&slab[cell], 1ull);
     }
 }
@@ -59,8 +72,8 @@ using namespace icephase;
 
 extern "C" int iph_atomic_probe(iph_ctx* ctx, uint32_t mode, uint64_t cells, uint64_t atomics, double* ms)
 {
-    if (!ctx || !ms || mode > 1 || cells == 0 || cells > 0xffffffffull || atomics == 0 || atomics > (1ull << 40))
-        return fail(ctx, IPH_E_INVALID, "Invalid atomic probe arguments (mode 0|1, 1 <= cells < 2^32, "
+    if (!ctx || !ms || mode > 2 || cells == 0 || cells > 0xffffffffull || atomics == 0 || atomics > (1ull << 40))
+        return fail(ctx, IPH_E_INVALID, "Invalid atomic probe arguments (mode 0|1|2, 1 <= cells < 2^32, "
                                         "1 <= atomics <= 2^40).");
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     DevBuf<unsigned long long> slab, total;

@@ -384,11 +384,15 @@ class Context:
                                         index.ctypes.data, kind.ctypes.data), self._p)
         return index, kind
 
-    def atomic_probe(self, cells: int, atomics: int, hot: bool = False) -> float:
-        """Device ms of `atomics` u64 atomicAdds over `cells` random cells (or one
-        hot cell): iph_atomic_probe."""
+    ATOMIC_PATTERNS = {"random": 0, "hot": 1, "coalesced": 2}
+
+    def atomic_probe(self, cells: int, atomics: int, pattern: str = "random") -> float:
+        """Device ms of `atomics` u64 atomicAdds over a slab of `cells` u64:
+        random cells, one hot cell or consecutive cells (iph_atomic_probe)."""
+        if pattern not in self.ATOMIC_PATTERNS:
+            raise ValueError(f"atomic probe pattern must be one of {sorted(self.ATOMIC_PATTERNS)}")
         ms = f64()
-        _check(lib().iph_atomic_probe(self._p, 1 if hot else 0, cells, atomics, C.byref(ms)), self._p)
+        _check(lib().iph_atomic_probe(self._p, self.ATOMIC_PATTERNS[pattern], cells, atomics, C.byref(ms)), self._p)
         return ms.value
 
     def build_cdf(self, mass: np.ndarray, nt: int, np_: int, out_theta: int = 0, out_phi: int = 0,

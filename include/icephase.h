@@ -207,7 +207,8 @@ int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, co
 
 /* Measurement only, no counterpart in the reference: the device time (ms) of
  * `atomics` u64 atomicAdd(+1) operations on a zeroed slab of `cells` u64, to
- * cells drawn by a counter hash (mode 0) or all to one cell (mode 1), launched
+ * cells drawn by a counter hash (mode 0), all to one cell (mode 1), or to
+ * consecutive cells in increment order, wrapping (mode 2: the ceiling), launched
  * like the trace kernels (8 workgroups of 256 per CU). The slab is summed
  * afterwards and IPH_E_AUDIT returned unless the sum equals `atomics`. It
  * calibrates the histogram atomics' throughput against what the device

@@ -58,10 +58,14 @@ def test_schedules_that_wrap_the_tile_queue_are_refused(ctx):
 def test_atomic_probe_counts_every_increment(ctx):
     """The atomic calibration lands every increment (the library checks the slab
     sum against the count and fails otherwise) and refuses bad arguments."""
-    print("case: atomic probe random 2^20 cells, hot cell, refusals", flush=True)
+    print("case: atomic probe random 2^20 cells, hot cell, coalesced, refusals", flush=True)
     assert ctx.atomic_probe(1 << 20, 1 << 24) > 0
-    assert ctx.atomic_probe(1, 1 << 20, hot=True) > 0
+    assert ctx.atomic_probe(1, 1 << 20, "hot") > 0
     assert ctx.atomic_probe(3, 1000) > 0
+    # the wrapping counter of the coalesced pattern: a cell count that divides
+    # the grid's stride, one that does not, and a single cell
+    for cells in (1 << 16, 1_620_001, 1):
+        assert ctx.atomic_probe(cells, (1 << 22) + 7, "coalesced") > 0
     for cells, n in ((0, 10), (1 << 32, 10), (10, 0), (10, (1 << 40) + 1)):
         with pytest.raises(ip.IcePhaseError):
             ctx.atomic_probe(cells, n)
