@@ -157,6 +157,40 @@ static bool finite3(const double* v)
     return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
 }
 
+bool drop_shape_bounds(const float coef[8], bool sphere, drop::Shape& shape, std::string& err)
+{
+    shape.sphere = sphere ? 1u : 0u;
+    for (int m = 0; m < 8; ++m) shape.c[m] = sphere ? 0.0f : coef[m];
+    if (!sphere) {
+        double cd[8];
+        for (int m = 0; m < 8; ++m) cd[m] = (double)shape.c[m];
+        if (!drop_convex(cd, err)) return false;
+    }
+    double s = 0;
+    for (int m = 1; m < 8; ++m) s += std::fabs((double)shape.c[m]);
+    const double center = 1.0 + (double)shape.c[0];
+    shape.inner = std::nextafter((float)(center - s - 0x1p-20), -INFINITY);
+    shape.outer = std::nextafter((float)(center + s + 0x1p-20), INFINITY);
+    if (sphere) {
+        shape.inner = 1.0f;
+        shape.outer = 1.0f;
+    }
+    // bounding radius of the intersector: max over chi of r(chi) on a dense grid,
+    // + 1e-6, rounded up to a multiple of 2^-20 (exact in single precision)
+    double rm = 0;
+    for (int k = 0; k <= 16384; ++k) {
+        const double chi = kPi * k / 16384;
+        double rr = 1 + (double)shape.c[0];
+        for (int m = 1; m < 8; ++m) rr += (double)shape.c[m] * std::cos(m * chi);
+        if (rr > rm) rm = rr;
+    }
+    shape.rmax = sphere ? 1.0f : (float)(std::ceil((rm + 1e-6) * 1048576.0) / 1048576.0);
+    shape.terms = 2;
+    for (uint32_t m = 2; m < 8; ++m)
+        if (shape.c[m] != 0.0f) shape.terms = m + 1;
+    return true;
+}
+
 bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, double& weight, double& area,
                     std::string& err)
 {
@@ -313,41 +347,15 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
         R.rays = n;
         drop::Config& c = R.drop;
         const bool sphere = in.shape == IPH_SHAPE_SPHERE;
-        c.shape.sphere = sphere ? 1u : 0u;
-        double s = 0;
+        float coef[8];
         for (int m = 0; m < 8; ++m) {
             if (!std::isfinite(in.shape_coefficients[m])) {
                 err = "Shape coefficients must be finite.";
                 return false;
             }
-            c.shape.c[m] = sphere ? 0.0f : (float)in.shape_coefficients[m];
+            coef[m] = (float)in.shape_coefficients[m];
         }
-        if (!sphere) {
-            double cd[8];
-            for (int m = 0; m < 8; ++m) cd[m] = (double)c.shape.c[m];
-            if (!drop_convex(cd, err)) return false;
-        }
-        for (int m = 1; m < 8; ++m) s += std::fabs((double)c.shape.c[m]);
-        const double center = 1.0 + (double)c.shape.c[0];
-        c.shape.inner = std::nextafter((float)(center - s - 0x1p-20), -INFINITY);
-        c.shape.outer = std::nextafter((float)(center + s + 0x1p-20), INFINITY);
-        if (sphere) {
-            c.shape.inner = 1.0f;
-            c.shape.outer = 1.0f;
-        }
-        // bounding radius of the intersector: max over chi of r(chi) on a dense grid,
-        // + 1e-6, rounded up to a multiple of 2^-20 (exact in single precision)
-        double rm = 0;
-        for (int k = 0; k <= 16384; ++k) {
-            const double chi = kPi * k / 16384;
-            double rr = 1 + (double)c.shape.c[0];
-            for (int m = 1; m < 8; ++m) rr += (double)c.shape.c[m] * std::cos(m * chi);
-            if (rr > rm) rm = rr;
-        }
-        c.shape.rmax = sphere ? 1.0f : (float)(std::ceil((rm + 1e-6) * 1048576.0) / 1048576.0);
-        c.shape.terms = 2;
-        for (uint32_t m = 2; m < 8; ++m)
-            if (c.shape.c[m] != 0.0f) c.shape.terms = m + 1;
+        if (!drop_shape_bounds(coef, sphere, c.shape, err)) return false;
         BinFrame f;
         if (!drop_frame(k_in, f)) {
             err = "Incident direction must be finite and nonzero.";

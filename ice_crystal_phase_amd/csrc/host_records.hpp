@@ -27,6 +27,11 @@ bool drop_frame(d3 k, BinFrame& f);
 // reference's interval check is src/projected_area.cpp:50-87).
 bool drop_convex(const double c[8], std::string& err);
 
+// The tracer's view of a drop: coefficients (zero for a sphere), the
+// conservative inner / outer radii, the intersector's bounding radius and the
+// number of Chebyshev terms to evaluate. False (err set) for a non-convex shape.
+bool drop_shape_bounds(const float coef[8], bool sphere, drop::Shape& shape, std::string& err);
+
 // Proper rotation check of a row-major body -> ensemble matrix.
 bool rotation_valid(const double m[9]);
 

@@ -245,6 +245,10 @@ uint32_t iph_struct_size(int which)
     case 6: return (uint32_t)sizeof(iph_ice_state);
     case 7: return (uint32_t)sizeof(iph_atlas_settings);
     case 8: return (uint32_t)sizeof(iph_atlas_node);
+    case 9: return (uint32_t)sizeof(iph_coherent_config);
+    case 10: return (uint32_t)sizeof(iph_coherent_stats);
+    case 11: return (uint32_t)sizeof(iph_coherent_direction);
+    case 12: return (uint32_t)sizeof(iph_coherent_gate);
     default: return 0;
     }
 }
@@ -290,6 +294,8 @@ void iph_ctx_destroy(iph_ctx* ctx)
     for (iph_sur* s : sur_live) iph_sur_destroy(s);
     const std::vector<iph_atlas_mass*> atlas_live = ctx->atlases;
     for (iph_atlas_mass* m : atlas_live) iph_atlas_mass_destroy(m);
+    const std::vector<iph_coherent*> coherent_live = ctx->coherents;
+    for (iph_coherent* c : coherent_live) iph_coherent_destroy(c);
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
@@ -740,12 +746,14 @@ int iph_batch_cdf(iph_batch* b, uint32_t rec, double* phi, double* theta, iph_cd
     return IPH_OK;
 }
 
-int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
-                  double* phi, double* theta, iph_cdf_report* report)
+}  // extern "C"
+
+namespace icephase {
+namespace {
+// The CDF of a host or device mass array (device_mass: already on the context's device).
+int cdf_of_mass(iph_ctx* ctx, const double* mass, bool device_mass, uint32_t nt, uint32_t np,
+                const iph_cdf_policy* policy, double* phi, double* theta, iph_cdf_report* report)
 {
-    if (!ctx) return fail(nullptr, IPH_E_INVALID, "ctx must not be NULL.");
-    if (!mass || !phi || !theta || !nt || !np || (uint64_t)nt * np > 0xffffffffull)
-        return fail(ctx, IPH_E_INVALID, "Invalid CDF arguments.");
     CdfSetup s;
     std::string err;
     if (!resolve_policy(policy, nt, np, s, err)) return fail(ctx, IPH_E_INVALID, err);
@@ -761,14 +769,19 @@ int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, co
     const size_t n = (size_t)nt * np;
     const size_t sz_theta = (size_t)s.onp * (s.ont + 1);
     DevBuf<double> buf;
-    const size_t total = n + e.size() + ce.size() + (s.onp + 1) + sz_theta + s.onp + kCdfScalars + (size_t)pblocks * 8 +
-                         (coarse ? (size_t)s.ont * s.onp : 0);
+    const size_t total = (device_mass ? 0 : n) + e.size() + ce.size() + (s.onp + 1) + sz_theta + s.onp + kCdfScalars +
+                         (size_t)pblocks * 8 + (coarse ? (size_t)s.ont * s.onp : 0);
     IPH_HIP(ctx, buf.alloc(total));
     double* p = buf.p;
     CdfJob J;
     std::memset(&J, 0, sizeof J);
     J.nt = nt; J.np = np; J.ont = s.ont; J.onp = s.onp;
-    double* d_mass = p; p += n;
+    const double* d_mass = mass;
+    if (!device_mass) {
+        IPH_HIP(ctx, hipMemcpyAsync(p, mass, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        d_mass = p;
+        p += n;
+    }
     double* d_e = p; p += e.size();
     double* d_ce = p; p += ce.size();
     J.mass = d_mass; J.edges = d_e; J.cedges = d_ce;
@@ -778,7 +791,6 @@ int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, co
     J.scalars = p; p += kCdfScalars;
     J.partials = p; p += (size_t)pblocks * 8;
     J.coarse = coarse ? p : nullptr;
-    IPH_HIP(ctx, hipMemcpyAsync(d_mass, mass, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     IPH_HIP(ctx, hipMemcpyAsync(d_e, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     IPH_HIP(ctx, hipMemcpyAsync(d_ce, ce.data(), ce.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     DevBuf<CdfJob> dj;
@@ -794,6 +806,25 @@ int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, co
     if (report) *report = rep;
     if (rep.status) return fail(ctx, IPH_E_AUDIT, report_reason(rep.status));
     return IPH_OK;
+}
+}  // namespace
+
+int build_cdf_from_device(iph_ctx* ctx, const double* d_mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
+                          double* phi, double* theta, iph_cdf_report* report)
+{
+    return cdf_of_mass(ctx, d_mass, true, nt, np, policy, phi, theta, report);
+}
+}  // namespace icephase
+
+extern "C" {
+
+int iph_build_cdf(iph_ctx* ctx, const double* mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
+                  double* phi, double* theta, iph_cdf_report* report)
+{
+    if (!ctx) return fail(nullptr, IPH_E_INVALID, "ctx must not be NULL.");
+    if (!mass || !phi || !theta || !nt || !np || (uint64_t)nt * np > 0xffffffffull)
+        return fail(ctx, IPH_E_INVALID, "Invalid CDF arguments.");
+    return cdf_of_mass(ctx, mass, false, nt, np, policy, phi, theta, report);
 }
 
 int iph_build_cdf_noalloc(iph_ctx* ctx, const double* d_mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,

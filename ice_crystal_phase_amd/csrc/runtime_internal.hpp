@@ -17,6 +17,7 @@ struct iph_ctx {
     std::vector<iph_batch*> batches;  // live batches; destroyed with the context
     std::vector<iph_sur*> surrogates; // live surrogates; destroyed with the context
     std::vector<iph_atlas_mass*> atlases;  // live device atlas masses; destroyed with the context
+    std::vector<iph_coherent*> coherents;  // live coherent drop records; destroyed with the context
     unsigned char* cdf_scratch = nullptr;  // device address of this device's fixed CDF scratch (runtime.hip)
 };
 
@@ -74,6 +75,11 @@ inline uint64_t default_tile_rays(uint64_t rays)
 // Grid (theta_count, phi_count) of record rec of a batch; false for an invalid
 // record index.
 bool batch_record_grid(const iph_batch* b, uint32_t rec, uint32_t& nt, uint32_t& np);
+
+// The CDF of FP64 cell masses already in device memory (theta-major nt x np),
+// as iph_build_cdf does for a host array; phi and theta are host outputs.
+int build_cdf_from_device(iph_ctx* ctx, const double* d_mass, uint32_t nt, uint32_t np, const iph_cdf_policy* policy,
+                          double* phi, double* theta, iph_cdf_report* report);
 
 }  // namespace icephase
 

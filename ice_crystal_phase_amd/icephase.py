@@ -105,6 +105,50 @@ class AtlasNode(C.Structure):
 _lib: C.CDLL | None = None
 
 
+class CoherentConfig(C.Structure):
+    _fields_ = [("radius_mm", C.c_float), ("wavelength_nm", C.c_float), ("interior_index", C.c_float),
+                ("reserved", C.c_float), ("inclination_deg", f64), ("grid", u32), ("sphere", u32),
+                ("theta_count", u32), ("phi_count", u32), ("focal_offsets", u32 * 4), ("primary_sigma_deg", f64),
+                ("transition_contrast", f64)]
+
+
+class CoherentStats(C.Structure):
+    _fields_ = [("status_counts", u32 * 8), ("patches", u32), ("folded", u32), ("bad_vertices", u32),
+                ("first_bad_vertex", u32), ("candidates", u64), ("hits", u64), ("ms_trace", f64),
+                ("ms_patches", f64), ("ms_folded", f64), ("ms_query", f64), ("ms_optics", f64),
+                ("ms_diffraction", f64)]
+
+
+class CoherentGate(C.Structure):
+    _fields_ = [("invalid_values", u32), ("incomplete", u32), ("underresolved", u32), ("first_problem", u32),
+                ("maximum", f64)]
+
+
+# Record layouts shared with the reference's drop pipeline (OutgoingVertex,
+# Field32, OutgoingPatch, FoldedPatchRecord, PatchQueryHit) and the per-direction
+# result (iph_coherent_direction).
+VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("bx", "<f4", 3), ("field", "<f4", 4),
+                         ("turns", "<i4"), ("frac", "<f4"), ("status", "<u4"), ("diag", "<u4")])
+FIELD_DTYPE = np.dtype([("field", "<f4", 4)])
+PATCH_DTYPE = np.dtype([("v", "<u4", 4), ("id", "<u4"), ("status", "<u4"), ("omega", "<f4"), ("area", "<f4")])
+BRANCH_DTYPE = np.dtype([("u0", "<f8"), ("u1", "<f8"), ("area", "<f8"), ("omega", "<f8"), ("ru", "<f8"),
+                         ("rv", "<f8"), ("sign", "<i4"), ("focal", "<u4"), ("qturns", "<u4"), ("extra", "<u4")])
+FOLDED_DTYPE = np.dtype([("compact", "<u4"), ("id", "<u4"), ("flags", "<u4"), ("nb", "<u4"), ("jac", "<f8", 4),
+                         ("b", BRANCH_DTYPE, 4), ("err", "<f8"), ("panels", "<u4"), ("pad", "<u4")])
+HIT_DTYPE = np.dtype([("compact", "<u4"), ("id", "<u4"), ("root", "<u4"), ("flags", "<u4"), ("u", "<f4"),
+                      ("v", "<f4"), ("t", "<f4"), ("res", "<f4")])
+DIRECTION_DTYPE = np.dtype([("incoherent_s", "<f8"), ("incoherent_p", "<f8"), ("path_s", "<f8"), ("path_p", "<f8"),
+                            ("focal_s", "<f8"), ("focal_p", "<f8"), ("diffraction_s", "<f8"),
+                            ("diffraction_p", "<f8"), ("family_incoherent", "<f8", 4), ("sigma_rad", "<f8"),
+                            ("blend", "<f8"), ("path_field", "<f8", 8), ("focal_field", "<f8", 8),
+                            ("hit_count", "<u4"), ("evaluated_hits", "<u4"), ("rejected_hits", "<u4"),
+                            ("optical_flags", "<u4"), ("query_flags", "<u4"), ("focal_flags", "<u4"),
+                            ("diffraction_flags", "<u4"), ("transition_flags", "<u4"), ("family_hits", "<u4", 4),
+                            ("transition_kind", "<u4"), ("folded_hits", "<u4"), ("regular_hits", "<u4"),
+                            ("refinement_hits", "<u4")])
+STAGES = {"incoherent": 1, "path": 2, "focal": 3, "diffraction": 4}
+
+
 def lib() -> C.CDLL:
     """Load libicephase.so; raise (never fall back) when it is absent."""
     global _lib
@@ -167,10 +211,30 @@ def lib() -> C.CDLL:
     L.iph_atlas_mass_destroy.argtypes = [C.c_void_p]
     L.iph_atlas_mass_destroy.restype = None
     L.iph_atlas_compose.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.iph_coherent_create.argtypes = [C.c_void_p, P(CoherentConfig), P(C.c_void_p)]
+    L.iph_coherent_destroy.argtypes = [C.c_void_p]
+    L.iph_coherent_destroy.restype = None
+    L.iph_coherent_trace.argtypes = [C.c_void_p]
+    L.iph_coherent_set_vertices.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.iph_coherent_get_vertices.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.iph_coherent_evaluate.argtypes = [C.c_void_p]
+    L.iph_coherent_stats_get.argtypes = [C.c_void_p, P(CoherentStats)]
+    L.iph_coherent_get_patches.argtypes = [C.c_void_p, C.c_void_p, u32, P(u32), C.c_void_p, u32, P(u32)]
+    L.iph_coherent_get_hits.argtypes = [C.c_void_p, u32, u32, C.c_void_p, u64, C.c_void_p, C.c_void_p]
+    L.iph_coherent_get_directions.argtypes = [C.c_void_p, C.c_void_p]
+    L.iph_coherent_density.argtypes = [C.c_void_p, u32, C.c_void_p, P(CoherentGate)]
+    L.iph_coherent_cdf.argtypes = [C.c_void_p, u32, u32, P(CdfPolicy), C.c_void_p, C.c_void_p, P(CdfReport),
+                                   P(CoherentGate)]
     for which, (name, st) in enumerate((("Record", Record), ("HistStats", HistStats), ("CdfReport", CdfReport),
                                         ("CdfPolicy", CdfPolicy), ("WaterState", WaterState), ("Area", Area),
                                         ("IceState", IceState), ("AtlasSettings", AtlasSettings),
-                                        ("AtlasNode", AtlasNode))):
+                                        ("AtlasNode", AtlasNode), ("CoherentConfig", CoherentConfig),
+                                        ("CoherentStats", CoherentStats), ("CoherentDirection", None),
+                                        ("CoherentGate", CoherentGate))):
+        if st is None:
+            if L.iph_struct_size(which) != DIRECTION_DTYPE.itemsize:
+                raise IcePhaseError(-1, f"ABI mismatch for {name}")
+            continue
         if L.iph_struct_size(which) != C.sizeof(st):
             raise IcePhaseError(-1, f"ABI mismatch for {name}: {L.iph_struct_size(which)} != {C.sizeof(st)}")
     if L.iph_abi_version() != ABI_VERSION:
@@ -204,7 +268,10 @@ EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_des
             "iph_sur_set_record", "iph_sur_set_target", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",
             "iph_sur_compare", "iph_sur_records", "iph_sur_timing", "iph_ice_index", "iph_atlas_node_rays",
             "iph_atlas_trace", "iph_write_npy", "iph_atlas_mass_create", "iph_atlas_mass_destroy",
-            "iph_atlas_compose")
+            "iph_atlas_compose", "iph_coherent_create", "iph_coherent_destroy", "iph_coherent_trace",
+            "iph_coherent_set_vertices", "iph_coherent_get_vertices", "iph_coherent_evaluate", "iph_coherent_stats_get",
+            "iph_coherent_get_patches", "iph_coherent_get_hits", "iph_coherent_get_directions",
+            "iph_coherent_density", "iph_coherent_cdf")
 
 
 def u_edges(nt: int) -> np.ndarray:
@@ -464,6 +531,111 @@ class AtlasMass:
 
 
 @dataclass(eq=False)
+class Coherent:
+    """One drop record on the patch model (iph_coherent): the coherent stages
+    path, focal and diffraction of rainbow_generate, and the patch-model
+    incoherent stage."""
+
+    def __init__(self, ctx: "Context", radius_mm: float, wavelength_nm: float, interior_index: float,
+                 grid: int = 129, theta: int = 360, phi: int = 720, inclination_deg: float = 20.0,
+                 sphere: bool = False, focal_offsets=(0, 0, 0, 0), primary_sigma_deg: float = 0.0,
+                 transition_contrast: float = 0.25):
+        self.ctx = ctx
+        self.config = CoherentConfig(radius_mm, wavelength_nm, interior_index, 0.0, inclination_deg, grid,
+                                     1 if sphere else 0, theta, phi, (u32 * 4)(*focal_offsets), primary_sigma_deg,
+                                     transition_contrast)
+        self.grid, self.theta, self.phi = grid, theta, phi
+        self.nv = 4 * grid * grid
+        self._p = C.c_void_p()
+        _check(lib().iph_coherent_create(ctx.ptr, C.byref(self.config), C.byref(self._p)), ctx.ptr)
+        ctx._batches.add(self)
+
+    def trace(self) -> None:
+        _check(lib().iph_coherent_trace(_live(self._p, "coherent record")), self.ctx.ptr)
+
+    def set_vertices(self, vertices: np.ndarray, second: np.ndarray) -> None:
+        v = np.ascontiguousarray(vertices)
+        f = np.ascontiguousarray(second)
+        if v.nbytes != self.nv * VERTEX_DTYPE.itemsize or f.nbytes != self.nv * FIELD_DTYPE.itemsize:
+            raise ValueError("vertex arrays must hold 4 * grid^2 records")
+        _check(lib().iph_coherent_set_vertices(_live(self._p, "coherent record"), v.ctypes.data, f.ctypes.data),
+               self.ctx.ptr)
+
+    def vertices(self):
+        v = np.zeros(self.nv, VERTEX_DTYPE)
+        f = np.zeros(self.nv, FIELD_DTYPE)
+        _check(lib().iph_coherent_get_vertices(_live(self._p, "coherent record"), v.ctypes.data, f.ctypes.data),
+               self.ctx.ptr)
+        return v, f
+
+    def evaluate(self) -> None:
+        _check(lib().iph_coherent_evaluate(_live(self._p, "coherent record")), self.ctx.ptr)
+
+    def stats(self) -> CoherentStats:
+        st = CoherentStats()
+        _check(lib().iph_coherent_stats_get(_live(self._p, "coherent record"), C.byref(st)), self.ctx.ptr)
+        return st
+
+    def patches(self):
+        n, nf = u32(), u32()
+        h = _live(self._p, "coherent record")
+        _check(lib().iph_coherent_get_patches(h, None, 0, C.byref(n), None, 0, C.byref(nf)), self.ctx.ptr)
+        p = np.zeros(n.value, PATCH_DTYPE)
+        f = np.zeros(nf.value, FOLDED_DTYPE)
+        _check(lib().iph_coherent_get_patches(h, p.ctypes.data, n.value, C.byref(n), f.ctypes.data, nf.value,
+                                              C.byref(nf)), self.ctx.ptr)
+        return p, f
+
+    def hits(self, first: int, count: int, cap: int = 1 << 20):
+        h = np.zeros(cap, HIT_DTYPE)
+        counts = np.zeros(count, np.uint32)
+        flags = np.zeros(count, np.uint32)
+        _check(lib().iph_coherent_get_hits(_live(self._p, "coherent record"), first, count, h.ctypes.data, cap,
+                                           counts.ctypes.data, flags.ctypes.data), self.ctx.ptr)
+        return h[:int(counts.sum())], counts, flags
+
+    def directions(self) -> np.ndarray:
+        out = np.zeros(self.theta * self.phi, DIRECTION_DTYPE)
+        _check(lib().iph_coherent_get_directions(_live(self._p, "coherent record"), out.ctypes.data), self.ctx.ptr)
+        return out
+
+    def density(self, stage: str):
+        out = np.zeros(self.theta * self.phi, np.float64)
+        g = CoherentGate()
+        _check(lib().iph_coherent_density(_live(self._p, "coherent record"), STAGES[stage], out.ctypes.data,
+                                          C.byref(g)), self.ctx.ptr)
+        return out.reshape(self.theta, self.phi), g
+
+    def cdf(self, stage: str, allow_underresolved: bool = False, out_theta: int = 0, out_phi: int = 0,
+            max_l1: float = 1e-10, max_lost: float = 1e-12, max_tv: float = 1.0):
+        pol = CdfPolicy(out_theta, out_phi, max_l1, max_lost, max_tv)
+        ont, onp = out_theta or self.theta, out_phi or self.phi
+        phi = np.zeros(onp + 1, np.float64)
+        theta = np.zeros(onp * (ont + 1), np.float64)
+        rep, g = CdfReport(), CoherentGate()
+        _check(lib().iph_coherent_cdf(_live(self._p, "coherent record"), STAGES[stage], 1 if allow_underresolved else 0,
+                                      C.byref(pol), phi.ctypes.data, theta.ctypes.data, C.byref(rep), C.byref(g)),
+               self.ctx.ptr)
+        return phi, theta.reshape(onp, ont + 1), rep, g
+
+    def close(self) -> None:
+        if self._p and self.ctx.ptr:
+            lib().iph_coherent_destroy(self._p)
+        self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Device state of several records (iph_batch)."""
 

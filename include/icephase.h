@@ -400,6 +400,100 @@ int iph_atlas_compose(iph_atlas_mass* m, const double* weights, double* out);
 /* NPY v1.0 '<f8' C-order file of the given shape (src/npy_writer.cpp:18-95). */
 int iph_write_npy(const char* path, const uint64_t* shape, uint32_t ndim, const double* data);
 
+/* ---- Coherent drop stages (rainbow_generate --stage path|focal|diffraction) ----
+ * One drop record on the patch model of the reference's default pipeline
+ * (apps/generate_phase.cpp:426-440): a grid x grid vertex trace through R, TT,
+ * TRT and TRRT with two unit-input Jones response columns and the optical path
+ * in whole turns plus a fraction, bilinear chord patches in direction space
+ * (folded branches where the map changes orientation), the patches that
+ * contain each query direction, and per direction the incoherent, path-
+ * coherent and focal (quarter-turn) intensities, then the Table II local
+ * Gaussian across rainbow transitions. Destroyed with its context if not before. */
+typedef struct iph_coherent iph_coherent;
+enum { IPH_STAGE_INCOHERENT = 1, IPH_STAGE_PATH = 2, IPH_STAGE_FOCAL = 3, IPH_STAGE_DIFFRACTION = 4 };
+
+/* RaindropSettings + WaveOptionsSettings of the reference
+ * (include/rainbow/raindrop_settings.hpp:10-44, include/rainbow/wave_optics_data.hpp:93-99):
+ * incident direction (cos a, -sin a, 0) for inclination a; grid vertices per
+ * axis; query grid theta x phi cell centres. primary_sigma_deg 0 selects Table II
+ * (radius 0.1..1 mm); transition_contrast is 0.25 in the reference. */
+typedef struct iph_coherent_config {
+    float radius_mm, wavelength_nm, interior_index, reserved;
+    double inclination_deg;
+    uint32_t grid, sphere, theta_count, phi_count;
+    uint32_t focal_offsets[4];
+    double primary_sigma_deg;
+    double transition_contrast;
+} iph_coherent_config;
+
+/* Counters and stage times (ms, HIP events) of the last evaluation. */
+typedef struct iph_coherent_stats {
+    uint32_t status_counts[8]; /* patch cells by PatchCellStatus (include/rainbow/outgoing_patch.hpp:15-25) */
+    uint32_t patches, folded, bad_vertices, first_bad_vertex;
+    uint64_t candidates, hits;
+    double ms_trace, ms_patches, ms_folded, ms_query, ms_optics, ms_diffraction;
+} iph_coherent_stats;
+
+/* One query direction (theta-major): the fields of the reference's
+ * PatchOpticalResult, FocalOpticalResult and DiffractionResult that the stages
+ * and their audits read (include/rainbow/patch_optics_data.hpp:46-82,
+ * include/rainbow/wave_optics_data.hpp:31-83). Fields: s re, s im, p re, p im
+ * of response column 0, then column 1. */
+typedef struct iph_coherent_direction {
+    double incoherent_s, incoherent_p, path_s, path_p, focal_s, focal_p, diffraction_s, diffraction_p;
+    double family_incoherent[4];
+    double sigma_rad, blend;
+    double path_field[8];
+    double focal_field[8];
+    uint32_t hit_count, evaluated_hits, rejected_hits, optical_flags;
+    uint32_t query_flags, focal_flags, diffraction_flags, transition_flags;
+    uint32_t family_hits[4];
+    uint32_t transition_kind, folded_hits, regular_hits, refinement_hits;
+} iph_coherent_direction;
+
+/* The CDF input gate of a stage (device/phase_cdf.cu:93-111): directions with a
+ * non-finite or negative density, incomplete directions (rejected or pending
+ * hits, invalid focal or diffraction results), underresolved Table II filters,
+ * the first such direction, and the largest density. */
+typedef struct iph_coherent_gate {
+    uint32_t invalid_values, incomplete, underresolved, first_problem;
+    double maximum;
+} iph_coherent_gate;
+
+int iph_coherent_create(iph_ctx* ctx, const iph_coherent_config* config, iph_coherent** out);
+void iph_coherent_destroy(iph_coherent* h);
+/* Replaces RaindropTracer::trace_unpolarized (src/raindrop_tracer.cpp:144-177). */
+int iph_coherent_trace(iph_coherent* h);
+/* Vertex arrays in the reference's layouts: 4 * grid^2 OutgoingVertex (68 bytes,
+ * path-major, include/rainbow/raindrop_trace_data.hpp:41-53) and as many
+ * second-column Field32 (16 bytes) - the input PatchAccel::build takes
+ * (src/patch_accel.cpp:62-75). set replaces the trace; get reads them back. */
+int iph_coherent_set_vertices(iph_coherent* h, const void* vertices, const void* second);
+int iph_coherent_get_vertices(iph_coherent* h, void* vertices, void* second);
+/* Replaces PatchAccel::build, PatchOptics::prepare_folded, PatchQuery::query_grid
+ * and PatchOptics::evaluate_wave_device (src/patch_accel.cpp:62-140,
+ * src/patch_optics.cpp:264-291, src/patch_query.cpp:137-212,
+ * src/patch_optics.cpp:54-234). IPH_E_AUDIT for failed vertices or invalid
+ * patch sources, as the reference refuses them. */
+int iph_coherent_evaluate(iph_coherent* h);
+int iph_coherent_stats_get(iph_coherent* h, iph_coherent_stats* out);
+/* Compact patches (32-byte OutgoingPatch, patch_id order) and folded records
+ * (320-byte FoldedPatchRecord, compact order); NULL arrays return counts only. */
+int iph_coherent_get_patches(iph_coherent* h, void* patches, uint32_t cap, uint32_t* count, void* folded,
+                             uint32_t folded_cap, uint32_t* folded_count);
+/* Ordered hit lists (32-byte PatchQueryHit) of directions [first, first+count). */
+int iph_coherent_get_hits(iph_coherent* h, uint32_t first, uint32_t count, void* hits, uint64_t cap,
+                          uint32_t* counts, uint32_t* flags);
+int iph_coherent_get_directions(iph_coherent* h, iph_coherent_direction* out);
+/* Density of a stage per direction (out: theta_count * phi_count, may be NULL) and its gate. */
+int iph_coherent_density(iph_coherent* h, uint32_t stage, double* out, iph_coherent_gate* gate);
+/* Replaces PhaseCdf::build for the drop stages (src/phase_cdf.cpp:275-472):
+ * the gate (IPH_E_AUDIT on invalid or incomplete directions, a zero maximum, or
+ * underresolved directions unless allowed), weights (d / max) du, and the CDF
+ * of iph_build_cdf. phi: phi_out + 1, theta: phi_out x (theta_out + 1). */
+int iph_coherent_cdf(iph_coherent* h, uint32_t stage, uint32_t allow_underresolved, const iph_cdf_policy* policy,
+                     double* phi, double* theta, iph_cdf_report* report, iph_coherent_gate* gate);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,7 +44,11 @@ def test_abi_version_and_struct_sizes():
     assert lib.iph_struct_size(6) == ctypes.sizeof(icephase.IceState) == 48
     assert lib.iph_struct_size(7) == ctypes.sizeof(icephase.AtlasSettings)
     assert lib.iph_struct_size(8) == ctypes.sizeof(icephase.AtlasNode)
-    assert lib.iph_struct_size(9) == 0
+    assert lib.iph_struct_size(9) == ctypes.sizeof(icephase.CoherentConfig) == 72
+    assert lib.iph_struct_size(10) == ctypes.sizeof(icephase.CoherentStats)
+    assert lib.iph_struct_size(11) == icephase.DIRECTION_DTYPE.itemsize == 304
+    assert lib.iph_struct_size(12) == ctypes.sizeof(icephase.CoherentGate)
+    assert lib.iph_struct_size(13) == 0
 
 
 def test_u_edges_equal_oracle(oracle):
