@@ -17,6 +17,11 @@ metric, each measured on the device with inputs resident in HBM:
              2^20-2^26 rays each, 900 x 1800 cells): records/s and rays/s of
              one timed pass after one warm-up pass, with the schedule the
              autotuner picks among three on a 1/64-ray proxy;
+  coherent   the coherent drop stages of rainbow_generate's default
+             (--stage diffraction): water drop r = 0.5 mm, 1025 x 1025
+             vertices x 4 path families, 900 x 1800 query cells, per record
+             trace + patches + folded branches + query + optics + Table II +
+             CDF; also at the 41 x 41, 90 x 180 parity size;
   surrogate  the learned surrogate (trained weights in
              ice_crystal_phase_amd/data/surrogate_drop_450x900.npy, when
              present) serving 450 x 900 CDF records, with its total variation
@@ -24,6 +29,8 @@ metric, each measured on the device with inputs resident in HBM:
   cpu_baseline      the drop port (FP32 oracle mirror) on one host core;
   cpu_baseline_hex  the compiled reference hex tracer + compensated histogram
                     (oracle/_ref) on one host core;
+  cpu_baseline_coherent  the compiled reference coherent pipeline (oracle/_ref)
+                    on one host core at the parity size;
   *_all_cores       the same on every host core (at most 16 processes).
 --config uneven runs the uneven batch alone (records/s, with the schedule
 tuned over the full candidate grid); --config surrogate runs the surrogate leg alone;
@@ -31,7 +38,8 @@ tuned over the full candidate grid); --config surrogate runs the surrogate leg a
 rainbow_generate process per condition (records written to disk); --config
 atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
 atlas tool; --config compose times the orientation composition of that atlas
-(records/s) beside the oracle's NumPy sum; --config atomics runs the u64
+(records/s) beside the oracle's NumPy sum; --config coherent times the coherent
+drop stages (rainbow_generate's default diffraction stage) alone; --config atomics runs the u64
 atomicAdd calibration alone (the histogram footprint of --records records).
 
 Prints one JSON line (rank 0). --gpus above 1 exits: the benchmark runs on one GPU.
@@ -201,6 +209,67 @@ def run_sphere(ctx):
         b.run(flags)
         tr, cdf, _ = b.timing()
     return {"rays": 2048 * 2048, "rays_per_s": 2048 * 2048 / (tr / 1e3), "trace_ms": tr, "cdf_ms": cdf}
+
+
+COHERENT_SMALL = dict(grid=41, theta=90, phi=180)
+
+
+def run_coherent(ctx, grid: int = 1025, theta: int = 900, phi: int = 1800, steps: int = 3) -> dict:
+    """Coherent drop stages of rainbow_generate's default (--stage diffraction):
+    water drop r = 0.5 mm at 550 nm, 20 degrees inclination, grid x grid incident
+    vertices, theta x phi query cells; per record: vertex trace, patches, folded
+    branches, patch query, optics, Table II filter, density gate and CDF (allowing
+    the underresolution the 0.22-degree bandwidth has on this grid). Also the
+    same pipeline at the small parity size, for the reference CPU comparison."""
+    import ice_crystal_phase_amd as ip
+    n = float(np.float32(ip.water_optics(550.0).refractive_index))
+
+    def one(g, t, p_, k):
+        with ip.Coherent(ctx, 0.5, 550.0, n, grid=g, theta=t, phi=p_, inclination_deg=20.0) as c:
+            times, stages = [], None
+            for it in range(k + 1):
+                t0 = time.perf_counter()
+                c.trace()
+                c.evaluate()
+                c.cdf("diffraction", allow_underresolved=True)
+                dt = (time.perf_counter() - t0) * 1e3
+                st = c.stats()
+                if it:  # the first pass warms the code and the allocations
+                    times.append(dt)
+                    stages = {k2: getattr(st, k2) for k2 in ("ms_trace", "ms_patches", "ms_folded", "ms_query",
+                                                             "ms_optics", "ms_diffraction")}
+            return min(times), stages, st
+
+    ms, stages, st = one(grid, theta, phi, steps)
+    small_ms, _, _ = one(COHERENT_SMALL["grid"], COHERENT_SMALL["theta"], COHERENT_SMALL["phi"], steps)
+    return {"records_per_s": 1e3 / ms, "ms_per_record": ms, "stage_ms": stages,
+            "patches": st.patches, "folded": st.folded, "candidates": st.candidates, "hits": st.hits,
+            "vertices": grid * grid, "small_ms_per_record": small_ms,
+            "config": f"water drop r=0.5mm 550nm, 20 deg, {grid}x{grid} vertices x 4 families, {theta}x{phi} "
+                      f"query cells, stage diffraction (Table II 0.22 deg), host-timed trace + evaluate + CDF "
+                      f"(min of {steps})"}
+
+
+def cpu_baseline_coherent() -> dict | None:
+    """The reference's coherent pipeline (its CPU trace, patch build, brute-force
+    query of its CPU tests, evaluator and Table II filter, compiled into
+    oracle/_ref) on one host core at the small parity size."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    try:
+        import coherent_ref as cr
+    except Exception:
+        return None
+    if not cr.available():
+        return None
+    cfg = cr.config(radius_mm=0.5, wavelength_nm=550.0, inclination_deg=20.0, grid=COHERENT_SMALL["grid"],
+                    theta=COHERENT_SMALL["theta"], phi=COHERENT_SMALL["phi"])
+    t0 = time.perf_counter()
+    cr.pipeline(cfg)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "records/s", "cores": 1, "kind": "reference",
+            "sample": f"one record at {COHERENT_SMALL['grid']}x{COHERENT_SMALL['grid']} vertices, "
+                      f"{COHERENT_SMALL['theta']}x{COHERENT_SMALL['phi']} query cells (compiled reference "
+                      f"pipeline with the brute-force query of its CPU tests) in {dt:.1f} s"}
 
 
 def run_surrogate(ctx, n_eval: int = 64, eval_grid: int = 16384, tracer_grid: int = 8192, train_steps: int = 3):
@@ -696,7 +765,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
-    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas", "compose", "atomics"),
+    ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas", "compose", "atomics",
+                                         "coherent"),
                     default="drops")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -734,6 +804,22 @@ def main() -> int:
                "scaling": "weak", "vs_baseline": None, "dtype": "f32 drop trace, f64 hex trace, f64 CDF",
                "data": "synthetic", "config": {"workload": u.pop("config")}}
         out.update(u)
+        print(json.dumps(out))
+        return 0
+    if args.config == "coherent":
+        with ip.Context(0) as ctx:
+            co = run_coherent(ctx, steps=args.steps if args.steps != ap.get_default("steps") else 3)
+        out = {"metric": "records/s of the coherent drop stages (rainbow_generate --stage diffraction) on 1 MI355X",
+               "value": co["records_per_s"], "unit": "records/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+               "ms_per_step": co["ms_per_record"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32 trace and patches, f64 folded quadrature / optics / CDF", "data": "synthetic",
+               "config": {"workload": co.pop("config")}}
+        out.update(co)
+        if not args.no_cpu_baseline:
+            cb = cpu_baseline_coherent()
+            if cb:
+                out["cpu_baseline"] = cb
+                out["small_gpu_over_cpu_reference"] = (1e3 / co["small_ms_per_record"]) / cb["value"]
         print(json.dumps(out))
         return 0
     if args.config == "atomics":
@@ -838,6 +924,7 @@ def main() -> int:
         out["hex"] = run_hex(ctx)
         out["sphere"] = run_sphere(ctx)
         out["uneven"] = run_uneven(ctx)
+        out["coherent"] = run_coherent(ctx)
         out["surrogate"] = run_surrogate(ctx)
     if not args.no_cpu_baseline:
         rps, n, dt = cpu_baseline(recs[len(recs) // 2], nt, np_)
@@ -858,6 +945,10 @@ def main() -> int:
             if "hex_reference" in all_cores:
                 out["hex"]["gpu_over_cpu_reference_all_cores"] = (out["hex"]["rays_per_s"] /
                                                                   all_cores["hex_reference"]["value"])
+            cb = cpu_baseline_coherent()
+            if cb:
+                out["cpu_baseline_coherent"] = cb
+                out["coherent"]["small_gpu_over_cpu_reference"] = (1e3 / out["coherent"]["small_ms_per_record"]) / cb["value"]
     ctx.close()
     print(json.dumps(out))
     return 0

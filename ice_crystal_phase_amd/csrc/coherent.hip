@@ -353,7 +353,9 @@ __device__ inline bool ray_meets_box(v3 w, v3 lo, v3 hi)
 }
 
 // Candidate window of each compact patch: the query cells whose centre lies in
-// the cone that bounds the patch's box, widened by one cell on each side.
+// the cone about the box centre that holds the patch's box (the largest angle
+// to a box vertex bounds the angle to every point of the box), widened by
+// 1e-6 rad for the rounding of the stored grid directions.
 extern "C" __global__ void __launch_bounds__(kThreads)
     icephase_coh_extent(const Vertex* __restrict__ V, const Patch* __restrict__ P, uint32_t np_, QueryGrid G,
                         Extent* __restrict__ ext, uint32_t* __restrict__ count)
@@ -382,20 +384,23 @@ extern "C" __global__ void __launch_bounds__(kThreads)
             const double th = acos(fmin(1.0, fmax(-1.0, dot(cd, ax))));
             const double ph = atan2(dot(cd, a1), dot(cd, a0));
             const double rstep = coh::kPi / G.nt, cstep = 2.0 * coh::kPi / G.np;
-            const long r0 = (long)floor((th - rho) / rstep - 0.5) - 1, r1 = (long)ceil((th + rho) / rstep - 0.5) + 1;
+            const long r0 = (long)ceil((th - rho) / rstep - 0.5), r1 = (long)floor((th + rho) / rstep - 0.5);
             const long rlo = r0 < 0 ? 0 : r0, rhi = r1 > (long)G.nt - 1 ? (long)G.nt - 1 : r1;
-            e.row0 = (uint32_t)rlo;
+            e.row0 = (uint32_t)(rlo < (long)G.nt ? rlo : (long)G.nt - 1);
             e.rows = rhi >= rlo ? (uint32_t)(rhi - rlo + 1) : 0u;
             if (th - rho <= 0.0 || th + rho >= coh::kPi || sin(rho) >= sin(th)) {
                 e.col0 = 0;
                 e.cols = G.np;
             } else {
-                const double half = asin(sin(rho) / sin(th));
-                const long c0 = (long)floor((ph - half + coh::kPi) / cstep - 0.5) - 1;
-                const long c1 = (long)ceil((ph + half + coh::kPi) / cstep - 0.5) + 1;
+                const double half = asin(sin(rho) / sin(th)) + 1e-6;
+                const long c0 = (long)ceil((ph - half + coh::kPi) / cstep - 0.5);
+                const long c1 = (long)floor((ph + half + coh::kPi) / cstep - 0.5);
                 if (c1 - c0 + 1 >= (long)G.np) {
                     e.col0 = 0;
                     e.cols = G.np;
+                } else if (c1 < c0) {
+                    e.col0 = 0;
+                    e.cols = 0;
                 } else {
                     e.col0 = (uint32_t)(((c0 % (long)G.np) + (long)G.np) % (long)G.np);
                     e.cols = (uint32_t)(c1 - c0 + 1);
@@ -762,9 +767,18 @@ __device__ inline void nan_diff(Diffracted& o)
 
 // Local spherical Gaussian of the nearest transition's bandwidth on the total
 // focal intensity, tapered between the inner blend and the support radius.
+// Rows holding at least one transition or unknown flag, counted per row.
+extern "C" __global__ void __launch_bounds__(kThreads)
+    icephase_coh_rowflags(const uint32_t* __restrict__ trans, uint32_t nt, uint32_t np, uint32_t* __restrict__ rows)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)nt * np || trans[i] == 0u) return;
+    atomicAdd(&rows[i / np], 1u);
+}
+
 extern "C" __global__ void __launch_bounds__(128)
-    icephase_coh_filter(const Optics* __restrict__ R, const uint32_t* __restrict__ trans, DiffConfig C,
-                        Diffracted* __restrict__ out)
+    icephase_coh_filter(const Optics* __restrict__ R, const uint32_t* __restrict__ trans,
+                        const uint64_t* __restrict__ rowoff, DiffConfig C, Diffracted* __restrict__ out)
 {
     const uint64_t index = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (index >= (uint64_t)C.nt * C.np) return;
@@ -790,6 +804,11 @@ extern "C" __global__ void __launch_bounds__(128)
     const double search = 2 * C.sigma * support;
     const int lo = imax(0, (int)floor((t - search) / dt) - 1);
     const int hi = imin(nr - 2, (int)ceil((t + search) / dt));
+    // no flag anywhere in the searched rows: the search would find nothing
+    if (hi < lo || rowoff[hi + 1] == rowoff[lo]) {
+        out[index] = o;
+        return;
+    }
     double best = support + 1;
     for (int rr = lo; rr <= hi; ++rr) {
         const double tr = (rr + 1.0) * dt;
@@ -953,14 +972,23 @@ extern "C" __global__ void __launch_bounds__(kThreads)
 }
 extern "C" __global__ void __launch_bounds__(64) icephase_scan_sums(uint64_t* __restrict__ sums, uint32_t nb)
 {
-    if (threadIdx.x != 0) return;
-    uint64_t run = 0;
-    for (uint32_t b = 0; b < nb; ++b) {
+    // one wave: lane l owns a contiguous chunk of the block sums
+    const uint32_t l = threadIdx.x, chunk = (nb + 63) / 64;
+    const uint32_t b0 = l * chunk < nb ? l * chunk : nb, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    uint64_t own = 0;
+    for (uint32_t b = b0; b < b1; ++b) own += sums[b];
+    uint64_t incl = own;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t up = __shfl_up(incl, off, 64);
+        if ((int)l >= off) incl += up;
+    }
+    uint64_t run = incl - own;
+    for (uint32_t b = b0; b < b1; ++b) {
         const uint64_t v = sums[b];
         sums[b] = run;
         run += v;
     }
-    sums[nb] = run;
+    if (l == 63) sums[nb] = incl;
 }
 extern "C" __global__ void __launch_bounds__(kThreads)
     icephase_scan_apply(const uint32_t* __restrict__ in, uint64_t n, const uint64_t* __restrict__ sums,
@@ -1024,7 +1052,8 @@ struct iph_coherent {
     DevBuf<QHit> H;
     DevBuf<Summary> S;
     DevBuf<Optics> R;
-    DevBuf<uint32_t> trans;
+    DevBuf<uint32_t> trans, rowcnt;
+    DevBuf<uint64_t> rowoff;
     DevBuf<Diffracted> D;
     DevBuf<double> density, weights, edges;
     DevBuf<unsigned long long> maxbits;
@@ -1456,10 +1485,16 @@ int iph_coherent_evaluate(iph_coherent* h)
         Timer t(s);
         IPH_HIP(ctx, h->trans.alloc(ndir));
         IPH_HIP(ctx, h->D.alloc(ndir));
+        IPH_HIP(ctx, h->rowcnt.alloc(h->grid.nt));
+        IPH_HIP(ctx, h->rowoff.alloc((size_t)h->grid.nt + 1));
         hipLaunchKernelGGL(icephase_coh_detect, dim3(blocks_for(ndir, kThreads)), dim3(kThreads), 0, s, h->R.p,
                            h->diff, h->trans.p);
+        IPH_HIP(ctx, fill_u32(h, h->rowcnt.p, h->grid.nt, 0));
+        hipLaunchKernelGGL(icephase_coh_rowflags, dim3(blocks_for(ndir, kThreads)), dim3(kThreads), 0, s,
+                           h->trans.p, h->grid.nt, h->grid.np, h->rowcnt.p);
+        IPH_HIP(ctx, scan(h, h->rowcnt.p, h->grid.nt, h->rowoff.p));
         hipLaunchKernelGGL(icephase_coh_filter, dim3(blocks_for(ndir, 128)), dim3(128), 0, s, h->R.p, h->trans.p,
-                           h->diff, h->D.p);
+                           h->rowoff.p, h->diff, h->D.p);
         IPH_HIP(ctx, hipGetLastError());
         st.ms_diffraction = t.stop();
     }
