@@ -312,6 +312,12 @@ struct QueryGrid {
 struct Extent {
     uint32_t row0, rows, col0, cols;
 };
+// What the query reads of a compact patch: corner directions, padded box, id.
+struct PatchGeom {
+    v3 c[4];
+    v3 lo, hi;
+    uint32_t id, pad[3];
+};
 
 __device__ inline void patch_corners(const Vertex* V, const Patch& P, v3 (&c)[4])
 {
@@ -358,13 +364,21 @@ __device__ inline bool ray_meets_box(v3 w, v3 lo, v3 hi)
 // 1e-6 rad for the rounding of the stored grid directions.
 extern "C" __global__ void __launch_bounds__(kThreads)
     icephase_coh_extent(const Vertex* __restrict__ V, const Patch* __restrict__ P, uint32_t np_, QueryGrid G,
-                        Extent* __restrict__ ext, uint32_t* __restrict__ count)
+                        Extent* __restrict__ ext, uint32_t* __restrict__ count, PatchGeom* __restrict__ geom)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np_) return;
     v3 c[4], lo, hi;
-    patch_corners(V, P[k], c);
+    const Patch Pk = P[k];
+    patch_corners(V, Pk, c);
     patch_box(c, lo, hi);
+    PatchGeom g;
+    for (int i = 0; i < 4; ++i) g.c[i] = c[i];
+    g.lo = lo;
+    g.hi = hi;
+    g.id = Pk.id;
+    g.pad[0] = g.pad[1] = g.pad[2] = 0;
+    geom[k] = g;
     const coh::d3 mid = coh::d3{0.5 * ((double)lo.x + hi.x), 0.5 * ((double)lo.y + hi.y), 0.5 * ((double)lo.z + hi.z)};
     const double ml = length(mid);
     Extent e{0, G.nt, 0, G.np};
@@ -415,8 +429,7 @@ extern "C" __global__ void __launch_bounds__(kThreads)
 }
 
 struct QueryPass {
-    const Vertex* V;
-    const Patch* P;
+    const PatchGeom* G;
     const Extent* ext;
     const uint64_t* coff;   // candidate offsets, np_ + 1
     const v3* dirs;         // nt * np query directions
@@ -428,23 +441,43 @@ struct QueryPass {
     const uint64_t* hoff;   // fill pass: hit offsets per direction
     unsigned int* cursor;   // fill pass: per-direction fill counters
     QHit* out;              // fill pass: hit storage
+    struct Staged* stage;   // count pass: hits in discovery order (up to cap)
+    unsigned long long* staged;
+    uint64_t cap;
     uint32_t fill;
+};
+
+// A hit found by the count pass with its direction, kept so that the hits need
+// not be recomputed to be stored.
+struct Staged {
+    uint32_t dir;
+    QHit hit;
 };
 
 extern "C" __global__ void __launch_bounds__(kThreads) icephase_coh_query(QueryPass Q)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < Q.total; k += stride) {
-        // the patch whose candidate range holds k
-        uint32_t lo = 0, hi = Q.npatch;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (Q.coff[mid] <= k)
-                lo = mid;
-            else
-                hi = mid;
+    const uint32_t lane = threadIdx.x & 63;
+    // every lane of a wave walks the same iterations (the bound is per wave),
+    // so the wave's first candidate is searched once and broadcast
+    for (uint64_t wk = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wk < Q.total; wk += stride) {
+        const uint64_t k = wk + lane;
+        uint32_t base = 0;
+        if (lane == 0) {
+            uint32_t lo = 0, hi = Q.npatch;  // the patch whose candidate range holds wk
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (Q.coff[mid] <= wk)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            base = lo;
         }
-        const uint32_t p = lo;
+        base = __shfl(base, 0, 64);
+        if (k >= Q.total) continue;
+        uint32_t p = base;  // consecutive candidates: step forward to the patch of k
+        while (p + 1 < Q.npatch && Q.coff[p + 1] <= k) ++p;
         const Extent e = Q.ext[p];
         const uint64_t r = k - Q.coff[p];
         const uint32_t row = e.row0 + (uint32_t)(r / e.cols);
@@ -452,23 +485,32 @@ extern "C" __global__ void __launch_bounds__(kThreads) icephase_coh_query(QueryP
         if (col >= Q.np) col -= Q.np;
         const uint64_t di = (uint64_t)row * Q.np + col;
         const v3 w = Q.dirs[di];
-        const Patch P = Q.P[p];
-        v3 c[4], blo, bhi;
-        patch_corners(Q.V, P, c);
-        patch_box(c, blo, bhi);
-        if (!ray_meets_box(w, blo, bhi)) continue;
+        const PatchGeom& g = Q.G[p];
+        if (!ray_meets_box(w, g.lo, g.hi)) continue;
+        v3 c[4];
+        for (int i = 0; i < 4; ++i) c[i] = g.c[i];
         const PHits h = bilinear_intersect(c, w);
+        const uint32_t pid = g.id;
         if (!Q.fill) {
             if (h.flags) atomicOr(&Q.notes[di], h.flags);
-            if (h.flags & (kQueryUnresolved | kQueryInvalid)) atomicMin(&Q.first[di], P.id);
-            if (h.count) atomicAdd(&Q.hits[di], h.count);
+            if (h.flags & (kQueryUnresolved | kQueryInvalid)) atomicMin(&Q.first[di], pid);
+            if (h.count) {
+                atomicAdd(&Q.hits[di], h.count);
+                const unsigned long long slot = atomicAdd(Q.staged, (unsigned long long)h.count);
+                for (uint32_t j = 0; j < h.count; ++j) {
+                    if (slot + j >= Q.cap) break;  // over capacity: the host recomputes with the fill pass
+                    Staged& z = Q.stage[slot + j];
+                    z.dir = (uint32_t)di;
+                    z.hit = QHit{p, pid, j, h.h[j].flags, h.h[j].u, h.h[j].v, h.h[j].t, h.h[j].res};
+                }
+            }
         } else {
             for (uint32_t j = 0; j < h.count; ++j) {
                 const uint32_t at = atomicAdd(&Q.cursor[di], 1u);
                 if (at >= Q.hits[di]) continue;  // more than counted: the finish pass flags the mismatch
                 QHit q;
                 q.compact = p;
-                q.id = P.id;
+                q.id = pid;
                 q.root = j;
                 q.flags = h.h[j].flags;
                 q.u = h.h[j].u;
@@ -479,6 +521,20 @@ extern "C" __global__ void __launch_bounds__(kThreads) icephase_coh_query(QueryP
             }
         }
     }
+}
+
+// Staged hits to their direction's segment (slots by per-direction counters;
+// the finish pass restores the order).
+extern "C" __global__ void __launch_bounds__(kThreads)
+    icephase_coh_scatter(const Staged* __restrict__ stage, uint64_t n, const uint64_t* __restrict__ hoff,
+                         const unsigned int* __restrict__ cnt, unsigned int* __restrict__ cursor, QHit* __restrict__ out)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Staged z = stage[k];
+    const uint32_t at = atomicAdd(&cursor[z.dir], 1u);
+    if (at >= cnt[z.dir]) return;  // the finish pass flags the mismatch
+    out[hoff[z.dir] + at] = z.hit;
 }
 
 struct Summary {
@@ -1048,8 +1104,11 @@ struct iph_coherent {
     DevBuf<Patch> P;
     DevBuf<Folded> rec;
     DevBuf<Extent> ext;
+    DevBuf<PatchGeom> geom;
     DevBuf<unsigned int> qcnt, qnotes, qfirst, qcursor, counters;
     DevBuf<QHit> H;
+    DevBuf<Staged> stage;
+    DevBuf<unsigned long long> staged;
     DevBuf<Summary> S;
     DevBuf<Optics> R;
     DevBuf<uint32_t> trans, rowcnt;
@@ -1068,16 +1127,22 @@ namespace {
 
 uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
+// Reallocate only when the buffer is smaller than n (records of one size reuse their buffers).
+template <class T>
+hipError_t grow(DevBuf<T>& b, size_t n)
+{
+    if (b.p && b.n >= n) return hipSuccess;
+    return b.alloc(n);
+}
+
 // exclusive scan of n u32 values into out[0..n] (u64), on the context stream
 hipError_t scan(iph_coherent* h, const uint32_t* in, uint64_t n, uint64_t* out)
 {
     const uint64_t per = (uint64_t)kThreads * 4;
     const uint32_t nb = (uint32_t)((n + per - 1) / per);
     const uint32_t nbx = nb ? nb : 1;
-    if (h->sums.n < (size_t)nbx + 1) {
-        const hipError_t e = h->sums.alloc((size_t)nbx + 1);
-        if (e != hipSuccess) return e;
-    }
+    const hipError_t ge = grow(h->sums, (size_t)nbx + 1);
+    if (ge != hipSuccess) return ge;
     hipStream_t s = h->ctx->stream;
     if (n == 0) {
         const uint64_t z = 0;
@@ -1345,8 +1410,8 @@ int iph_coherent_evaluate(iph_coherent* h)
         IPH_HIP(ctx, fill_u32(h, h->counters.p + 9, 1, 0xffffffffu));
         hipLaunchKernelGGL(icephase_coh_validate, dim3(blocks_for(h->nv, kThreads)), dim3(kThreads), 0, s, h->V.p,
                            h->F2.p, h->nv, h->counters.p + 8, h->counters.p + 9);
-        IPH_HIP(ctx, h->flag.alloc(h->L.cells));
-        IPH_HIP(ctx, h->offsets.alloc((size_t)h->L.cells + 1));
+        IPH_HIP(ctx, grow(h->flag, h->L.cells));
+        IPH_HIP(ctx, grow(h->offsets, (size_t)h->L.cells + 1));
         hipLaunchKernelGGL(icephase_coh_classify, dim3(blocks_for(h->L.cells, kThreads)), dim3(kThreads), 0, s, h->V.p,
                            h->L, h->flag.p, h->counters.p);
         IPH_HIP(ctx, hipGetLastError());
@@ -1366,10 +1431,10 @@ int iph_coherent_evaluate(iph_coherent* h)
         h->npatch = (uint32_t)np64;
         st.patches = h->npatch;
         const uint32_t npx = h->npatch ? h->npatch : 1;
-        IPH_HIP(ctx, h->P.alloc(npx));
-        IPH_HIP(ctx, h->fold_flag.alloc(npx));
-        IPH_HIP(ctx, h->fold_slot.alloc((size_t)npx + 1));
-        IPH_HIP(ctx, h->lookup.alloc(npx));
+        IPH_HIP(ctx, grow(h->P, npx));
+        IPH_HIP(ctx, grow(h->fold_flag, npx));
+        IPH_HIP(ctx, grow(h->fold_slot, (size_t)npx + 1));
+        IPH_HIP(ctx, grow(h->lookup, npx));
         hipLaunchKernelGGL(icephase_coh_compact, dim3(blocks_for(h->L.cells, kThreads)), dim3(kThreads), 0, s, h->V.p,
                            h->L, h->flag.p, h->offsets.p, h->P.p, h->fold_flag.p);
         IPH_HIP(ctx, hipGetLastError());
@@ -1384,7 +1449,7 @@ int iph_coherent_evaluate(iph_coherent* h)
         IPH_HIP(ctx, hipStreamSynchronize(s));
         h->nfold = (uint32_t)nf;
         st.folded = h->nfold;
-        IPH_HIP(ctx, h->rec.alloc(h->nfold ? h->nfold : 1));
+        IPH_HIP(ctx, grow(h->rec, h->nfold ? h->nfold : 1));
         if (h->npatch)
             hipLaunchKernelGGL(icephase_coh_fold, dim3(blocks_for(h->npatch, 64)), dim3(64), 0, s, h->V.p, h->nv,
                                h->P.p, h->npatch, h->fold_flag.p, h->fold_slot.p, h->tp.w, h->focal, 1u, h->fold,
@@ -1397,30 +1462,30 @@ int iph_coherent_evaluate(iph_coherent* h)
     {
         Timer t(s);
         const uint32_t npx = h->npatch ? h->npatch : 1;
-        IPH_HIP(ctx, h->ext.alloc(npx));
-        IPH_HIP(ctx, h->cand_count.alloc(npx));
-        IPH_HIP(ctx, h->coff.alloc((size_t)npx + 1));
+        IPH_HIP(ctx, grow(h->ext, npx));
+        IPH_HIP(ctx, grow(h->geom, npx));
+        IPH_HIP(ctx, grow(h->cand_count, npx));
+        IPH_HIP(ctx, grow(h->coff, (size_t)npx + 1));
         if (h->npatch)
             hipLaunchKernelGGL(icephase_coh_extent, dim3(blocks_for(h->npatch, kThreads)), dim3(kThreads), 0, s,
-                               h->V.p, h->P.p, h->npatch, h->grid, h->ext.p, h->cand_count.p);
+                               h->V.p, h->P.p, h->npatch, h->grid, h->ext.p, h->cand_count.p, h->geom.p);
         IPH_HIP(ctx, scan(h, h->cand_count.p, h->npatch, h->coff.p));
         uint64_t total = 0;
         IPH_HIP(ctx, hipMemcpyAsync(&total, h->coff.p + h->npatch, sizeof total, hipMemcpyDeviceToHost, s));
         IPH_HIP(ctx, hipStreamSynchronize(s));
         h->candidates = total;
         st.candidates = total;
-        IPH_HIP(ctx, h->qcnt.alloc(ndir));
-        IPH_HIP(ctx, h->qnotes.alloc(ndir));
-        IPH_HIP(ctx, h->qfirst.alloc(ndir));
-        IPH_HIP(ctx, h->qcursor.alloc(ndir));
-        IPH_HIP(ctx, h->hoff.alloc(ndir + 1));
+        IPH_HIP(ctx, grow(h->qcnt, ndir));
+        IPH_HIP(ctx, grow(h->qnotes, ndir));
+        IPH_HIP(ctx, grow(h->qfirst, ndir));
+        IPH_HIP(ctx, grow(h->qcursor, ndir));
+        IPH_HIP(ctx, grow(h->hoff, ndir + 1));
         IPH_HIP(ctx, fill_u32(h, h->qcnt.p, ndir, 0));
         IPH_HIP(ctx, fill_u32(h, h->qnotes.p, ndir, 0));
         IPH_HIP(ctx, fill_u32(h, h->qfirst.p, ndir, 0xffffffffu));
         IPH_HIP(ctx, fill_u32(h, h->qcursor.p, ndir, 0));
         QueryPass Q{};
-        Q.V = h->V.p;
-        Q.P = h->P.p;
+        Q.G = h->geom.p;
         Q.ext = h->ext.p;
         Q.coff = h->coff.p;
         Q.dirs = h->dirs.p;
@@ -1430,6 +1495,14 @@ int iph_coherent_evaluate(iph_coherent* h)
         Q.hits = h->qcnt.p;
         Q.notes = h->qnotes.p;
         Q.first = h->qfirst.p;
+        // staging for the count pass: four hits per direction to start with,
+        // grown to the last record's count
+        IPH_HIP(ctx, grow(h->stage, std::max<uint64_t>(std::max<uint64_t>(4 * ndir, h->nhits), 1u << 20)));
+        IPH_HIP(ctx, grow(h->staged, 1));
+        IPH_HIP(ctx, hipMemsetAsync(h->staged.p, 0, sizeof(unsigned long long), s));
+        Q.stage = h->stage.p;
+        Q.staged = h->staged.p;
+        Q.cap = h->stage.n;
         const uint32_t qblocks = (uint32_t)std::min<uint64_t>(blocks_for(total ? total : 1, kThreads),
                                                               (uint64_t)ctx->cus * 16);
         if (total) hipLaunchKernelGGL(icephase_coh_query, dim3(qblocks), dim3(kThreads), 0, s, Q);
@@ -1440,14 +1513,20 @@ int iph_coherent_evaluate(iph_coherent* h)
         IPH_HIP(ctx, hipStreamSynchronize(s));
         h->nhits = nh;
         st.hits = nh;
-        IPH_HIP(ctx, h->H.alloc(nh ? nh : 1));
-        Q.fill = 1;
-        Q.hoff = h->hoff.p;
-        Q.cursor = h->qcursor.p;
-        Q.out = h->H.p;
-        if (total) hipLaunchKernelGGL(icephase_coh_query, dim3(qblocks), dim3(kThreads), 0, s, Q);
+        IPH_HIP(ctx, grow(h->H, nh ? nh : 1));
+        if (nh <= Q.cap) {
+            if (nh)
+                hipLaunchKernelGGL(icephase_coh_scatter, dim3(blocks_for(nh, kThreads)), dim3(kThreads), 0, s,
+                                   h->stage.p, nh, h->hoff.p, h->qcnt.p, h->qcursor.p, h->H.p);
+        } else {
+            Q.fill = 1;
+            Q.hoff = h->hoff.p;
+            Q.cursor = h->qcursor.p;
+            Q.out = h->H.p;
+            if (total) hipLaunchKernelGGL(icephase_coh_query, dim3(qblocks), dim3(kThreads), 0, s, Q);
+        }
         IPH_HIP(ctx, hipGetLastError());
-        IPH_HIP(ctx, h->S.alloc(ndir));
+        IPH_HIP(ctx, grow(h->S, ndir));
         hipLaunchKernelGGL(icephase_coh_finish, dim3(blocks_for(ndir, kThreads)), dim3(kThreads), 0, s, h->H.p,
                            h->hoff.p, h->qcnt.p, h->qcursor.p, h->qnotes.p, h->qfirst.p, h->P.p, h->npatch, ndir,
                            h->S.p);
@@ -1457,7 +1536,7 @@ int iph_coherent_evaluate(iph_coherent* h)
     // optics
     {
         Timer t(s);
-        IPH_HIP(ctx, h->R.alloc(ndir));
+        IPH_HIP(ctx, grow(h->R, ndir));
         OpticsPass O{};
         O.V = h->V.p;
         O.F2 = h->F2.p;
@@ -1483,10 +1562,10 @@ int iph_coherent_evaluate(iph_coherent* h)
     // diffraction (needs a sigma: Table II range or an explicit one)
     {
         Timer t(s);
-        IPH_HIP(ctx, h->trans.alloc(ndir));
-        IPH_HIP(ctx, h->D.alloc(ndir));
-        IPH_HIP(ctx, h->rowcnt.alloc(h->grid.nt));
-        IPH_HIP(ctx, h->rowoff.alloc((size_t)h->grid.nt + 1));
+        IPH_HIP(ctx, grow(h->trans, ndir));
+        IPH_HIP(ctx, grow(h->D, ndir));
+        IPH_HIP(ctx, grow(h->rowcnt, h->grid.nt));
+        IPH_HIP(ctx, grow(h->rowoff, (size_t)h->grid.nt + 1));
         hipLaunchKernelGGL(icephase_coh_detect, dim3(blocks_for(ndir, kThreads)), dim3(kThreads), 0, s, h->R.p,
                            h->diff, h->trans.p);
         IPH_HIP(ctx, fill_u32(h, h->rowcnt.p, h->grid.nt, 0));
@@ -1626,8 +1705,8 @@ int iph_coherent_density(iph_coherent* h, uint32_t stage, double* out, iph_coher
     IPH_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     const uint64_t n = (uint64_t)h->grid.nt * h->grid.np;
-    IPH_HIP(ctx, h->density.alloc(n));
-    IPH_HIP(ctx, h->maxbits.alloc(1));
+    IPH_HIP(ctx, grow(h->density, n));
+    IPH_HIP(ctx, grow(h->maxbits, 1));
     IPH_HIP(ctx, hipMemsetAsync(h->counters.p, 0, 16 * sizeof(unsigned int), s));
     IPH_HIP(ctx, fill_u32(h, h->counters.p + 3, 1, 0xffffffffu));
     IPH_HIP(ctx, hipMemsetAsync(h->maxbits.p, 0, sizeof(unsigned long long), s));
@@ -1667,7 +1746,7 @@ int iph_coherent_cdf(iph_coherent* h, uint32_t stage, uint32_t allow_underresolv
                         (g.underresolved && !allow_underresolved ? " (diffraction is underresolved on this query grid)"
                                                                  : ""));
     const uint64_t n = (uint64_t)h->grid.nt * h->grid.np;
-    IPH_HIP(ctx, h->weights.alloc(n));
+    IPH_HIP(ctx, grow(h->weights, n));
     hipLaunchKernelGGL(icephase_coh_weights, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
                        h->density.p, h->edges.p, h->grid.np, n, h->maxbits.p, h->weights.p);
     IPH_HIP(ctx, hipGetLastError());
