@@ -5,11 +5,15 @@
 // and renamed), the same error convention ("rainbow_generate: <message>" on
 // stderr, exit 1).
 //
-// The density is the incoherent ray histogram of the MI355X kernels (the
-// reference's patch solver computes the same estimand; its coherent stages
-// path/focal/diffraction are not built here and are rejected by name). Flags
-// added by this tool: --shape-coefficients c0,..,c7 (overrides Table I),
-// --max-internal-hits K, --tail-tolerance E.
+// --stage path|focal|diffraction (the default, as in the reference) run the
+// patch-model pipeline on the device (iph_coherent_*: vertex trace, patches,
+// folded branches, patch query, coherent optics, focal quarter turns, Table II
+// filter, CDF), --grid counting incident vertices per axis (default 129).
+// --stage incoherent is the ray histogram of the MI355X kernels, the same
+// estimand as the reference's incoherent patch stage, --grid counting rays per
+// axis (default 513). Flags added by this tool, for the incoherent stage only:
+// --shape-coefficients c0,..,c7 (overrides Table I), --max-internal-hits K,
+// --tail-tolerance E.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -28,14 +32,16 @@ constexpr double kPi = 3.141592653589793238462643383279502884;
 
 struct Command {
     std::string out;
-    float radius_mm = 1.0f;
+    float radius_mm = 0.4f;  // RaindropSettings default (include/rainbow/raindrop_settings.hpp:12)
     float wavelength_nm = 0;
     double wavelength_requested = 0;
     bool wavelength_given = false, sphere = false, allow_underresolved = false;
     bool water = true, index_given = false, state_given = false;
     double ior = 0, celsius = 20.0, pascal = 101325.0;
-    uint32_t grid = 513, theta = 360, phi = 720, cdf_theta = 0, cdf_phi = 0;
-    double inclination = 20.0, max_tv = 1.0, max_l1 = 1e-10, max_lost = 1e-12, transition_contrast = 0;
+    uint32_t grid = 0, theta = 360, phi = 720, cdf_theta = 0, cdf_phi = 0;
+    double inclination = 20.0, max_tv = 1.0, max_l1 = 1e-10, max_lost = 1e-12, transition_contrast = 0.25;
+    bool histogram_options = false;  // --shape-coefficients / --max-internal-hits / --tail-tolerance given
+    double paper_sigma = 0;          // Table II primary sigma (degrees) for focal / diffraction
     int device = 0;
     std::string stage = "diffraction";  // the reference default (apps/generate_phase.cpp:54)
     unsigned focal[4] = {0, 0, 0, 0};
@@ -49,17 +55,45 @@ struct Command {
     float shape[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
+// Table II of Sadeghi et al. (2012): primary bandwidth (degrees) by drop radius
+// (mm), linearly interpolated; 0 outside 0.1..1 mm (include/rainbow/rainbow_diffraction.hpp:17-29).
+double table_sigma_degrees(float radius)
+{
+    static const float r[10] = {0.1f, 0.2f, 0.3f, 0.4f, 0.5f, 0.6f, 0.7f, 0.8f, 0.9f, 1.0f};
+    static const double s[10] = {0.70, 0.45, 0.30, 0.25, 0.22, 0.20, 0.18, 0.17, 0.16, 0.15};
+    if (!(radius >= r[0] && radius <= r[9])) return 0;
+    for (unsigned j = 0; j < 10; ++j) {
+        if (radius == r[j]) return s[j];
+        if (j && radius < r[j]) return std::fma(((double)radius - r[j - 1]) / ((double)r[j] - r[j - 1]), s[j] - s[j - 1], s[j - 1]);
+    }
+    return 0;
+}
+
+// The solver's conservative outer radius in drop units, FP32 rounded outward
+// (include/rainbow/raindrop_shape.hpp:45-64): the incident grid spans 1.01 of
+// it and the reference planes lie at twice it.
+float outer_radius(const float* c)
+{
+    float sum = 0.0f;
+    for (int k = 1; k < 8; ++k) sum = std::nextafter(sum + std::fabs(c[k]), INFINITY);
+    return std::nextafter(1.0f + c[0] + sum + 0x1p-20f, INFINITY);
+}
+
 void usage()
 {
     std::cout << "rainbow_generate --out DIRECTORY [--modules DIRECTORY] [--sphere]\n"
-                 "  --radius-mm R --wavelength-nm L (required, vacuum nm) --grid RAYS_PER_AXIS\n"
+                 "  --radius-mm R --wavelength-nm L (required, vacuum nm) --grid VERTICES_PER_AXIS\n"
                  "  --inclination-deg A --query-theta T --query-phi P\n"
-                 "  --stage incoherent|path|focal|diffraction (default diffraction; only incoherent is built)\n"
+                 "  --stage diffraction|focal|incoherent|path (default diffraction)\n"
                  "  --material water (default) --temperature-c T --pressure-pa P\n"
                  "  --material constant --ior N (explicit nondispersive reference)\n"
                  "  --cdf-max-l1 E --cdf-max-lost-mass E --device ORDINAL\n"
                  "  --cdf-theta T --cdf-phi P (integer divisors of query dimensions)\n"
                  "  --max-coarsening-tv E (quality policy, not a blur width)\n"
+                 "  --focal-offsets 0,0,0,0\n"
+                 "  diffraction width: Table II radius lookup, secondary = 2*primary\n"
+                 "  --allow-underresolved  (explicitly accept flagged angular underresolution)\n"
+                 "  --stage incoherent only (ray histogram, --grid = rays per axis, default 513):\n"
                  "  --shape-coefficients c0,c1,...  (up to 8; default Table I by radius)\n"
                  "  --max-internal-hits K (default 8) --tail-tolerance E (default 0)\n"
                  "  No additional/global storage Gaussian is applied.\n";
@@ -172,11 +206,14 @@ bool parse(int argc, char** argv, Command& c)
                 begin = end + 1;
             }
             c.coefficients_given = true;
-        } else if (option == "--max-internal-hits")
+            c.histogram_options = true;
+        } else if (option == "--max-internal-hits") {
             c.max_hits = cli::positive_u32(value);
-        else if (option == "--tail-tolerance")
+            c.histogram_options = true;
+        } else if (option == "--tail-tolerance") {
             c.tail = cli::number(value);
-        else
+            c.histogram_options = true;
+        } else
             throw std::invalid_argument("Unknown option: " + option);
     }
     if (c.out.empty()) throw std::invalid_argument("--out is required.");
@@ -196,6 +233,20 @@ bool parse(int argc, char** argv, Command& c)
         c.interior_index = (float)c.ior;
         if (!(c.interior_index > 0)) throw std::invalid_argument("Index underflows solver precision.");
     }
+    const bool coherent = c.stage != "incoherent";
+    if (coherent && c.histogram_options)
+        throw std::invalid_argument(
+            "--shape-coefficients, --max-internal-hits and --tail-tolerance apply to --stage incoherent only.");
+    if (!c.grid) c.grid = coherent ? 129u : 513u;
+    if (coherent && (c.grid < 2 || (uint64_t)c.grid * c.grid > 0xffffffffull / 4))
+        throw std::invalid_argument("Invalid grid dimensions or 32-bit path-major index overflow.");
+    if (c.stage == "focal" || c.stage == "diffraction") {
+        c.paper_sigma = table_sigma_degrees(c.radius_mm);
+        if (!(c.paper_sigma > 0))
+            throw std::invalid_argument(
+                "Table II covers radius 0.1..1.0 mm only. No bandwidth extrapolation is performed. The existing focal "
+                "path also computes diffraction; use incoherent/path for out-of-table reference runs.");
+    }
     if (!c.cdf_theta) c.cdf_theta = c.theta;
     if (!c.cdf_phi) c.cdf_phi = c.phi;
     if (c.cdf_theta > c.theta || c.cdf_phi > c.phi || c.theta % c.cdf_theta || c.phi % c.cdf_phi)
@@ -213,13 +264,6 @@ bool parse(int argc, char** argv, Command& c)
     if (fs::exists(c.out)) throw std::runtime_error("Output already exists.");
     if (fs::exists(c.out + ".part"))
         throw std::runtime_error("Previous .part directory exists; inspect it before recomputing.");
-    // Every other argument is checked first, as the reference does before it
-    // runs the stage; the coherent stages (its default is diffraction) are then
-    // refused by name rather than replaced silently by another estimand.
-    if (c.stage != "incoherent")
-        throw std::invalid_argument("Stage '" + c.stage +
-                                    "' (coherent path/focal phases, Table II diffraction) is not built in this tool; "
-                                    "use --stage incoherent.");
     return true;
 }
 
@@ -244,40 +288,71 @@ int run(int argc, char** argv)
     std::cout << std::setprecision(17) << "Material=" << (c.water ? "water_iapws_r9_97" : "constant")
               << " wavelength_vacuum_nm=" << c.wavelength_nm << " interior_index_solver=" << c.interior_index
               << " exterior_index=1; additional_storage_gaussian=none\n";
+    const bool coherent = c.stage != "incoherent";
+    if (coherent && c.paper_sigma > 0)
+        std::cout << "Local diffraction sigma_primary_deg=" << c.paper_sigma
+                  << " sigma_secondary_deg=" << 2 * c.paper_sigma << " query_step_theta_deg=" << 180.0 / c.theta
+                  << " query_step_phi_deg=" << 360.0 / c.phi << " (phi arc step additionally varies with theta)\n";
     const auto start = std::chrono::steady_clock::now();
 
-    iph_record r;
-    std::memset(&r, 0, sizeof r);
-    r.shape = sphere ? IPH_SHAPE_SPHERE : IPH_SHAPE_DROP;
-    r.max_internal_hits = c.max_hits;
-    r.theta_count = c.theta;
-    r.phi_count = c.phi;
-    r.grid_w = r.grid_h = c.grid;
-    r.radius_mm = (double)c.radius_mm;
-    for (int k = 0; k < 8; ++k) r.shape_coefficients[k] = (double)c.shape[k];
-    for (int k = 0; k < 3; ++k) r.incident[k] = (double)kf[k];
-    r.interior_index = (double)c.interior_index;
-    r.exterior_index = 1.0;
-    r.wavelength_nm = (double)c.wavelength_nm;
-    r.tail_tolerance = c.tail;
     iph_cdf_policy pol{c.cdf_theta, c.cdf_phi, c.max_l1, c.max_lost, c.max_tv};
-
     iph_ctx* ctx = nullptr;
     cli::check(iph_ctx_create(c.device, &ctx));
     struct Guard {
         iph_ctx* c;
         ~Guard() { iph_ctx_destroy(c); }
     } guard{ctx};
-    iph_batch* b = nullptr;
-    cli::check(iph_batch_create(ctx, &r, 1, &pol, &b), ctx);
-    const int rc = iph_batch_run(b, IPH_RUN_TRACE | IPH_RUN_CDF);
-    iph_hist_stats st;
-    cli::check(iph_batch_stats(b, 0, &st), ctx);
     std::vector<double> phi(c.cdf_phi + 1), theta((size_t)c.cdf_phi * (c.cdf_theta + 1));
-    iph_cdf_report rep;
-    cli::check(iph_batch_cdf(b, 0, phi.data(), theta.data(), &rep), ctx);
-    cli::check(rc, ctx);
-    if (st.invalid) throw std::runtime_error("Histogram has invalid outgoing directions.");
+    iph_cdf_report rep{};
+    iph_hist_stats st{};
+    iph_coherent_stats cst{};
+    iph_coherent_gate gate{};
+    iph_record r;
+    std::memset(&r, 0, sizeof r);
+    r.radius_mm = (double)c.radius_mm;
+    if (coherent) {
+        iph_coherent_config cc{};
+        cc.radius_mm = c.radius_mm;
+        cc.wavelength_nm = c.wavelength_nm;
+        cc.interior_index = c.interior_index;
+        cc.inclination_deg = c.inclination;
+        cc.grid = c.grid;
+        cc.sphere = c.sphere ? 1u : 0u;
+        cc.theta_count = c.theta;
+        cc.phi_count = c.phi;
+        for (int k = 0; k < 4; ++k) cc.focal_offsets[k] = c.focal[k];
+        cc.primary_sigma_deg = 0.0;  // Table II
+        cc.transition_contrast = c.transition_contrast;
+        iph_coherent* h = nullptr;
+        cli::check(iph_coherent_create(ctx, &cc, &h), ctx);
+        cli::check(iph_coherent_trace(h), ctx);
+        cli::check(iph_coherent_evaluate(h), ctx);
+        const uint32_t code = c.stage == "path" ? IPH_STAGE_PATH : (c.stage == "focal" ? IPH_STAGE_FOCAL
+                                                                                        : IPH_STAGE_DIFFRACTION);
+        const int rc = iph_coherent_cdf(h, code, c.allow_underresolved ? 1u : 0u, &pol, phi.data(), theta.data(),
+                                        &rep, &gate);
+        cli::check(iph_coherent_stats_get(h, &cst), ctx);
+        cli::check(rc, ctx);
+    } else {
+        r.shape = sphere ? IPH_SHAPE_SPHERE : IPH_SHAPE_DROP;
+        r.max_internal_hits = c.max_hits;
+        r.theta_count = c.theta;
+        r.phi_count = c.phi;
+        r.grid_w = r.grid_h = c.grid;
+        for (int k = 0; k < 8; ++k) r.shape_coefficients[k] = (double)c.shape[k];
+        for (int k = 0; k < 3; ++k) r.incident[k] = (double)kf[k];
+        r.interior_index = (double)c.interior_index;
+        r.exterior_index = 1.0;
+        r.wavelength_nm = (double)c.wavelength_nm;
+        r.tail_tolerance = c.tail;
+        iph_batch* b = nullptr;
+        cli::check(iph_batch_create(ctx, &r, 1, &pol, &b), ctx);
+        const int rc = iph_batch_run(b, IPH_RUN_TRACE | IPH_RUN_CDF);
+        cli::check(iph_batch_stats(b, 0, &st), ctx);
+        cli::check(iph_batch_cdf(b, 0, phi.data(), theta.data(), &rep), ctx);
+        cli::check(rc, ctx);
+        if (st.invalid) throw std::runtime_error("Histogram has invalid outgoing directions.");
+    }
 
     // frame columns of the binning frame: e0 = normalize(+z - k (k.z)), e1 = k x e0
     double k[3] = {kd[0], kd[1], kd[2]};
@@ -296,14 +371,13 @@ int run(int argc, char** argv)
     }
     const double f[3] = {k[1] * e[2] - k[2] * e[1], k[2] * e[0] - k[0] * e[2], k[0] * e[1] - k[1] * e[0]};
     const double q = std::ldexp(1.0, -(int)st.qbits) * st.ray_weight_mm2;  // mm^2 per quantum
-    // incident grid half extent (mm): 1.01 x the conservative outer radius, as the tracer sets it
-    double half_extent = 1.01 * (double)r.radius_mm;
-    if (!sphere) {
-        double sum = 0;
-        for (int n = 1; n < 8; ++n) sum += std::fabs((double)c.shape[n]);
-        const float outer = std::nextafter((float)(1.0 + (double)c.shape[0] + sum + 0x1p-20), INFINITY);
-        half_extent = (double)(float)((double)outer * 1.01) * (double)r.radius_mm;
-    }
+    // incident grid half extent and reference distances in drop units, as the solver sets them
+    float solver_shape[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!c.sphere)
+        for (int k = 0; k < 8; ++k) solver_shape[k] = c.shape[k];
+    const float outer = outer_radius(solver_shape);
+    const double half_extent = (double)(outer * 1.01f);
+    const double reference_distance = (double)(2.0f * outer);
 
     cli::Json j;
     j.kv("coordinate_contract_id", "rainbow.phase_cdf.coordinates.v1");
@@ -366,10 +440,15 @@ int run(int argc, char** argv)
     j.close_object();
     j.key("diffraction_policy").open_object();
     j.kv("name", "Sadeghi2012_TableII_local_v1");
-    j.kv("computed", false);
-    j.kv("selected_output_includes_diffraction", false);
-    j.key("primary_sigma_degrees").null();
-    j.key("secondary_sigma_degrees").null();
+    j.kv("computed", c.paper_sigma > 0);
+    j.kv("selected_output_includes_diffraction", c.stage == "diffraction");
+    if (c.paper_sigma > 0) {
+        j.kv("primary_sigma_degrees", c.paper_sigma);
+        j.kv("secondary_sigma_degrees", 2 * c.paper_sigma);
+    } else {
+        j.key("primary_sigma_degrees").null();
+        j.key("secondary_sigma_degrees").null();
+    }
     j.kv("bandwidth_source", "paper Table II; secondary sigma x2");
     j.kv("between_table_radii", "linear interpolation (project convention)");
     j.kv("outside_table_radii", "reject");
@@ -407,8 +486,9 @@ int run(int argc, char** argv)
     j.kv("force_sphere", c.sphere);
     j.key("incident_grid").open_array().value(c.grid).value(c.grid).close_array();
     j.kv("incident_grid_half_extent", half_extent);
-    j.kv("incident_grid_placement", "cell_centres_over_square_1.01_outer_radius_misses_discarded");
-    j.key("reference_distances").open_array().value(2.0 * (double)c.radius_mm).value(2.0 * (double)c.radius_mm).close_array();
+    j.kv("incident_grid_placement", coherent ? "vertices_over_square_1.01_outer_radius"
+                                             : "cell_centres_over_square_1.01_outer_radius_misses_discarded");
+    j.key("reference_distances").open_array().value(reference_distance).value(reference_distance).close_array();
     j.key("query_frame_axis").array(kd, 3);
     j.key("query_frame_e0").array(e, 3);
     j.key("sampling_frame_columns").open_array();
@@ -428,18 +508,18 @@ int run(int argc, char** argv)
     j.kv("diffraction_primary_sigma_degrees_setting", 0.0);
     j.kv("diffraction_zero_setting_selects_Table_II", true);
     j.kv("diffraction_transition_contrast", c.transition_contrast);
-    j.kv("folded_branch_model", "none_ray_histogram");
-    j.kv("folded_relative_tolerance", 0.0);
+    j.kv("folded_branch_model", coherent ? "finite_branch_area_ratio_v1" : "none_ray_histogram");
+    j.kv("folded_relative_tolerance", coherent ? 1e-8 : 0.0);
     j.kv("source_commit", "unknown");
     j.kv("source_dirty_at_configure", true);
-    j.kv("generator_version", "icephase.mi355x.drop_histogram.v1");
+    j.kv("generator_version", coherent ? "icephase.mi355x.drop_patches.v1" : "icephase.mi355x.drop_histogram.v1");
     j.key("quality").open_object();
     j.kv("source_coverage_certified", false);
     j.kv("angular_convergence_certified", false);
-    j.kv("missing_source_cells", 0);
-    j.kv("invalid_values", st.invalid);
-    j.kv("incomplete_directions", 0);
-    j.kv("underresolved_directions", 0);
+    j.kv("missing_source_cells", coherent ? cst.status_counts[4] : 0u);
+    j.kv("invalid_values", coherent ? (uint64_t)gate.invalid_values : st.invalid);
+    j.kv("incomplete_directions", coherent ? gate.incomplete : 0u);
+    j.kv("underresolved_directions", coherent ? gate.underresolved : 0u);
     j.kv("allow_underresolved", c.allow_underresolved);
     j.kv("cdf_l1_mass_error", rep.l1_error);
     j.kv("cdf_maximum_cell_mass_error", rep.max_cell_error);
@@ -450,21 +530,31 @@ int run(int argc, char** argv)
     j.close_object();
     j.kv("incident_inclination_degrees", c.inclination);
     j.key("query_grid").open_array().value(c.theta).value(c.phi).close_array();
-    j.key("histogram").open_object();
-    j.kv("method", "fixed_point_u64_ray_histogram");
-    j.kv("quantum_bits", st.qbits);
-    j.kv("ray_weight_mm2", st.ray_weight_mm2);
-    j.kv("rays", st.rays);
-    j.kv("rays_missed", st.status[8]);
-    j.kv("outgoing_samples", st.samples);
-    j.kv("max_internal_hits", c.max_hits);
-    j.kv("tail_tolerance_per_ray", c.tail);
-    j.kv("escaped_mm2", (double)st.q_escaped * q);
-    j.kv("unresolved_mm2", (double)st.q_unresolved * q);
-    j.kv("forward_axis_mass_mm2", (double)st.q_forward * q);
-    j.kv("backward_axis_mass_mm2", (double)st.q_backward * q);
-    j.kv("boundary_snapped_samples", st.snapped);
-    j.close_object();
+    if (coherent) {
+        j.key("patch_model").open_object();
+        j.kv("method", "bilinear_chord_patches_device_query");
+        j.kv("patches", cst.patches);
+        j.kv("folded_patches", cst.folded);
+        j.kv("candidate_pairs", cst.candidates);
+        j.kv("hits", cst.hits);
+        j.close_object();
+    } else {
+        j.key("histogram").open_object();
+        j.kv("method", "fixed_point_u64_ray_histogram");
+        j.kv("quantum_bits", st.qbits);
+        j.kv("ray_weight_mm2", st.ray_weight_mm2);
+        j.kv("rays", st.rays);
+        j.kv("rays_missed", st.status[8]);
+        j.kv("outgoing_samples", st.samples);
+        j.kv("max_internal_hits", c.max_hits);
+        j.kv("tail_tolerance_per_ray", c.tail);
+        j.kv("escaped_mm2", (double)st.q_escaped * q);
+        j.kv("unresolved_mm2", (double)st.q_unresolved * q);
+        j.kv("forward_axis_mass_mm2", (double)st.q_forward * q);
+        j.kv("backward_axis_mass_mm2", (double)st.q_backward * q);
+        j.kv("boundary_snapped_samples", st.snapped);
+        j.close_object();
+    }
     j.key("storage_processing").open_object();
     j.kv("additional_filter", "none");
     j.kv("additional_filter_is_paper_diffraction", false);

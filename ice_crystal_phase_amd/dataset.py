@@ -20,9 +20,10 @@ process per condition; here the conditions are traced in batches of many
 records by one persistent launch of libicephase.so (`iph_batch_run`: every
 record's rays in one tile queue, then every record's CDF), in this process,
 and each record is written with exactly the metadata and arrays
-`bin/rainbow_generate` writes for the same condition. The stage is the
-incoherent ray histogram; the coherent stages are rejected by name, as the
-command-line tool does.
+`bin/rainbow_generate` writes for the same condition. The incoherent stage is
+the ray histogram, traced in batches; the coherent stages (path, focal,
+diffraction) run the patch-model pipeline (iph_coherent_*) one record at a
+time on the same context.
 
 Run as `python -m ice_crystal_phase_amd.dataset --config CONFIG [--execute]`.
 """
@@ -41,6 +42,7 @@ import tempfile
 import time
 from concurrent.futures import ThreadPoolExecutor
 from contextlib import contextmanager
+from fractions import Fraction
 from dataclasses import dataclass
 from decimal import Decimal, InvalidOperation
 from pathlib import Path
@@ -53,6 +55,31 @@ from . import icephase as ip
 PACKAGE = Path(__file__).resolve().parent
 VERSION = "rainbow.condition_batch.v1"
 GENERATOR_VERSION = "icephase.mi355x.drop_histogram.v1"
+COHERENT_GENERATOR_VERSION = "icephase.mi355x.drop_patches.v1"
+TABLE_II_MESSAGE = ("Table II covers radius 0.1..1.0 mm only. No bandwidth extrapolation is performed. The existing "
+                    "focal path also computes diffraction; use incoherent/path for out-of-table reference runs.")
+
+
+def generator_version(stage: str) -> str:
+    return GENERATOR_VERSION if stage == "incoherent" else COHERENT_GENERATOR_VERSION
+
+
+def table_sigma_degrees(radius: float) -> float | None:
+    """Table II primary bandwidth (degrees) of an FP32 radius, linearly
+    interpolated between the tabulated radii; None outside 0.1..1 mm
+    (include/rainbow/rainbow_diffraction.hpp:17-29)."""
+    r = [fp32(x) for x in (0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0)]
+    sig = [0.70, 0.45, 0.30, 0.25, 0.22, 0.20, 0.18, 0.17, 0.16, 0.15]
+    if not (r[0] <= radius <= r[9]):
+        return None
+    for j in range(10):
+        if radius == r[j]:
+            return sig[j]
+        if j and radius < r[j]:
+            t = (radius - r[j - 1]) / (r[j] - r[j - 1])
+            # one rounding of t * (s1 - s0) + s0, as the tool's fma
+            return float(Fraction(t) * Fraction(sig[j] - sig[j - 1]) + Fraction(sig[j - 1]))
+    return None
 ARRAYS = ("phi_cdf.npy", "theta_given_phi_cdf.npy", "u_edges.npy")
 CONFIG_KEYS = {"output_root", "generator", "radius_mm", "force_sphere", "temperature_c", "pressure_pa",
                "wavelengths", "inclinations", "solver", "reserve_gib"}
@@ -183,9 +210,6 @@ def _solver(s: Any) -> dict[str, Any]:
         raise ValueError("CDF grids must divide query grids")
     if s["stage"] not in STAGES:
         raise ValueError("Unknown stage")
-    if s["stage"] != "incoherent":
-        raise ValueError(f"Stage '{s['stage']}' (coherent path/focal phases, Table II diffraction) is not built; "
-                         "set solver.stage to incoherent.")
     out["stage"] = s["stage"]
     out["device"] = integer(s["device"], *_SOLVER_INTEGERS["device"])
     out["allow_underresolved"] = flag(s["allow_underresolved"])
@@ -204,6 +228,8 @@ def normalized_config(raw: dict[str, Any]) -> dict[str, Any]:
         raise ValueError(f"Unexpected/missing config keys: {set(raw) ^ CONFIG_KEYS}")
     solver = _solver(raw["solver"])
     values = {k: scalar(raw[k], lo, hi) for k, (lo, hi) in _CONDITION_SCALARS.items()}
+    if solver["stage"] in ("focal", "diffraction") and table_sigma_degrees(fp32(values["radius_mm"])) is None:
+        raise ValueError(TABLE_II_MESSAGE)
     return {"radius_mm": values["radius_mm"], "force_sphere": flag(raw["force_sphere"]),
             "temperature_c": values["temperature_c"], "pressure_pa": values["pressure_pa"],
             "wavelength_nm": wavelengths(raw["wavelengths"]), "inclination_degrees": inclinations(raw["inclinations"]),
@@ -304,7 +330,8 @@ def _record_checks(m: dict[str, Any], c: dict[str, Any], ii: int, ll: int) -> li
     """(label, passed) for every rule a record of condition (ii, ll) obeys."""
     s = c["solver"]
     get = lambda dotted: _dig(m, dotted)   # noqa: E731
-    expected = {"complete": True, "schema": "rainbow.phase_cdf.numpy.v2", "generator_version": GENERATOR_VERSION,
+    expected = {"complete": True, "schema": "rainbow.phase_cdf.numpy.v2",
+                "generator_version": generator_version(s["stage"]),
                 "coordinate_contract_id": "rainbow.phase_cdf.coordinates.v1",
                 "direction_convention": "physical_propagation", "dtype": "<f8", "order": "C",
                 "density_measure": "solid_angle_sr", "cdf_axis_order": ["phi_cell", "theta_edge"],
@@ -451,12 +478,15 @@ def policy(s: dict[str, Any]) -> ip.CdfPolicy:
 
 @dataclass
 class Traced:
-    """Device results of one condition: histogram statistics, the CDF arrays
-    and the CDF report (status 0 = every audit passed)."""
+    """Device results of one condition: histogram (or patch-model) statistics,
+    the CDF arrays, the CDF report (status 0 = every audit passed) and, for a
+    coherent stage, the density gate and the refusal message if any."""
     stats: Any
     phi: np.ndarray
     theta: np.ndarray
     report: Any
+    gate: Any = None
+    failure: str | None = None
 
 
 def _frame(kf: list[float]) -> tuple[list[float], list[float], list[float]]:
@@ -473,16 +503,24 @@ def _frame(kf: list[float]) -> tuple[list[float], list[float], list[float]]:
     return e, f, k
 
 
-def half_extent(cond: Condition) -> float:
-    """Half side (mm) of the incident ray grid: 1.01 x the conservative outer
-    radius of the shape, rounded as the tracer rounds it."""
-    if cond.sphere:
-        return 1.01 * cond.radius
-    ripple = 0.0
+def outer_radius(cond: Condition) -> np.float32:
+    """The solver's conservative outer radius in drop units, FP32 rounded
+    outward step by step (include/rainbow/raindrop_shape.hpp:45-64)."""
+    f32 = np.float32
+    ripple = f32(0.0)
     for v in cond.shape[1:]:
-        ripple += abs(v)
-    outer = np.nextafter(np.float32(1.0 + cond.shape[0] + ripple + 2.0 ** -20), np.float32(np.inf))
-    return float(np.float32(float(outer) * 1.01)) * cond.radius
+        ripple = np.nextafter(f32(ripple + f32(abs(v))), f32(np.inf))
+    return np.nextafter(f32(f32(f32(1.0) + f32(cond.shape[0])) + ripple) + f32(2.0 ** -20), f32(np.inf))
+
+
+def half_extent(cond: Condition) -> float:
+    """Half side of the incident grid in drop units: 1.01 x the outer radius (FP32)."""
+    return float(np.float32(outer_radius(cond) * np.float32(1.01)))
+
+
+def reference_distance(cond: Condition) -> float:
+    """Distance of both reference planes in drop units: twice the outer radius (FP32)."""
+    return float(np.float32(np.float32(2.0) * outer_radius(cond)))
 
 
 def metadata(cond: Condition, c: dict[str, Any], t: Traced) -> dict[str, Any]:
@@ -492,7 +530,34 @@ def metadata(cond: Condition, c: dict[str, Any], t: Traced) -> dict[str, Any]:
     st, rep, area, w = t.stats, t.report, cond.area, cond.water
     e, f, k = _frame(cond.kf)
     kd = list(cond.kf)
-    q = math.ldexp(1.0, -int(st.qbits)) * st.ray_weight_mm2  # mm^2 per quantum
+    coherent = s["stage"] != "incoherent"
+    sigma = table_sigma_degrees(cond.radius) if s["stage"] in ("focal", "diffraction") else None
+    if coherent:
+        g = t.gate
+        model = {"patch_model": {"method": "bilinear_chord_patches_device_query", "patches": int(st.patches),
+                                 "folded_patches": int(st.folded), "candidate_pairs": int(st.candidates),
+                                 "hits": int(st.hits)}}
+        quality = {"missing_source_cells": int(st.status_counts[4]), "invalid_values": int(g.invalid_values),
+                   "incomplete_directions": int(g.incomplete), "underresolved_directions": int(g.underresolved)}
+    else:
+        q = math.ldexp(1.0, -int(st.qbits)) * st.ray_weight_mm2  # mm^2 per quantum
+        model = {"histogram": {
+            "method": "fixed_point_u64_ray_histogram",
+            "quantum_bits": int(st.qbits),
+            "ray_weight_mm2": st.ray_weight_mm2,
+            "rays": int(st.rays),
+            "rays_missed": int(st.status[8]),
+            "outgoing_samples": int(st.samples),
+            "max_internal_hits": MAX_INTERNAL_HITS,
+            "tail_tolerance_per_ray": TAIL_TOLERANCE,
+            "escaped_mm2": float(st.q_escaped) * q,
+            "unresolved_mm2": float(st.q_unresolved) * q,
+            "forward_axis_mass_mm2": float(st.q_forward) * q,
+            "backward_axis_mass_mm2": float(st.q_backward) * q,
+            "boundary_snapped_samples": int(st.snapped),
+        }}
+        quality = {"missing_source_cells": 0, "invalid_values": int(st.invalid), "incomplete_directions": 0,
+                   "underresolved_directions": 0}
     return {
         "coordinate_contract_id": "rainbow.phase_cdf.coordinates.v1",
         "direction_convention": "physical_propagation",
@@ -553,10 +618,10 @@ def metadata(cond: Condition, c: dict[str, Any], t: Traced) -> dict[str, Any]:
         },
         "diffraction_policy": {
             "name": "Sadeghi2012_TableII_local_v1",
-            "computed": False,
-            "selected_output_includes_diffraction": False,
-            "primary_sigma_degrees": None,
-            "secondary_sigma_degrees": None,
+            "computed": sigma is not None,
+            "selected_output_includes_diffraction": s["stage"] == "diffraction",
+            "primary_sigma_degrees": sigma,
+            "secondary_sigma_degrees": None if sigma is None else 2 * sigma,
             "bandwidth_source": "paper Table II; secondary sigma x2",
             "between_table_radii": "linear interpolation (project convention)",
             "outside_table_radii": "reject",
@@ -593,8 +658,9 @@ def metadata(cond: Condition, c: dict[str, Any], t: Traced) -> dict[str, Any]:
         "force_sphere": c["force_sphere"],
         "incident_grid": [s["grid"], s["grid"]],
         "incident_grid_half_extent": half_extent(cond),
-        "incident_grid_placement": "cell_centres_over_square_1.01_outer_radius_misses_discarded",
-        "reference_distances": [2.0 * cond.radius, 2.0 * cond.radius],
+        "incident_grid_placement": ("vertices_over_square_1.01_outer_radius" if coherent
+                                    else "cell_centres_over_square_1.01_outer_radius_misses_discarded"),
+        "reference_distances": [reference_distance(cond), reference_distance(cond)],
         "query_frame_axis": kd,
         "query_frame_e0": e,
         "sampling_frame_columns": [[e[r], f[r], k[r]] for r in range(3)],
@@ -602,19 +668,16 @@ def metadata(cond: Condition, c: dict[str, Any], t: Traced) -> dict[str, Any]:
         "focal_quarter_turn_offsets": list(s["focal_offsets"]),
         "diffraction_primary_sigma_degrees_setting": 0.0,
         "diffraction_zero_setting_selects_Table_II": True,
-        "diffraction_transition_contrast": 0.0,
-        "folded_branch_model": "none_ray_histogram",
-        "folded_relative_tolerance": 0.0,
+        "diffraction_transition_contrast": 0.25,
+        "folded_branch_model": "finite_branch_area_ratio_v1" if coherent else "none_ray_histogram",
+        "folded_relative_tolerance": 1e-8 if coherent else 0.0,
         "source_commit": "unknown",
         "source_dirty_at_configure": True,
-        "generator_version": GENERATOR_VERSION,
+        "generator_version": generator_version(s["stage"]),
         "quality": {
             "source_coverage_certified": False,
             "angular_convergence_certified": False,
-            "missing_source_cells": 0,
-            "invalid_values": int(st.invalid),
-            "incomplete_directions": 0,
-            "underresolved_directions": 0,
+            **quality,
             "allow_underresolved": s["allow_underresolved"],
             "cdf_l1_mass_error": rep.l1_error,
             "cdf_maximum_cell_mass_error": rep.max_cell_error,
@@ -625,21 +688,7 @@ def metadata(cond: Condition, c: dict[str, Any], t: Traced) -> dict[str, Any]:
         },
         "incident_inclination_degrees": cond.alpha,
         "query_grid": [s["query_theta"], s["query_phi"]],
-        "histogram": {
-            "method": "fixed_point_u64_ray_histogram",
-            "quantum_bits": int(st.qbits),
-            "ray_weight_mm2": st.ray_weight_mm2,
-            "rays": int(st.rays),
-            "rays_missed": int(st.status[8]),
-            "outgoing_samples": int(st.samples),
-            "max_internal_hits": MAX_INTERNAL_HITS,
-            "tail_tolerance_per_ray": TAIL_TOLERANCE,
-            "escaped_mm2": float(st.q_escaped) * q,
-            "unresolved_mm2": float(st.q_unresolved) * q,
-            "forward_axis_mass_mm2": float(st.q_forward) * q,
-            "backward_axis_mass_mm2": float(st.q_backward) * q,
-            "boundary_snapped_samples": int(st.snapped),
-        },
+        **model,
         "storage_processing": {
             "additional_filter": "none",
             "additional_filter_is_paper_diffraction": False,
@@ -675,6 +724,8 @@ def trace_conditions(ctx: ip.Context, conds: list[Condition], s: dict[str, Any])
     rays in one tile queue, then all CDFs) and read back each record's results.
     Audits are evaluated per record afterwards, so a failing record does not
     hide the ones before it."""
+    if s["stage"] != "incoherent":
+        return [trace_coherent(ctx, cond, s) for cond in conds]
     records = [device_record(cond, s) for cond in conds]
     with ip.Batch(ctx, records, policy(s)) as b:
         b.run(ip.RUN_TRACE | ip.RUN_CDF | ip.RUN_NO_AUDIT)
@@ -683,6 +734,24 @@ def trace_conditions(ctx: ip.Context, conds: list[Condition], s: dict[str, Any])
             phi, theta, rep = b.cdf(r)
             out.append(Traced(b.stats(r), phi, theta, rep))
     return out
+
+
+def trace_coherent(ctx: ip.Context, cond: Condition, s: dict[str, Any]) -> Traced:
+    """One condition through the coherent pipeline: vertex trace, patches,
+    query, optics, Table II, gate and CDF; a refused record keeps its message."""
+    pol = policy(s)
+    with ip.Coherent(ctx, cond.radius, cond.wavelength, cond.n_solver, grid=s["grid"], theta=s["query_theta"],
+                     phi=s["query_phi"], inclination_deg=cond.alpha, sphere=bool(cond.sphere and not any(cond.shape)),
+                     focal_offsets=tuple(s["focal_offsets"])) as h:
+        h.trace()
+        h.evaluate()
+        try:
+            phi, theta, rep, gate = h.cdf(s["stage"], s["allow_underresolved"], pol.out_theta, pol.out_phi,
+                                          pol.max_l1, pol.max_lost_mass, pol.max_coarsening_tv)
+            return Traced(h.stats(), phi, theta, rep, gate)
+        except ip.IcePhaseError as exc:
+            empty = np.zeros(0)
+            return Traced(h.stats(), empty, empty, ip.CdfReport(), None, str(exc))
 
 
 def batch_capacity(s: dict[str, Any]) -> int:
@@ -886,10 +955,10 @@ def _finish(cond: Condition, t: Traced, c: dict[str, Any], plan: dict[str, Any],
     n = done + 1
     print(f"[{n}/{count}] {cond.job}: alpha={c['inclination_degrees'][cond.ii]} "
           f"lambda={c['wavelength_nm'][cond.ll]}", flush=True)
-    failure = None
-    if t.report.status:
+    failure = t.failure
+    if failure is None and t.report.status:
         failure = f"CDF audit failed (status {t.report.status})"
-    elif t.stats.invalid:
+    elif failure is None and c["solver"]["stage"] == "incoherent" and t.stats.invalid:
         failure = "Histogram has invalid outgoing directions."
     if failure is not None:
         with log.open("x", encoding="utf-8") as f:

@@ -530,7 +530,6 @@ class AtlasMass:
             pass
 
 
-@dataclass(eq=False)
 class Coherent:
     """One drop record on the patch model (iph_coherent): the coherent stages
     path, focal and diffraction of rainbow_generate, and the patch-model
@@ -636,6 +635,7 @@ class Coherent:
             pass
 
 
+@dataclass(eq=False)
 class Batch:
     """Device state of several records (iph_batch)."""
 

@@ -170,3 +170,13 @@ def test_closed_handles_raise_before_the_library():
     a._p, a.ctx, a.shape = ctypes.c_void_p(), SimpleNamespace(ptr=None), (1, 1, 2, 3)
     with pytest.raises(ip.IcePhaseError, match="atlas mass is closed"):
         a.compose(np.ones((1, 1, 2, 3)))
+
+
+def test_python_handles_take_their_arguments():
+    """The handle classes keep their constructors (a dataclass decorator that
+    slips to another class leaves Batch without arguments)."""
+    import inspect
+    assert list(inspect.signature(ip.Batch).parameters)[:3] == ["ctx", "records", "policy"]
+    assert list(inspect.signature(ip.Coherent).parameters)[:4] == ["ctx", "radius_mm", "wavelength_nm",
+                                                                    "interior_index"]
+    assert list(inspect.signature(ip.AtlasMass).parameters) == ["ctx", "mass"]

@@ -33,12 +33,15 @@ def built():
     (["--out", "x", "--wavelength-nm", "550", "--material", "constant", "--ior", "1.3", "--temperature-c", "4"],
      "Temperature/pressure do not define"),
     (["--out", "x", "--wavelength-nm", "550", "--query-theta", "100", "--cdf-theta", "30"], "CDF dimensions must divide"),
-    (["--out", "x", "--wavelength-nm", "550", "--stage", "focal"], "not built in this tool"),
+    (["--out", "x", "--wavelength-nm", "550", "--stage", "focal", "--radius-mm", "2"], "Table II covers radius 0.1..1.0 mm"),
+    (["--out", "x", "--wavelength-nm", "550", "--radius-mm", "1.5"], "Table II covers radius 0.1..1.0 mm"),
+    (["--out", "x", "--wavelength-nm", "550", "--stage", "path", "--max-internal-hits", "3"],
+     "apply to --stage incoherent only"),
     (["--out", "x", "--wavelength-nm", "550", "--stage", "bogus"], "Unknown stage."),
     (["--out", "x", "--wavelength-nm", "550", "--focal-offsets", "0,1,2"], "Expected four focal offsets."),
     (["--out", "x", "--wavelength-nm", "550", "--grid", "0"], "Invalid positive integer option."),
     (["--out", "x", "--wavelength-nm", "abc"], "Invalid numeric option."),
-    (["--out", "x", "--wavelength-nm", "550", "--radius-mm", "3.5"], "Table I range"),
+    (["--out", "x", "--wavelength-nm", "550", "--radius-mm", "3.5", "--stage", "path"], "Table I range"),
     (["--out", "x", "--wavelength-nm", "550", "--temperature-c", "80"], "liquid branch at 0..60 C"),
     (["--out", "x", "--wavelength-nm", "550", "--bogus", "1"], "Unknown option: --bogus"),
 ])
@@ -56,13 +59,12 @@ def test_rainbow_generate_refuses_existing_part(tmp_path):
 
 def test_rainbow_generate_default_stage_is_the_references(tmp_path):
     """Without --stage the reference runs its diffraction stage
-    (apps/generate_phase.cpp:54); the tool takes the same default and refuses it
-    by name after every other argument has been checked, instead of silently
-    producing the incoherent estimand."""
-    r = _run(BIN / "rainbow_generate", "--out", "x", "--wavelength-nm", "550", cwd=tmp_path)
-    assert r.returncode == 1 and "Stage 'diffraction'" in r.stderr, r.stderr
+    (apps/generate_phase.cpp:54), which needs a Table II radius; the tool takes
+    the same default (the run itself is a GPU test), and argument errors are
+    still reported first, in the reference's order."""
+    r = _run(BIN / "rainbow_generate", "--out", "x", "--wavelength-nm", "550", "--radius-mm", "2", cwd=tmp_path)
+    assert r.returncode == 1 and "Table II covers radius 0.1..1.0 mm only" in r.stderr, r.stderr
     assert not (tmp_path / "x").exists() and not (tmp_path / "x.part").exists()
-    # an argument error is still reported first, as in the reference's order
     r = _run(BIN / "rainbow_generate", "--out", "x", "--wavelength-nm", "550", "--ior", "1.3", cwd=tmp_path)
     assert r.returncode == 1 and "Do not specify --ior for water" in r.stderr
 

@@ -154,10 +154,27 @@ def test_unknown_config(h):
 
 
 @pytest.mark.parametrize("stage", ["diffraction", "focal", "path"])
-def test_coherent_stage_rejected_by_name(h, stage):
+def test_coherent_stages_and_table_ii_range(h, stage):
+    """The coherent stages are accepted; focal and diffraction need a Table II
+    radius (0.1..1 mm), as rainbow_generate requires."""
     h.raw["solver"]["stage"] = stage
-    with pytest.raises(ValueError, match="not built"):
-        ds.normalized_config(h.raw)
+    h.raw["radius_mm"] = 0.5
+    assert ds.normalized_config(h.raw)["solver"]["stage"] == stage
+    h.raw["radius_mm"] = 1.5
+    if stage == "path":
+        assert ds.normalized_config(h.raw)["solver"]["stage"] == "path"
+    else:
+        with pytest.raises(ValueError, match="Table II covers radius 0.1..1.0 mm only"):
+            ds.normalized_config(h.raw)
+
+
+def test_table_ii_sigma_is_the_tools():
+    """Table II values at and between the tabulated radii (one rounding, as an fma)."""
+    assert ds.table_sigma_degrees(ds.fp32(0.5)) == 0.22
+    assert ds.table_sigma_degrees(ds.fp32(1.0)) == 0.15
+    assert ds.table_sigma_degrees(ds.fp32(0.05)) is None
+    x = ds.table_sigma_degrees(ds.fp32(0.45))
+    assert 0.22 < x < 0.25 and abs(x - 0.235) < 1e-6
 
 
 def test_source_index_range(h):
@@ -382,13 +399,19 @@ def test_condition_setup_matches_the_command_line_tool(h, tmp_path):
     r = ds.device_record(cond, c["solver"])
     assert (r.shape, r.grid_w, r.grid_h, r.theta_count, r.phi_count) == (ip.SHAPE_DROP, 9, 9, 4, 8)
     assert r.max_internal_hits == 8 and r.tail_tolerance == 0.0 and r.rays == 0
-    # outer radius bound of the Table I shape, 1 % margin, FP32 as the tracer rounds it
-    ripple = sum(abs(v) for v in cond.shape[1:])
-    outer = np.nextafter(np.float32(1 + cond.shape[0] + ripple + 2.0 ** -20), np.float32(2))
-    assert ds.half_extent(cond) == float(np.float32(float(outer) * 1.01))
+    # grid half extent and reference distances (drop units): the values the
+    # reference's RaindropSettings::make_config gives (compiled into oracle/_ref)
+    import coherent_ref as cr
     h.raw["force_sphere"] = True
     sph = ds.resolve(ds.normalized_config(h.raw), 0, 0, tmp_path)
-    assert sph.sphere and sph.shape == [0.0] * 8 and ds.half_extent(sph) == 1.01
+    assert sph.sphere and sph.shape == [0.0] * 8
+    if cr.available():
+        for cnd, sphere in ((cond, False), (sph, True)):
+            tc = np.zeros(16, np.float32)
+            assert cr.lib().refc_trace_config(cr.config(radius_mm=1.0, wavelength_nm=705.0, sphere=sphere),
+                                              tc.ctypes.data) == 0
+            assert ds.half_extent(cnd) == float(tc[6])
+            assert ds.reference_distance(cnd) == float(tc[7]) == float(tc[8])
 
 
 def test_frame_is_right_handed_and_follows_inclination():

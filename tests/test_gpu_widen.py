@@ -212,3 +212,44 @@ def test_rainbow_generate_record(tmp_path, extra):
     # an existing destination is refused
     again = _run(args, tmp_path)
     assert again.returncode == 1 and "already exists" in again.stderr
+
+
+@pytest.mark.parametrize("stage", ["path", "focal", None])
+def test_rainbow_generate_coherent_record(tmp_path, stage):
+    """The coherent stages through the tool: path, focal and the default
+    (diffraction), a 0.5 mm drop, 65 x 65 vertices, 90 x 180 cells; the record
+    holds the reference's key set and the CDF of the library's stage."""
+    print(f"case: rainbow_generate coherent stage {stage or 'default'}", flush=True)
+    args = [BIN / "rainbow_generate", "--out", "rec", "--radius-mm", "0.5", "--wavelength-nm", "550",
+            "--inclination-deg", "20", "--grid", "65", "--query-theta", "90", "--query-phi", "180"]
+    if stage:
+        args += ["--stage", stage]
+    if stage is None:
+        # Table II at 0.22 deg is underresolved on a 2-degree grid: refused, as the reference does, unless allowed
+        res = _run(args, tmp_path)
+        assert res.returncode == 1 and "CDF input rejected" in res.stderr, res.stderr
+        assert not (tmp_path / "rec").exists() and not (tmp_path / "rec.part").exists()
+        args += ["--allow-underresolved"]
+    res = _run(args, tmp_path)
+    assert res.returncode == 0, res.stderr
+    m = record_check.check_record(tmp_path / "rec")
+    assert DROP_KEYS <= set(m), DROP_KEYS - set(m)
+    want = stage or "diffraction"
+    assert m["stage"] == want
+    assert m["incident_grid"] == [65, 65]
+    assert m["diffraction_policy"]["computed"] == (want != "path")
+    assert m["diffraction_policy"]["selected_output_includes_diffraction"] == (want == "diffraction")
+    if want != "path":
+        assert m["diffraction_policy"]["primary_sigma_degrees"] == 0.22
+        assert "Local diffraction sigma_primary_deg=0.22" in res.stdout
+    assert m["folded_branch_model"] == "finite_branch_area_ratio_v1"
+    assert m["quality"]["incomplete_directions"] == 0
+    n = float(np.float32(ip.water_optics(550.0).refractive_index))
+    with ip.Context(0) as ctx, ip.Coherent(ctx, 0.5, 550.0, n, grid=65, theta=90, phi=180,
+                                           inclination_deg=20.0) as c:
+        c.trace()
+        c.evaluate()
+        phi, theta, rep, _ = c.cdf(want, allow_underresolved=True)
+    assert np.array_equal(np.load(tmp_path / "rec" / "phi_cdf.npy"), phi)
+    assert np.array_equal(np.load(tmp_path / "rec" / "theta_given_phi_cdf.npy"), theta)
+    assert m["hg"]["g"] == rep.g_stored
