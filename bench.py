@@ -339,10 +339,27 @@ def run_surrogate(ctx, n_eval: int = 64, eval_grid: int = 16384, tracer_grid: in
             out["tv_inside_mean"] = float(tv[inside].mean())
             out["tv_inside_max"] = float(tv[inside].max())
             out["dg_inside_mean"] = float(dg[inside].mean())
+            out["tv_inside_p90"] = float(np.quantile(tv[inside], 0.9))
         n_in = int(mixed.by_surrogate.sum())
         out["mixed_service"] = {"records": 64, "by_surrogate": n_in,
                                 "records_per_s": 64 / ((mixed.surrogate_ms + mixed.tracer_ms) / 1e3),
                                 "surrogate_ms": mixed.surrogate_ms, "tracer_ms": mixed.tracer_ms}
+    # What the surrogate is worth, stated at equal accuracy: over the whole family it misses the
+    # TV <= 1e-2 target (tv_mean); inside its validated domain it is about as accurate as the
+    # tracer at 2^24 rays, so the honest speedup is against that tracer, and the honest rate is the
+    # gated service (network inside the domain, tracer at the workload's rays outside it).
+    quarter = out.get("tracer_quarter_rays", {})
+    out["headline"] = {
+        "gated_service_records_per_s": out.get("mixed_service", {}).get("records_per_s"),
+        "equal_accuracy_speedup": quarter.get("speedup_vs_it"),
+        "equal_accuracy_tracer": "2^24 rays per record (TV %.4f against 2^28-ray records)" % quarter.get(
+            "tv_mean_vs_eval", float("nan")),
+        "tv_target": 1e-2,
+        "tv_target_met_over_family": bool(out["tv_mean"] <= 1e-2),
+        "tv_inside_validated_domain_mean": out.get("tv_inside_mean"),
+        "tv_inside_validated_domain_p90": out.get("tv_inside_p90"),
+        "network_only_records_per_s": out["records_per_s"],
+    }
     return out
 
 
@@ -845,9 +862,11 @@ def main() -> int:
         if sur is None:
             print("bench.py: no trained surrogate weights at " + str(SURROGATE_WEIGHTS), file=sys.stderr)
             return 1
-        out = {"metric": "surrogate records/s at TV<=1e-2 (450x900 water-drop records, 1 MI355X)",
-               "value": sur["records_per_s"], "unit": "records/s", "n_gpus": 1, "steps": 3, "warmup": 1,
-               "ms_per_step": sur["ms_per_64_records"], "higher_is_better": True, "scaling": "weak",
+        gated = sur["headline"]["gated_service_records_per_s"]
+        out = {"metric": "records/s of the gated surrogate service (network inside its validated domain, tracer "
+                         "outside; 450x900 water-drop records, 1 MI355X)",
+               "value": gated, "unit": "records/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+               "ms_per_step": 64e3 / gated if gated else None, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "bf16 MFMA (fp32 accumulate), f64 CDF", "data": "synthetic",
                "config": {"workload": sur.pop("config")}}
         out.update(sur)
