@@ -208,10 +208,20 @@ struct iph_batch {
     uint32_t blocks[2] = {0, 0};
     uint64_t tile_rays = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // second stream (default priority) for the hex kernel of a mixed batch, with
+    // the fork / join events that order it after the zero kernel and before the CDF
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};  // [0] zero + trace, [1] CDF
     double ms[3] = {0, 0, 0};
     ~iph_batch()
     {
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
         for (auto& g : graphs)
             if (g) (void)hipGraphExecDestroy(g);
         for (auto& e : ev)
@@ -491,6 +501,9 @@ int iph_batch_create(iph_ctx* ctx, const iph_record* recs, uint32_t nrec, const 
     IPH_HIP(ctx, b->d_jobs.alloc(b->jobs.size()));
     IPH_HIP(ctx, hipMemcpy(b->d_jobs.p, b->jobs.data(), b->jobs.size() * sizeof(CdfJob), hipMemcpyHostToDevice));
     for (auto& e : b->ev) IPH_HIP(ctx, hipEventCreate(&e));
+    IPH_HIP(ctx, hipStreamCreate(&b->side));
+    IPH_HIP(ctx, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    IPH_HIP(ctx, hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
     iph_batch* raw = b.release();
     const int rc = iph_batch_set_schedule(raw, 0, 0);
     if (rc != IPH_OK) {
@@ -583,9 +596,24 @@ static int enqueue(iph_batch* b, uint32_t flags, bool timed)
     if (timed) IPH_HIP(ctx, hipEventRecord(b->ev[0], s));
     if (flags & IPH_RUN_TRACE) {
         IPH_HIP(ctx, launch_zero(b->d_recs.p, b->nrec, b->d_queue.p, s));
-        for (int k = 0; k < 2; ++k)
-            if (b->map[k].total)
-                IPH_HIP(ctx, launch_trace(k, b->wide[k], b->d_recs.p, b->map[k], b->d_queue.p + k, b->blocks[k], 256, s));
+        // A mixed batch runs its two kinds at once: the FP64 hex kernel holds one
+        // wave per SIMD, the drop waves fill the rest (both add order-free into
+        // their own records' slabs, so the histograms are the same bits).
+        const bool both = b->map[0].total && b->map[1].total && b->side;
+        if (both) {
+            IPH_HIP(ctx, hipEventRecord(b->fork, s));
+            IPH_HIP(ctx, hipStreamWaitEvent(b->side, b->fork, 0));
+            IPH_HIP(ctx, launch_trace(1, b->wide[1], b->d_recs.p, b->map[1], b->d_queue.p + 1, b->blocks[1], 256,
+                                      b->side));
+            IPH_HIP(ctx, launch_trace(0, b->wide[0], b->d_recs.p, b->map[0], b->d_queue.p, b->blocks[0], 256, s));
+            IPH_HIP(ctx, hipEventRecord(b->join, b->side));
+            IPH_HIP(ctx, hipStreamWaitEvent(s, b->join, 0));
+        } else {
+            for (int k = 0; k < 2; ++k)
+                if (b->map[k].total)
+                    IPH_HIP(ctx, launch_trace(k, b->wide[k], b->d_recs.p, b->map[k], b->d_queue.p + k, b->blocks[k],
+                                              256, s));
+        }
         for (uint32_t r : b->lobe_recs)
             IPH_HIP(ctx, launch_lobe(b->d_recs.p, r, b->host[r].lobe.rows, b->host[r].np, s));
     }
