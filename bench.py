@@ -39,7 +39,8 @@ rainbow_generate process per condition (records written to disk); --config
 atlas times the deterministic hex atlas (nodes/s) beside the reference's CPU
 atlas tool; --config compose times the orientation composition of that atlas
 (records/s) beside the oracle's NumPy sum; --config coherent times the coherent
-drop stages (rainbow_generate's default diffraction stage) alone; --config atomics runs the u64
+drop stages (rainbow_generate's default diffraction stage) alone; --config hex
+runs the hexagonal column record alone; --config atomics runs the u64
 atomicAdd calibration alone (the histogram footprint of --records records).
 
 Prints one JSON line (rank 0). --gpus above 1 exits: the benchmark runs on one GPU.
@@ -783,7 +784,7 @@ def main() -> int:
     ap.add_argument("--tile-rays", type=int, default=0)
     ap.add_argument("--blocks", type=int, default=0)
     ap.add_argument("--config", choices=("drops", "uneven", "surrogate", "dataset", "atlas", "compose", "atomics",
-                                         "coherent"),
+                                         "coherent", "hex"),
                     default="drops")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only (no hex/sphere/surrogate)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -822,6 +823,13 @@ def main() -> int:
                "data": "synthetic", "config": {"workload": u.pop("config")}}
         out.update(u)
         print(json.dumps(out))
+        return 0
+    if args.config == "hex":
+        # the hexagonal column record (C3) alone, for per-kernel counter profiles
+        with ip.Context(0) as ctx:
+            hx = run_hex(ctx)
+        print(json.dumps({"metric": "rays/s of the hex column record on 1 MI355X", "value": hx["rays_per_s"],
+                          "unit": "rays/s", "hex": hx}))
         return 0
     if args.config == "coherent":
         with ip.Context(0) as ctx:
