@@ -72,10 +72,11 @@ BASE_COEFFS = (-0.0458, -0.0335, -0.1211, -0.0227)
 # by `python oracle/census.py` and checked by tests/test_census_cpu.py. miss: a
 # ray that misses; entry: a hitting ray's set-up, entry solve, first split and
 # its reflection's power; hit: one internal encounter with its transmitted
-# branch's power; emit: the binning of one emitted direction. With FMA
-# contraction off each operation is one instruction.
+# branch's power; emit: the binning of one emitted direction. An explicit
+# fused multiply-add counts as two operations (as the FP32 peak counts it); the
+# compiler fuses nothing else (FMA contraction off).
 CENSUS = {
-    "drop": {"miss": 79.7, "entry": 646.9, "hit": 607.8, "emit": 46.5},
+    "drop": {"miss": 74.8, "entry": 616.7, "hit": 582.4, "emit": 46.5},
     "sphere": {"miss": 12.0, "entry": 269.0, "hit": 273.0, "emit": 46.5},
 }
 PEAK_TFLOPS_FP32 = 157.3   # MI355X published FP32 vector peak

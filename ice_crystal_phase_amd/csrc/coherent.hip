@@ -73,7 +73,7 @@ __device__ inline uint32_t entry_hit(const drop::Shape& S, v3 o, v3 w, float rho
                 t1 = t1 < ta ? ta : t1;
                 t1 = t1 > -ta ? -ta : t1;
                 float g1;
-                const float f1 = surface_slope(S, of + wf * t1, wf, g1);
+                const float f1 = surface_slope(S, axpy(wf, t1, of), wf, g1);
                 if (f1 < 0.0f) {
                     tm = t1;
                     inside = true;
@@ -89,11 +89,11 @@ __device__ inline uint32_t entry_hit(const drop::Shape& S, v3 o, v3 w, float rho
         }
         if (!inside) return kMissed;
         const float rm = 1.0f + S.c[0];
-        const float g2 = rm * rm - rho2;
-        const float x0 = g2 > 0.0f ? -sqrtf(g2) : ta + (tm - ta) * 0.5f;
+        const float g2 = fmaf(rm, rm, -rho2);
+        const float x0 = g2 > 0.0f ? -sqrtf(g2) : fmaf(tm - ta, 0.5f, ta);
         t = newton(S, of, wf, ta, tm, true, x0);
     }
-    const f3 pf = of + wf * t;
+    const f3 pf = axpy(wf, t, of);
     p = Vd(pf);
     n = Vd(outward_normal(S, pf));
     return kHit;
@@ -108,19 +108,19 @@ __device__ inline uint32_t exit_hit(const drop::Shape& S, v3 p, v3 d, v3& q, v3&
     float tx;
     if (S.sphere) {
         const float cc = drop::dot(pf, pf) - 1.0f;
-        float disc = b * b - cc;
+        float disc = fmaf(b, b, -cc);
         if (disc < 0.0f) disc = 0.0f;
         tx = -b + sqrtf(disc);
     } else {
         const float cb = drop::dot(pf, pf) - S.rmax * S.rmax;
-        float disc = b * b - cb;
+        float disc = fmaf(b, b, -cb);
         if (disc < 0.0f) disc = 0.0f;
         const float tb = -b + sqrtf(disc);
         const float ts = 0x1p-16f;
-        if (!(ts < tb) || !(surface(S, pf + df * ts) < 0.0f)) return kFailed;
+        if (!(ts < tb) || !(surface(S, axpy(df, ts, pf)) < 0.0f)) return kFailed;
         tx = newton(S, pf, df, ts, tb, false, -2.0f * b);
     }
-    const f3 qf = pf + df * tx;
+    const f3 qf = axpy(df, tx, pf);
     q = Vd(qf);
     n = Vd(outward_normal(S, qf));
     return kHit;

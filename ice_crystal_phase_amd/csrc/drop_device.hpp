@@ -19,9 +19,13 @@
 // linear chain of hex_trace/include/ice_crystal/hex_trace_core.hpp:101-163:
 // external R at entry, T at every internal encounter, continue with R.
 //
-// Every operation is a separately rounded IEEE single-precision operation
-// (built with -ffp-contract=off, correctly rounded division and sqrt), so the
-// CPU mirror in the oracle reproduces each emitted direction and power bitwise.
+// Every operation is one correctly rounded IEEE single-precision operation:
+// +, -, *, / and sqrt, and fused multiply-adds (fmaf, one rounding) written out
+// at fixed places (dot and cross products, the Chebyshev recurrence, points on
+// a ray, the Fresnel and Jones algebra). The build has -ffp-contract=off, so
+// the compiler fuses nothing else, and the CPU mirror in the oracle, which
+// calls fmaf at the same places, reproduces each emitted direction and power
+// bitwise.
 #pragma once
 
 #include "phase_bin.hpp"
@@ -37,11 +41,14 @@ __device__ inline f3 operator+(f3 a, f3 b) { return F3(a.x + b.x, a.y + b.y, a.z
 __device__ inline f3 operator-(f3 a, f3 b) { return F3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ inline f3 operator*(f3 a, float s) { return F3(a.x * s, a.y * s, a.z * s); }
 __device__ inline f3 operator/(f3 a, float s) { return F3(a.x / s, a.y / s, a.z / s); }
-__device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// x-term product first, then the y and z terms fused in
+__device__ inline float dot(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 __device__ inline f3 cross(f3 a, f3 b)
 {
-    return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+    return F3(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
 }
+// o + d t, one rounding per component
+__device__ inline f3 axpy(f3 d, float t, f3 o) { return F3(fmaf(d.x, t, o.x), fmaf(d.y, t, o.y), fmaf(d.z, t, o.z)); }
 __device__ inline f3 unit(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
 
 struct Shape {
@@ -59,15 +66,16 @@ struct Shape {
 __device__ inline float cheb_radius(const float* c, uint32_t terms, float mu, float& der)
 {
     float prev = 1.0f, cur = mu, dprev = 0.0f, dcur = 1.0f;
-    float rad = (1.0f + c[0]) + c[1] * mu;
+    const float mu2 = 2.0f * mu;
+    float rad = fmaf(c[1], mu, 1.0f + c[0]);
     float d = c[1];
 #pragma unroll
     for (int n = 2; n < 8; ++n) {
         if ((uint32_t)n >= terms) break;
-        const float next = (2.0f * mu) * cur - prev;
-        const float dnext = (2.0f * cur + (2.0f * mu) * dcur) - dprev;
-        rad = rad + c[n] * next;
-        d = d + c[n] * dnext;
+        const float next = fmaf(mu2, cur, -prev);                 // T_{n}
+        const float dnext = fmaf(mu2, dcur, 2.0f * cur) - dprev;  // T'_{n}
+        rad = fmaf(c[n], next, rad);
+        d = fmaf(c[n], dnext, d);
         prev = cur;
         cur = next;
         dprev = dcur;
@@ -75,6 +83,13 @@ __device__ inline float cheb_radius(const float* c, uint32_t terms, float mu, fl
     }
     der = d;
     return rad;
+}
+
+// Outward (unnormalized) gradient u - (g - u mu) k, g = (0,-1,0), k = R'(mu)/r:
+// components u_i + (u_i mu) k and u_y + (1 + u_y mu) k.
+__device__ inline f3 outward(f3 u, float mu, float k)
+{
+    return F3(fmaf(u.x * mu, k, u.x), fmaf(fmaf(u.y, mu, 1.0f), k, u.y), fmaf(u.z * mu, k, u.z));
 }
 
 __device__ inline float surface(const Shape& s, f3 p)
@@ -96,9 +111,7 @@ __device__ inline f3 gradient(const Shape& s, f3 p)
     const float mu = -u.y;
     float der;
     (void)cheb_radius(s.c, s.terms, mu, der);
-    const float k = der * ir;
-    const f3 a = F3(0.0f - u.x * mu, -1.0f - u.y * mu, 0.0f - u.z * mu);
-    return u - a * k;
+    return outward(u, mu, der * ir);
 }
 
 // surface(s, p) and dot(gradient(s, p), d) from one evaluation of r, mu and the
@@ -115,9 +128,7 @@ __device__ inline float surface_slope(const Shape& s, f3 p, f3 d, float& slope)
     const float mu = -u.y;
     float der;
     const float rad = cheb_radius(s.c, s.terms, mu, der);
-    const float k = der * ir;
-    const f3 a = F3(0.0f - u.x * mu, -1.0f - u.y * mu, 0.0f - u.z * mu);
-    slope = dot(u - a * k, d);
+    slope = dot(outward(u, mu, der * ir), d);
     return r < s.inner * 0.25f ? r - s.inner : r - rad;
 }
 
@@ -125,9 +136,9 @@ __device__ inline float surface_slope(const Shape& s, f3 p, f3 d, float& slope)
 __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, bool a_pos, float x)
 {
     const float tol = 0x1p-21f;
-    if (!(x > a && x < b)) x = a + (b - a) * 0.5f;
+    if (!(x > a && x < b)) x = fmaf(b - a, 0.5f, a);
     for (int it = 0; it < 48; ++it) {
-        const f3 p = o + d * x;
+        const f3 p = axpy(d, x, o);
         float slope;
         const float fx = surface_slope(s, p, d, slope);
         if (fx == 0.0f) return x;
@@ -143,7 +154,7 @@ __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, boo
             x = cand;
             if (step <= tol * scale) return x;
         } else {
-            x = a + (b - a) * 0.5f;
+            x = fmaf(b - a, 0.5f, a);
         }
     }
     return x;
@@ -152,10 +163,17 @@ __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, boo
 struct c32 {
     float re, im;
 };
-__device__ inline c32 cmul(c32 a, c32 b) { return c32{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline c32 cmul(c32 a, c32 b)
+{
+    return c32{fmaf(a.re, b.re, -(a.im * b.im)), fmaf(a.re, b.im, a.im * b.re)};
+}
 __device__ inline c32 cscale(c32 a, float b) { return c32{a.re * b, a.im * b}; }
-__device__ inline c32 cadd(c32 a, c32 b) { return c32{a.re + b.re, a.im + b.im}; }
-__device__ inline float cn2(c32 a) { return a.re * a.re + a.im * a.im; }
+// x a + y b
+__device__ inline c32 clin(c32 x, float a, c32 y, float b)
+{
+    return c32{fmaf(y.re, b, x.re * a), fmaf(y.im, b, x.im * a)};
+}
+__device__ inline float cn2(c32 a) { return fmaf(a.im, a.im, a.re * a.re); }
 
 struct Jones {
     c32 c0x, c0y, c1x, c1y;
@@ -174,8 +192,8 @@ struct Split {
 __device__ inline c32 tir_coefficient(float a, float b)
 {
     const float s = fabsf(a) > fabsf(b) ? fabsf(a) : fabsf(b);
-    const float x = a / s, y = b / s, den = x * x + y * y;
-    return c32{(x * x - y * y) / den, (-2.0f * x * y) / den};
+    const float x = a / s, y = b / s, den = fmaf(y, y, x * x);
+    return c32{fmaf(x, x, -(y * y)) / den, (-2.0f * x * y) / den};
 }
 
 // n points into the incident medium.
@@ -196,18 +214,18 @@ __device__ inline Split split(const Ray& in, f3 n, float ni, float nt)
     if (s2 > 0x1p-40f)
         s = s * (1.0f / sine);
     else
-        s = unit(in.e0 - d * dot(d, in.e0));
+        s = unit(axpy(d, -dot(d, in.e0), in.e0));
     const f3 pin = cross(d, s);  // d, s unit and orthogonal
     const float a = dot(in.e0, s), b = dot(in.e1, s);
     const float cc = dot(in.e0, pin), dd = dot(in.e1, pin);
     Jones j;
-    j.c0x = cadd(cscale(in.j.c0x, a), cscale(in.j.c0y, b));
-    j.c0y = cadd(cscale(in.j.c0x, cc), cscale(in.j.c0y, dd));
-    j.c1x = cadd(cscale(in.j.c1x, a), cscale(in.j.c1y, b));
-    j.c1y = cadd(cscale(in.j.c1x, cc), cscale(in.j.c1y, dd));
+    j.c0x = clin(in.j.c0x, a, in.j.c0y, b);
+    j.c0y = clin(in.j.c0x, cc, in.j.c0y, dd);
+    j.c1x = clin(in.j.c1x, a, in.j.c1y, b);
+    j.c1y = clin(in.j.c1x, cc, in.j.c1y, dd);
     const float eta = ni / nt;
     const float st = eta * sine;
-    o.r.dir = unit(d + n * (2.0f * ci));
+    o.r.dir = unit(axpy(n, 2.0f * ci, d));
     o.r.e0 = s;
     o.r.e1 = cross(o.r.dir, s);
     c32 rs, rp;
@@ -218,13 +236,13 @@ __device__ inline Split split(const Ray& in, f3 n, float ni, float nt)
         o.tir = true;
     } else {
         const float ct = sqrtf((1.0f - st) * (1.0f + st));
-        const float ds = ni * ci + nt * ct, dp = nt * ci + ni * ct;
+        const float ds = fmaf(nt, ct, ni * ci), dp = fmaf(ni, ct, nt * ci);
         if (!(ds > 0.0f && dp > 0.0f)) return o;
-        rs = c32{(ni * ci - nt * ct) / ds, 0.0f};
-        rp = c32{(nt * ci - ni * ct) / dp, 0.0f};
+        rs = c32{fmaf(ni, ci, -(nt * ct)) / ds, 0.0f};
+        rp = c32{fmaf(nt, ci, -(ni * ct)) / dp, 0.0f};
         const float num = (2.0f * sqrtf(ni * ci)) * sqrtf(nt * ct);
         const float ts = num / ds, tp = num / dp;
-        o.t.dir = unit((d + n * ci) * eta - n * ct);
+        o.t.dir = unit(axpy(n, -ct, axpy(n, ci, d) * eta));
         o.t.e0 = s;
         o.t.e1 = cross(o.t.dir, s);
         o.t.j.c0x = cscale(j.c0x, ts);

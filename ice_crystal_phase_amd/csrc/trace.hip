@@ -198,12 +198,12 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
             const uint32_t ix = (uint32_t)(id % C.w), iy = (uint32_t)(id / C.w);
             u = ((float)(2u * ix + 1u) / (float)C.w - 1.0f) * C.h;
             v = ((float)(2u * iy + 1u) / (float)C.hgt - 1.0f) * C.h;
-            rho2 = u * u + v * v;
+            rho2 = fmaf(v, v, u * u);
             if (!(rho2 < rmax2)) {
                 count_status(c, kMiss);
                 continue;
             }
-            o = C.e0 * u + C.e1 * v;
+            o = axpy(C.e1, v, C.e0 * u);
             if (C.shape.sphere) {
                 have = true;
                 need = false;
@@ -224,7 +224,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
                 t1 = t1 < ta ? ta : t1;
                 t1 = t1 > -ta ? -ta : t1;
                 float g1;
-                const float f1 = surface_slope(C.shape, o + C.ki * t1, C.ki, g1);
+                const float f1 = surface_slope(C.shape, axpy(C.ki, t1, o), C.ki, g1);
                 if (f1 < 0.0f) {
                     tm = t1;
                     have = true;
@@ -250,11 +250,11 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
             t = -sqrtf(1.0f - rho2);
         } else {
             const float rm = 1.0f + C.shape.c[0];
-            const float g2 = rm * rm - rho2;
-            const float x0 = g2 > 0.0f ? -sqrtf(g2) : ta + (tm - ta) * 0.5f;
+            const float g2 = fmaf(rm, rm, -rho2);
+            const float x0 = g2 > 0.0f ? -sqrtf(g2) : fmaf(tm - ta, 0.5f, ta);
             t = newton(C.shape, o, C.ki, ta, tm, true, x0);
         }
-        f3 p = o + C.ki * t;
+        f3 p = axpy(C.ki, t, o);
         Ray ray;
         ray.dir = C.ki;
         ray.e0 = C.e0;
@@ -286,22 +286,22 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
                     float tx;
                     if (C.shape.sphere) {
                         const float cc = dot(p, p) - 1.0f;
-                        float disc = b * b - cc;
+                        float disc = fmaf(b, b, -cc);
                         if (disc < 0.0f) disc = 0.0f;
                         tx = -b + sqrtf(disc);
                     } else {
                         const float cb = dot(p, p) - rmax2;
-                        float disc = b * b - cb;
+                        float disc = fmaf(b, b, -cb);
                         if (disc < 0.0f) disc = 0.0f;
                         const float tb = -b + sqrtf(disc);
                         const float ts = 0x1p-16f;
-                        if (!(ts < tb) || !(surface(C.shape, p + ray.dir * ts) < 0.0f)) {
+                        if (!(ts < tb) || !(surface(C.shape, axpy(ray.dir, ts, p)) < 0.0f)) {
                             status = kUnresolved;
                             break;
                         }
                         tx = newton(C.shape, p, ray.dir, ts, tb, false, -2.0f * b);
                     }
-                    const f3 q = p + ray.dir * tx;
+                    const f3 q = axpy(ray.dir, tx, p);
                     const f3 nout = outward_normal(C.shape, q);
                     sp = split(ray, nout * -1.0f, C.n_int, C.n_ext);
                     if (!sp.ok) {

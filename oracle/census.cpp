@@ -5,9 +5,10 @@
 // The oracle's drop trace (oracle_drop_impl.h, the FP32 mirror of the kernel)
 // and its phase_bin rule (oracle_bin_rule.h) are instantiated here with
 // counting number types. One operation is counted for every +, -, *, / and
-// square root, and for every libm call the algorithm makes (hypot, atan2,
-// floor); comparisons, absolute values, min/max, negations and conversions are
-// not counted. The trace marks its events (oracle_drop_impl.h, CENSUS_MARK), so
+// square root, two for every fused multiply-add (a product and a sum, the
+// convention of the FP32 peak rate), and one for every libm call the algorithm
+// makes (hypot, atan2, floor); comparisons, absolute values, min/max,
+// negations and conversions are not counted. The trace marks its events (oracle_drop_impl.h, CENSUS_MARK), so
 // the operations between marks are attributed to them:
 //   miss      a ray that misses the drop: ray set-up and the entry search;
 //   entry     a hitting ray: set-up, entry solve, first split and its external
@@ -58,6 +59,7 @@ using D = Num<double>;
 
 inline F sqrt_f(F a) { ++ops; return F::of(std::sqrt(a.v)); }
 inline F fabs_f(F a) { return F::of(std::fabs(a.v)); }
+inline F fma_f(F a, F b, F c) { ops += 2; return F::of(std::fma(a.v, b.v, c.v)); }
 inline D sqrt_d(D a) { ++ops; return D::of(std::sqrt(a.v)); }
 inline D hypot_d(D a, D b) { ++ops; return D::of(std::hypot(a.v, b.v)); }
 inline D atan2_d(D a, D b) { ++ops; return D::of(std::atan2(a.v, b.v)); }
@@ -87,6 +89,7 @@ using census::F;
 #define SFX c
 #define SQRT census::sqrt_f
 #define FABS census::fabs_f
+#define FMA census::fma_f
 #define TOL 0x1p-21f
 #define TINY_S2 0x1p-40f
 #define SELF_OFFSET 0x1p-16f
@@ -96,6 +99,7 @@ using census::F;
 #undef SFX
 #undef SQRT
 #undef FABS
+#undef FMA
 
 // the phase_bin rule in FP64, counted
 #define BREAL D

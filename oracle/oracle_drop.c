@@ -19,6 +19,7 @@
 #define SFX f
 #define SQRT sqrtf
 #define FABS fabsf
+#define FMA fmaf
 #define TOL 0x1p-21f
 #define TINY_S2 0x1p-40f
 #define SELF_OFFSET 0x1p-16f
@@ -28,6 +29,7 @@
 #undef SFX
 #undef SQRT
 #undef FABS
+#undef FMA
 #undef TOL
 #undef TINY_S2
 #undef SELF_OFFSET
@@ -38,6 +40,7 @@
 #define SFX d
 #define SQRT sqrt
 #define FABS fabs
+#define FMA fma
 #define TOL 0x1p-44
 #define TINY_S2 0x1p-90
 #define SELF_OFFSET 0x1p-30
@@ -47,6 +50,7 @@
 #undef SFX
 #undef SQRT
 #undef FABS
+#undef FMA
 #undef TOL
 #undef TINY_S2
 #undef SELF_OFFSET
@@ -188,19 +192,19 @@ int orc_drop_entry_point(const orc_drop_record* r, int precision, double u, doub
     if (precision == 32) {
         rec_f R; load_f(r, &R);
         const float uf = (float)u, vf = (float)v;
-        const v3f o = add_f(scl_f(R.e0, uf), scl_f(R.e1, vf));
+        const v3f o = axpy_f(R.e1, vf, scl_f(R.e0, uf));
         float t;
         if (!entry_f(&R, o, uf, vf, &t)) return 0;
-        const v3f q = add_f(o, scl_f(R.ki, t));
+        const v3f q = axpy_f(R.ki, t, o);
         const v3f n = outward_normal_f(&R, q);
         p[0] = q.x; p[1] = q.y; p[2] = q.z;
         nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
     } else {
         rec_d R; load_d(r, &R);
-        const v3d o = add_d(scl_d(R.e0, u), scl_d(R.e1, v));
+        const v3d o = axpy_d(R.e1, v, scl_d(R.e0, u));
         double t;
         if (!entry_d(&R, o, u, v, &t)) return 0;
-        const v3d q = add_d(o, scl_d(R.ki, t));
+        const v3d q = axpy_d(R.ki, t, o);
         const v3d n = outward_normal_d(&R, q);
         p[0] = q.x; p[1] = q.y; p[2] = q.z;
         nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
@@ -217,7 +221,7 @@ int orc_drop_exit_point(const orc_drop_record* r, int precision, const double p[
         const v3f df = mk_f((float)d[0], (float)d[1], (float)d[2]);
         float t;
         if (!exit_f(&R, pf, df, &t)) return 0;
-        const v3f x = add_f(pf, scl_f(df, t));
+        const v3f x = axpy_f(df, t, pf);
         const v3f n = outward_normal_f(&R, x);
         q[0] = x.x; q[1] = x.y; q[2] = x.z;
         nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
@@ -226,7 +230,7 @@ int orc_drop_exit_point(const orc_drop_record* r, int precision, const double p[
         const v3d pd = mk_d(p[0], p[1], p[2]), dd = mk_d(d[0], d[1], d[2]);
         double t;
         if (!exit_d(&R, pd, dd, &t)) return 0;
-        const v3d x = add_d(pd, scl_d(dd, t));
+        const v3d x = axpy_d(dd, t, pd);
         const v3d n = outward_normal_d(&R, x);
         q[0] = x.x; q[1] = x.y; q[2] = x.z;
         nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;

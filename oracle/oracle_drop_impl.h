@@ -2,8 +2,11 @@
  * Oracle (test infrastructure only): the drop tracer algorithm written once for
  * REAL = float (the FP32 mirror of the HIP kernel's arithmetic, operation for
  * operation) and REAL = double (its FP64 twin). Included twice by oracle_drop.c
- * with SFX naming the instantiation. No FMA: every product and sum is a
- * separately rounded IEEE operation, as in the kernel (built -ffp-contract=off).
+ * with SFX naming the instantiation. Every operation is one correctly rounded
+ * IEEE operation, as in the kernel (built -ffp-contract=off): +, -, *, /,
+ * SQRT, and the fused multiply-adds FMA(a, b, c) = a b + c with one rounding
+ * (fmaf / fma) at exactly the kernel's places (dot and cross products, the
+ * Chebyshev recurrence, points on a ray, the Fresnel and Jones algebra).
  *
  * The reference has no CPU drop histogram (its drop density is an OptiX patch
  * solver). The geometry follows include/rainbow/raindrop_shape.hpp:70-100
@@ -33,13 +36,13 @@ typedef struct { V dir, e0, e1; J j; } FN(ray_);
 typedef struct { FN(ray_) r, t; int valid, has_t, tir; } FN(split_);
 
 static inline V FN(mk_)(REAL x, REAL y, REAL z) { V r = {x, y, z}; return r; }
-static inline V FN(add_)(V a, V b) { return FN(mk_)(a.x + b.x, a.y + b.y, a.z + b.z); }
-static inline V FN(sub_)(V a, V b) { return FN(mk_)(a.x - b.x, a.y - b.y, a.z - b.z); }
 static inline V FN(scl_)(V a, REAL s) { return FN(mk_)(a.x * s, a.y * s, a.z * s); }
 static inline V FN(div_)(V a, REAL s) { return FN(mk_)(a.x / s, a.y / s, a.z / s); }
-static inline REAL FN(dot_)(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline REAL FN(dot_)(V a, V b) { return FMA(a.z, b.z, FMA(a.y, b.y, a.x * b.x)); }
 static inline V FN(crs_)(V a, V b)
-{ return FN(mk_)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+{ return FN(mk_)(FMA(a.y, b.z, -(a.z * b.y)), FMA(a.z, b.x, -(a.x * b.z)), FMA(a.x, b.y, -(a.y * b.x))); }
+/* o + d t, one rounding per component */
+static inline V FN(axpy_)(V d, REAL t, V o) { return FN(mk_)(FMA(d.x, t, o.x), FMA(d.y, t, o.y), FMA(d.z, t, o.z)); }
 static inline V FN(unit_)(V a) { return FN(scl_)(a, (REAL)1 / SQRT(FN(dot_)(a, a))); }
 
 /* Chebyshev radius and slope, raindrop_shape.hpp:71-84 recurrence order,
@@ -48,13 +51,14 @@ static inline V FN(unit_)(V a) { return FN(scl_)(a, (REAL)1 / SQRT(FN(dot_)(a, a
 static inline REAL FN(radius_)(const REAL* c, uint32_t terms, REAL mu, REAL* der)
 {
     REAL prev = (REAL)1, cur = mu, dprev = (REAL)0, dcur = (REAL)1;
-    REAL rad = ((REAL)1 + c[0]) + c[1] * mu;
+    const REAL mu2 = (REAL)2 * mu;
+    REAL rad = FMA(c[1], mu, (REAL)1 + c[0]);
     REAL d = c[1];
     for (int n = 2; n < 8 && (uint32_t)n < terms; ++n) {
-        const REAL next = ((REAL)2 * mu) * cur - prev;
-        const REAL dnext = ((REAL)2 * cur + ((REAL)2 * mu) * dcur) - dprev;
-        rad = rad + c[n] * next;
-        d = d + c[n] * dnext;
+        const REAL next = FMA(mu2, cur, -prev);
+        const REAL dnext = FMA(mu2, dcur, (REAL)2 * cur) - dprev;
+        rad = FMA(c[n], next, rad);
+        d = FMA(c[n], dnext, d);
         prev = cur; cur = next; dprev = dcur; dcur = dnext;
     }
     *der = d;
@@ -69,6 +73,11 @@ typedef struct {
     uint32_t w, hgt, max_hits, sphere, terms;
 } FN(rec_);
 
+/* outward (unnormalized) gradient u - (g - u mu) k with g = (0,-1,0), k = R'(mu)/r:
+ * components u_i + (u_i mu) k and u_y + (1 + u_y mu) k */
+static inline V FN(outward_)(V u, REAL mu, REAL k)
+{ return FN(mk_)(FMA(u.x * mu, k, u.x), FMA(FMA(u.y, mu, (REAL)1), k, u.y), FMA(u.z * mu, k, u.z)); }
+
 static inline REAL FN(surf_)(const FN(rec_)* R, V p)
 {
     const REAL r = SQRT(FN(dot_)(p, p));
@@ -78,7 +87,7 @@ static inline REAL FN(surf_)(const FN(rec_)* R, V p)
     REAL der;
     return r - FN(radius_)(R->c, R->terms, mu, &der);
 }
-/* outward (unnormalized) gradient: u - (g - u mu) R'(mu)/r with g=(0,-1,0) */
+/* outward (unnormalized) gradient of f at p */
 static inline V FN(grad_)(const FN(rec_)* R, V p)
 {
     const REAL r = SQRT(FN(dot_)(p, p));
@@ -88,9 +97,7 @@ static inline V FN(grad_)(const FN(rec_)* R, V p)
     const REAL mu = -u.y;
     REAL der;
     (void)FN(radius_)(R->c, R->terms, mu, &der);
-    const REAL k = der * ir;
-    const V a = FN(mk_)((REAL)0 - u.x * mu, (REAL)-1 - u.y * mu, (REAL)0 - u.z * mu);
-    return FN(sub_)(u, FN(scl_)(a, k));
+    return FN(outward_)(u, mu, der * ir);
 }
 
 /* f(p) and grad f(p) . d from one evaluation of r, mu and the Chebyshev
@@ -108,9 +115,7 @@ static inline REAL FN(surf_slope_)(const FN(rec_)* R, V p, V d, REAL* slope)
     const REAL mu = -u.y;
     REAL der;
     const REAL rad = FN(radius_)(R->c, R->terms, mu, &der);
-    const REAL k = der * ir;
-    const V a = FN(mk_)((REAL)0 - u.x * mu, (REAL)-1 - u.y * mu, (REAL)0 - u.z * mu);
-    *slope = FN(dot_)(FN(sub_)(u, FN(scl_)(a, k)), d);
+    *slope = FN(dot_)(FN(outward_)(u, mu, der * ir), d);
     return r < R->inner * (REAL)0.25 ? r - R->inner : r - rad;
 }
 
@@ -119,9 +124,9 @@ static inline REAL FN(surf_slope_)(const FN(rec_)* R, V p, V d, REAL* slope)
 static REAL FN(newton_)(const FN(rec_)* R, V o, V d, REAL a, REAL b, int a_pos, REAL x)
 {
     const REAL tol = TOL;
-    if (!(x > a && x < b)) x = a + (b - a) * (REAL)0.5;
+    if (!(x > a && x < b)) x = FMA(b - a, (REAL)0.5, a);
     for (int it = 0; it < 48; ++it) {
-        const V p = FN(add_)(o, FN(scl_)(d, x));
+        const V p = FN(axpy_)(d, x, o);
         REAL slope;
         const REAL fx = FN(surf_slope_)(R, p, d, &slope);
         if (fx == (REAL)0) return x;
@@ -134,7 +139,7 @@ static REAL FN(newton_)(const FN(rec_)* R, V o, V d, REAL a, REAL b, int a_pos, 
             x = cand;
             if (step <= tol * scale) return x;
         } else {
-            x = a + (b - a) * (REAL)0.5;
+            x = FMA(b - a, (REAL)0.5, a);
         }
     }
     return x;
@@ -144,7 +149,7 @@ static REAL FN(newton_)(const FN(rec_)* R, V o, V d, REAL a, REAL b, int a_pos, 
  * Returns 1 and writes t on a hit, 0 on a miss. */
 static int FN(entry_)(const FN(rec_)* R, V o, REAL u, REAL v, REAL* tout)
 {
-    const REAL rho2 = u * u + v * v;
+    const REAL rho2 = FMA(v, v, u * u);
     if (R->sphere) {
         if (!(rho2 < (REAL)1)) return 0;
         *tout = -SQRT((REAL)1 - rho2);
@@ -167,7 +172,7 @@ static int FN(entry_)(const FN(rec_)* R, V o, REAL u, REAL v, REAL* tout)
         for (int it = 0; it < 8; ++it) {
             if (t1 < ta) t1 = ta;
             if (t1 > -ta) t1 = -ta;
-            const V p1 = FN(add_)(o, FN(scl_)(R->ki, t1));
+            const V p1 = FN(axpy_)(R->ki, t1, o);
             REAL g1;
             const REAL f1 = FN(surf_slope_)(R, p1, R->ki, &g1);
             if (f1 < (REAL)0) { tm = t1; found = 1; break; }
@@ -181,8 +186,8 @@ static int FN(entry_)(const FN(rec_)* R, V o, REAL u, REAL v, REAL* tout)
     }
     /* sphere-like first guess from the mean radius */
     const REAL rm = (REAL)1 + R->c[0];
-    const REAL g2 = rm * rm - rho2;
-    const REAL x0 = g2 > (REAL)0 ? -SQRT(g2) : ta + (tm - ta) * (REAL)0.5;
+    const REAL g2 = FMA(rm, rm, -rho2);
+    const REAL x0 = g2 > (REAL)0 ? -SQRT(g2) : FMA(tm - ta, (REAL)0.5, ta);
     *tout = FN(newton_)(R, o, R->ki, ta, tm, 1, x0);
     return 1;
 }
@@ -193,33 +198,34 @@ static int FN(exit_)(const FN(rec_)* R, V p, V d, REAL* tout)
     const REAL b = FN(dot_)(p, d);
     if (R->sphere) {
         const REAL c = FN(dot_)(p, p) - (REAL)1;
-        REAL disc = b * b - c;
+        REAL disc = FMA(b, b, -c);
         if (disc < (REAL)0) disc = (REAL)0;
         *tout = -b + SQRT(disc);
         return 1;
     }
     const REAL cb = FN(dot_)(p, p) - R->rmax * R->rmax;
-    REAL disc = b * b - cb;
+    REAL disc = FMA(b, b, -cb);
     if (disc < (REAL)0) disc = (REAL)0;
     const REAL tb = -b + SQRT(disc);
     const REAL ts = SELF_OFFSET;
     if (!(ts < tb)) return 0;
-    if (!(FN(surf_)(R, FN(add_)(p, FN(scl_)(d, ts))) < (REAL)0)) return 0;
+    if (!(FN(surf_)(R, FN(axpy_)(d, ts, p)) < (REAL)0)) return 0;
     *tout = FN(newton_)(R, p, d, ts, tb, 0, (REAL)-2 * b);
     return 1;
 }
 
-static inline C FN(cm_)(C a, C b) { C r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
+static inline C FN(cm_)(C a, C b) { C r = {FMA(a.re, b.re, -(a.im * b.im)), FMA(a.re, b.im, a.im * b.re)}; return r; }
 static inline C FN(cr_)(C a, REAL b) { C r = {a.re * b, a.im * b}; return r; }
-static inline C FN(ca_)(C a, C b) { C r = {a.re + b.re, a.im + b.im}; return r; }
-static inline REAL FN(n2_)(C a) { return a.re * a.re + a.im * a.im; }
+/* x a + y b */
+static inline C FN(cl_)(C x, REAL a, C y, REAL b) { C r = {FMA(y.re, b, x.re * a), FMA(y.im, b, x.im * a)}; return r; }
+static inline REAL FN(n2_)(C a) { return FMA(a.im, a.im, a.re * a.re); }
 static inline REAL FN(pow_)(const J* j)
 { return (REAL)0.5 * (((FN(n2_)(j->c0x) + FN(n2_)(j->c0y)) + FN(n2_)(j->c1x)) + FN(n2_)(j->c1y)); }
 static inline C FN(tir_)(REAL a, REAL b)
 {
     const REAL s = FABS(a) > FABS(b) ? FABS(a) : FABS(b);
-    const REAL x = a / s, y = b / s, den = x * x + y * y;
-    C r = {(x * x - y * y) / den, ((REAL)-2 * x * y) / den};
+    const REAL x = a / s, y = b / s, den = FMA(y, y, x * x);
+    C r = {FMA(x, x, -(y * y)) / den, ((REAL)-2 * x * y) / den};
     return r;
 }
 
@@ -237,18 +243,18 @@ static FN(split_) FN(split_fn_)(const FN(ray_)* in, V n, REAL ni, REAL nt)
     const REAL s2 = FN(dot_)(s, s);
     const REAL sine = SQRT(s2);
     if (s2 > TINY_S2) s = FN(scl_)(s, (REAL)1 / sine);
-    else s = FN(unit_)(FN(sub_)(in->e0, FN(scl_)(d, FN(dot_)(d, in->e0))));
+    else s = FN(unit_)(FN(axpy_)(d, -FN(dot_)(d, in->e0), in->e0));
     const V pin = FN(crs_)(d, s); /* d, s unit and orthogonal */
     const REAL a = FN(dot_)(in->e0, s), b = FN(dot_)(in->e1, s);
     const REAL cc = FN(dot_)(in->e0, pin), dd = FN(dot_)(in->e1, pin);
     J j;
-    j.c0x = FN(ca_)(FN(cr_)(in->j.c0x, a), FN(cr_)(in->j.c0y, b));
-    j.c0y = FN(ca_)(FN(cr_)(in->j.c0x, cc), FN(cr_)(in->j.c0y, dd));
-    j.c1x = FN(ca_)(FN(cr_)(in->j.c1x, a), FN(cr_)(in->j.c1y, b));
-    j.c1y = FN(ca_)(FN(cr_)(in->j.c1x, cc), FN(cr_)(in->j.c1y, dd));
+    j.c0x = FN(cl_)(in->j.c0x, a, in->j.c0y, b);
+    j.c0y = FN(cl_)(in->j.c0x, cc, in->j.c0y, dd);
+    j.c1x = FN(cl_)(in->j.c1x, a, in->j.c1y, b);
+    j.c1y = FN(cl_)(in->j.c1x, cc, in->j.c1y, dd);
     const REAL eta = ni / nt;
     const REAL st = eta * sine;
-    o.r.dir = FN(unit_)(FN(add_)(d, FN(scl_)(n, (REAL)2 * ci)));
+    o.r.dir = FN(unit_)(FN(axpy_)(n, (REAL)2 * ci, d));
     o.r.e0 = s;
     o.r.e1 = FN(crs_)(o.r.dir, s);
     C rs, rp;
@@ -259,13 +265,13 @@ static FN(split_) FN(split_fn_)(const FN(ray_)* in, V n, REAL ni, REAL nt)
         o.tir = 1;
     } else {
         const REAL ct = SQRT(((REAL)1 - st) * ((REAL)1 + st));
-        const REAL ds = ni * ci + nt * ct, dp = nt * ci + ni * ct;
+        const REAL ds = FMA(nt, ct, ni * ci), dp = FMA(ni, ct, nt * ci);
         if (!(ds > (REAL)0 && dp > (REAL)0)) return o;
-        rs.re = (ni * ci - nt * ct) / ds; rs.im = (REAL)0;
-        rp.re = (nt * ci - ni * ct) / dp; rp.im = (REAL)0;
+        rs.re = FMA(ni, ci, -(nt * ct)) / ds; rs.im = (REAL)0;
+        rp.re = FMA(nt, ci, -(ni * ct)) / dp; rp.im = (REAL)0;
         const REAL num = ((REAL)2 * SQRT(ni * ci)) * SQRT(nt * ct);
         const REAL ts = num / ds, tp = num / dp;
-        o.t.dir = FN(unit_)(FN(sub_)(FN(scl_)(FN(add_)(d, FN(scl_)(n, ci)), eta), FN(scl_)(n, ct)));
+        o.t.dir = FN(unit_)(FN(axpy_)(n, -ct, FN(scl_)(FN(axpy_)(n, ci, d), eta)));
         o.t.e0 = s;
         o.t.e1 = FN(crs_)(o.t.dir, s);
         o.t.j.c0x = FN(cr_)(j.c0x, ts); o.t.j.c0y = FN(cr_)(j.c0y, tp);
@@ -289,10 +295,10 @@ static orc_ray_audit FN(trace_origin_)(const FN(rec_)* R, uint64_t id, REAL u, R
     memset(&a, 0, sizeof a);
     a.status = 8; /* miss */
     CENSUS_MARK(0);
-    const V o = FN(add_)(FN(scl_)(R->e0, u), FN(scl_)(R->e1, v));
+    const V o = FN(axpy_)(R->e1, v, FN(scl_)(R->e0, u));
     REAL t;
     if (!FN(entry_)(R, o, u, v, &t)) { a.unresolved = 0; CENSUS_MARK(1); return a; }
-    V p = FN(add_)(o, FN(scl_)(R->ki, t));
+    V p = FN(axpy_)(R->ki, t, o);
     FN(ray_) ray;
     ray.dir = R->ki; ray.e0 = R->e0; ray.e1 = R->e1;
     memset(&ray.j, 0, sizeof ray.j);
@@ -326,7 +332,7 @@ static orc_ray_audit FN(trace_origin_)(const FN(rec_)* R, uint64_t id, REAL u, R
         if (pw <= R->tail) { a.status = 1; break; }
         REAL tx;
         if (!FN(exit_)(R, p, ray.dir, &tx)) { a.status = 3; break; }
-        const V q = FN(add_)(p, FN(scl_)(ray.dir, tx));
+        const V q = FN(axpy_)(ray.dir, tx, p);
         const V nout = FN(outward_normal_)(R, q);
         sp = FN(split_fn_)(&ray, FN(scl_)(nout, (REAL)-1), R->n_int, R->n_ext);
         if (!sp.valid) { a.status = ST_NUMERIC_D; break; }
