@@ -76,7 +76,7 @@ BASE_COEFFS = (-0.0458, -0.0335, -0.1211, -0.0227)
 # fused multiply-add counts as two operations (as the FP32 peak counts it); the
 # compiler fuses nothing else (FMA contraction off).
 CENSUS = {
-    "drop": {"miss": 74.8, "entry": 616.7, "hit": 582.4, "emit": 46.5},
+    "drop": {"miss": 74.8, "entry": 617.5, "hit": 565.4, "emit": 46.5},
     "sphere": {"miss": 12.0, "entry": 269.0, "hit": 273.0, "emit": 46.5},
 }
 PEAK_TFLOPS_FP32 = 157.3   # MI355X published FP32 vector peak
@@ -157,9 +157,12 @@ def drop_traffic(args):
     return None
 
 
-def atomic_calibration(ctx, count: int, trace_ms: float, batch_cells: int, record_cells: int) -> dict:
-    """The trace kernel's u64 atomicAdd rate (one per binned emission) beside
-    what the device sustains for the same operation alone (iph_atomic_probe):
+def atomic_calibration(ctx, count: int, trace_ms: float, batch_cells: int, record_cells: int,
+                       emissions: int | None = None) -> dict:
+    """The trace kernel's u64 atomicAdd rate (the cell atomics it issued, from
+    the records' cell_atomics counters; the drop kernel merges equal cells of
+    adjacent lanes, so there are fewer than binned emissions) beside what the
+    device sustains for the same operation alone (iph_atomic_probe):
     consecutive cells of one record's slab (a wave updates 64 adjacent cells:
     the ceiling the fraction is taken against), random cells over the batch's
     whole histogram slab and over one record's slab, and one hot cell."""
@@ -175,6 +178,7 @@ def atomic_calibration(ctx, count: int, trace_ms: float, batch_cells: int, recor
         rates[name] = {"cells": cells, "atomics": n, "ms": ms, "per_s": n / (ms / 1e3)}
     peak = rates["coalesced_record_slab"]["per_s"]
     return {"count_per_launch": count, "per_s": per_s, "peak": peak,
+            "emissions_per_launch": emissions,
             "peak_definition": "u64 atomicAdd(+1) to consecutive cells of one record's histogram slab (each "
                                "wave's 64 lanes on 64 adjacent cells), launched like the trace kernels "
                                "(iph_atomic_probe mode 2)",
@@ -910,7 +914,8 @@ def main() -> int:
     trace_avg = float(np.mean(trace_ms)) if trace_ms[0] >= 0 else float("nan")
     cdf_avg = float(np.mean(cdf_ms)) if cdf_ms[0] >= 0 else float("nan")
     flops = census_flops(stats, ["drop"] * len(stats))
-    atomics = sum(int(st.samples) for st in stats)  # one u64 atomicAdd per binned sample
+    atomics = sum(int(st.cell_atomics) for st in stats)   # u64 atomicAdds issued into cells
+    emissions = sum(int(st.samples) for st in stats)      # binned samples (emissions with power)
     achieved = flops / (trace_avg / 1e3) / 1e12 if trace_avg == trace_avg else None
     # CDF HBM figure (algorithmic bytes: read 8 B per cell, write phi + theta)
     nt, np_ = args.theta, args.phi
@@ -945,7 +950,7 @@ def main() -> int:
                      "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
                      "frac": (achieved / PEAK_TFLOPS_FP32) if achieved else None,
                      "traffic": drop_traffic(args), "flops_per_launch": flops},
-        "atomics": atomic_calibration(ctx, atomics, trace_avg, len(recs) * nt * np_, nt * np_),
+        "atomics": atomic_calibration(ctx, atomics, trace_avg, len(recs) * nt * np_, nt * np_, emissions),
     }
     batch.close()
     if not args.no_extras:

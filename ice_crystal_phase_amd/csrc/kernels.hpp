@@ -18,6 +18,7 @@ enum StatSlot : uint32_t {
     kStatEscaped = 14,
     kStatUnresolved = 15,
     kStatInternalHits = 16,
+    kStatCellAtomics = 20,   // u64 atomicAdds issued into histogram cells
     kStatDiffraction = 17,   // quanta of the Fraunhofer lobe
     kStatRayBalance = 18,    // area-rule records: max |escaped + unresolved - 1| of a ray (FP64 bits)
     kStatFaceBalance = 19,   // area-rule records: max |R + T - incident| of an interface (FP64 bits)

@@ -7,8 +7,9 @@
 // edges with the 64-eps / 1e-7-cell snap, phi = atan2(y, x) with the periodic
 // seam and the same snap rule. Three changes of method keep the result and cut
 // the cost on the GPU:
-//   * a direction whose FP32 (theta, phi) lies well inside a cell (see bin_core)
-//     takes that cell directly; only the ~4 % near cell edges run the FP64 rules;
+//   * a direction whose FP32 (theta, phi) lies well inside a cell (bin_fast)
+//     takes that cell directly; only the ~4 % near cell edges run the FP64 rules
+//     (bin_exact);
 //   * the u row is guessed from u = sin^2(pi i / 2n) in FP32 and then corrected
 //     against the FP64 edges, which returns the same row as the reference's
 //     binary search for any u in [0,1];
@@ -76,33 +77,37 @@ struct CellHit {
 constexpr double kPhiMargin = 2e-3;
 constexpr float kCellMargin = 1e-2f;
 
-// The phase_bin rules after the unit-length checks: (x, y, z) are the frame
-// projections of the direction, n their norm. edge(k) returns the FP64 u edge k.
+constexpr int32_t kUndecided = 3;
+
+// Fast cell decision: theta and phi in FP32 from the FP64 projections (x, y, z)
+// rounded to FP32. That rounding is a relative error of 6e-8, so theta and phi
+// carry at most ~1.2e-6 rad of error; when both fractional cell positions are at
+// least kCellMargin (1e-2 cells = 3.5e-5 rad for theta, 3.5e-5 rad for phi at
+// 1800 columns) from an edge, the FP64 rules (bin_exact) give this same cell with
+// no snap and no pole. Returns that cell (kind 0) or kind kUndecided.
+__host__ __device__ inline CellHit bin_fast(uint32_t nt, uint32_t np, double x, double y, double z)
+{
+    const float xf = (float)x, yf = (float)y, zf = (float)z;
+    const float th = atan2f(sqrtf(xf * xf + yf * yf), zf);
+    const float ri = th * ((float)nt * (float)(1.0 / kPi));
+    const float ph = atan2f(yf, xf);
+    const float ci = (ph + (float)kPi) * ((float)np * (float)(0.5 / kPi));
+    const float rfl = floorf(ri), cfl = floorf(ci);
+    const float rfr = ri - rfl, cfr = ci - cfl;
+    if (rfr > kCellMargin && rfr < 1.0f - kCellMargin && cfr > kCellMargin && cfr < 1.0f - kCellMargin &&
+        rfl >= 0.0f && rfl < (float)nt && cfl >= 0.0f && cfl < (float)np)
+        return CellHit{(uint32_t)rfl * np + (uint32_t)cfl, 0, 0};
+    return CellHit{0, kUndecided, 0};
+}
+
+// The phase_bin rules after the unit-length checks, in FP64: (x, y, z) are the
+// frame projections of the direction, n their norm. edge(k) returns the FP64 u
+// edge k.
 template <class EdgeLoad>
-__host__ __device__ inline CellHit bin_core(uint32_t nt, uint32_t np, double x, double y, double z, double n,
-                                            EdgeLoad edge)
+__host__ __device__ inline CellHit bin_exact(uint32_t nt, uint32_t np, double x, double y, double z, double n,
+                                             EdgeLoad edge)
 {
     CellHit h{0, 2, 0};
-    // Fast path: theta and phi in FP32 from the FP64 projections. Rounding x, y, z
-    // to FP32 is a relative error of 6e-8, so theta and phi carry at most ~1.2e-6
-    // rad of error; when both fractional cell positions are at least kCellMargin
-    // (1e-2 cells = 3.5e-5 rad for theta, 3.5e-5 rad for phi at 1800 columns) from
-    // an edge, the FP64 rules below give this same cell with no snap and no pole.
-    {
-        const float xf = (float)x, yf = (float)y, zf = (float)z;
-        const float th = atan2f(sqrtf(xf * xf + yf * yf), zf);
-        const float ri = th * ((float)nt * (float)(1.0 / kPi));
-        const float ph = atan2f(yf, xf);
-        const float ci = (ph + (float)kPi) * ((float)np * (float)(0.5 / kPi));
-        const float rfl = floorf(ri), cfl = floorf(ci);
-        const float rfr = ri - rfl, cfr = ci - cfl;
-        if (rfr > kCellMargin && rfr < 1.0f - kCellMargin && cfr > kCellMargin && cfr < 1.0f - kCellMargin &&
-            rfl >= 0.0f && rfl < (float)nt && cfl >= 0.0f && cfl < (float)np) {
-            h.index = (uint32_t)rfl * np + (uint32_t)cfl;
-            h.kind = 0;
-            return h;
-        }
-    }
     const double mu = z / n, s = hypot(x, y) / n;
     if (s <= 256 * DBL_EPSILON) {
         h.kind = mu >= 0 ? 1 : -1;
@@ -157,6 +162,15 @@ __host__ __device__ inline CellHit bin_core(uint32_t nt, uint32_t np, double x, 
     h.kind = 0;
     h.snapped = snapped;
     return h;
+}
+
+// The fast decision, then the FP64 rules where it leaves the cell undecided.
+template <class EdgeLoad>
+__host__ __device__ inline CellHit bin_core(uint32_t nt, uint32_t np, double x, double y, double z, double n,
+                                            EdgeLoad edge)
+{
+    const CellHit f = bin_fast(nt, np, x, y, z);
+    return f.kind == 0 ? f : bin_exact(nt, np, x, y, z, n, edge);
 }
 
 // phase_bin of a unit FP64 direction (hex records): the reference's unit checks first.

@@ -731,6 +731,7 @@ int iph_batch_stats(iph_batch* b, uint32_t rec, iph_hist_stats* out)
     out->ray_weight_mm2 = b->weight[rec];
     out->projected_area_mm2 = b->group_area[rec];
     out->q_diffraction = st[kStatDiffraction];
+    out->cell_atomics = st[kStatCellAtomics];
     out->diffraction_captured = 0;
     bool first = true;
     for (uint32_t r : b->lobe_recs) {

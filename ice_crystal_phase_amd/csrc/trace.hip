@@ -28,16 +28,18 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
 struct LaneCounters {
     unsigned long long fwd, bwd, samples, invalid, snapped, escaped, unresolved, hits;
     uint32_t status[9];
+    uint32_t atomics;  // cell atomicAdds issued (per tile, so 32 bits per lane suffice)
 };
 
 __device__ inline void flush_counters(LaneCounters& c, unsigned long long* stats)
 {
-    const unsigned long long v[8] = {c.fwd, c.bwd, c.samples, c.invalid, c.snapped, c.escaped, c.unresolved, c.hits};
-    const uint32_t slot[8] = {kStatForward, kStatBackward, kStatSamples, kStatInvalid, kStatSnapped, kStatEscaped,
-                              kStatUnresolved, kStatInternalHits};
+    const unsigned long long v[9] = {c.fwd,     c.bwd,     c.samples,    c.invalid,         c.snapped,
+                                     c.escaped, c.unresolved, c.hits, (unsigned long long)c.atomics};
+    const uint32_t slot[9] = {kStatForward, kStatBackward, kStatSamples,    kStatInvalid,    kStatSnapped,
+                              kStatEscaped, kStatUnresolved, kStatInternalHits, kStatCellAtomics};
     const bool lead = (threadIdx.x & 63) == 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 9; ++k) {
         const unsigned long long s = wave_sum(v[k]);
         if (lead && s) atomicAdd(&stats[slot[k]], s);
     }
@@ -82,6 +84,7 @@ __device__ inline void deposit(const DevRecord& R, d3 dir, double power, EdgeLoa
         c.bwd += q;
     } else if (q) {
         atomicAdd(&R.bins[h.index], q);
+        ++c.atomics;
     }
 }
 
@@ -136,42 +139,137 @@ __device__ inline void trace_tile(const DevRecord& R, uint64_t first, uint64_t c
     }
 }
 
+// Cell bookkeeping of one binned emission: pole quanta and audit counters stay
+// in lane registers; a cell gets the quantum by one u64 atomicAdd.
+__device__ inline void record_hit(const DevRecord& R, const CellHit& h, unsigned long long q, LaneCounters& c)
+{
+    if (h.kind == 2) {
+        ++c.invalid;
+        return;
+    }
+    c.snapped += h.snapped;
+    if (h.kind == 1)
+        c.fwd += q;
+    else if (h.kind == -1)
+        c.bwd += q;
+    else if (q) {
+        atomicAdd(&R.bins[h.index], q);
+        ++c.atomics;
+    }
+}
+
+// An emission whose cell the FP32 decision (bin_fast) leaves open: its FP32
+// direction and power, binned later by the FP64 rules.
+struct Deferred {
+    float x, y, z, pw;
+};
+constexpr uint32_t kRing = 128;  // deferred emissions per wave (a power of two, at least 2 x 64)
+
+// phase_bin of a deferred drop emission: bin_raw's checks and FP64 rules (the
+// FP32 decision already failed, and where it succeeds it agrees with them).
+template <class EdgeLoad>
+__device__ inline void bin_deferred(const DevRecord& R, const Deferred& e, EdgeLoad edge, LaneCounters& c)
+{
+    const unsigned long long q = quantize((double)e.pw, R.qbits);
+    const d3 d = D3((double)e.x, (double)e.y, (double)e.z);
+    CellHit h{0, 2, 0};
+    if (finite64(d)) {
+        const double x = dot(R.frame.e0, d), y = dot(R.frame.e1, d), z = dot(R.frame.ki, d);
+        const double n = sqrt(x * x + y * y + z * z);
+        if (n > 0 && finite64(n)) h = bin_exact(R.nt, R.np, x, y, z, n, edge);
+    }
+    record_hit(R, h, q, c);
+}
+
 // Drop tile. Lanes first acquire a ray that hits the drop: ids come from the
 // tile's LDS counter (one wave-aggregated atomic per round), rays outside the
 // bounding sphere are counted as misses on the spot, and the entry bracket of a
 // candidate is set up here (foot of the perpendicular, else a secant search for
-// the minimum of f along the line). The acquired rays are then traced together:
-// with no tail tolerance every hitting ray runs the same number of encounters,
-// so the wave stays converged apart from Newton iteration counts. The sequence
-// of operations applied to each ray is the per-ray algorithm the CPU mirror runs.
+// the minimum of f along the line). The acquired rays are then traced together,
+// one encounter per round of the whole wave: with no tail tolerance every
+// hitting ray runs the same number of encounters, so the wave stays converged
+// apart from Newton iteration counts. The sequence of operations applied to each
+// ray is the per-ray algorithm the CPU mirror runs.
+//
+// Emissions are binned where the whole wave runs (after the entry and after each
+// encounter round). The FP32 decision bins ~96 % of them on the spot; the rest
+// go to the wave's LDS ring and are binned by the FP64 rules 64 at a time, so the
+// FP64 path runs with every lane busy instead of in nearly every round for a
+// lane or two (with 4 % undecided, 93 % of 64-lane rounds hold one). Integer
+// quanta make the histogram independent of when an emission is binned.
 template <class EdgeLoad>
 __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t count, uint32_t* s_next,
-                                 EdgeLoad edge, LaneCounters& c)
+                                 Deferred* ring, EdgeLoad edge, LaneCounters& c)
 {
     using namespace drop;
     const Config& C = R.drop;
     const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const float rmax2 = C.shape.sphere ? 1.0f : C.shape.rmax * C.shape.rmax;
     bool exhausted = false;
     unsigned long long rq = 0;
-    auto emit = [&](const Ray& r) {
-        const float pw = r.j.power();
-        if (pw > 0.0f) {
-            const unsigned long long q = quantize((double)pw, R.qbits);
-            rq += q;
-            ++c.samples;
-            const CellHit h = bin_raw(R.frame, R.nt, R.np, D3((double)r.dir.x, (double)r.dir.y, (double)r.dir.z), edge);
-            if (h.kind == 2) {
-                ++c.invalid;
-                return;
+    // ring positions: changed only where every lane of the wave runs, so every
+    // lane holds the same values
+    uint32_t head = 0, tail = 0;
+    auto drain = [&](uint32_t n) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < n) bin_deferred(R, ring[(head + lane) & (kRing - 1)], edge, c);
+        head += n;
+    };
+    // One emission per lane where `valid`; called by the whole wave. Adjacent
+    // lanes trace adjacent rays, so their emissions of one round often share a
+    // cell (the headline oblate-drop batch: 63 emissions per round fall in 25
+    // cells, always as runs of adjacent lanes): a segmented sum over each run of
+    // equal cells leaves one u64 atomicAdd per run, issued by its first lane.
+    auto emit = [&](bool valid, const Ray& r) {
+        bool defer = false;
+        float pw = 0.0f;
+        uint32_t cell = 0xffffffffu;  // no cell: no quantum to add
+        unsigned long long sum = 0;
+        if (valid) {
+            pw = r.j.power();
+            if (pw > 0.0f) {
+                const unsigned long long q = quantize((double)pw, R.qbits);
+                rq += q;
+                ++c.samples;
+                const d3 d = D3((double)r.dir.x, (double)r.dir.y, (double)r.dir.z);
+                if (finite64(d)) {
+                    const CellHit h =
+                        bin_fast(R.nt, R.np, dot(R.frame.e0, d), dot(R.frame.e1, d), dot(R.frame.ki, d));
+                    if (h.kind == 0) {
+                        cell = h.index;
+                        sum = q;
+                    } else {
+                        defer = true;
+                    }
+                } else {
+                    defer = true;
+                }
             }
-            c.snapped += h.snapped;
-            if (h.kind == 1)
-                c.fwd += q;
-            else if (h.kind == -1)
-                c.bwd += q;
-            else if (q)
-                atomicAdd(&R.bins[h.index], q);
+        }
+        {
+            // every lane shuffles: a lane outside the exec mask would hand an
+            // undefined value to its neighbour
+            const uint32_t prev = __shfl_up(cell, 1, 64);
+            const bool first_of_run = lane == 0 || prev != cell;
+            // run number of each lane: first-of-run lanes at or below it
+            const uint32_t run = (uint32_t)__popcll(__ballot(first_of_run) & (below | (1ull << lane)));
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {  // sum of the lane's run from the lane on
+                const unsigned long long v = __shfl_down(sum, off, 64);
+                const uint32_t vrun = __shfl_down(run, off, 64);  // every lane shuffles
+                if (lane + off < 64 && vrun == run) sum += v;
+            }
+            if (first_of_run && cell != 0xffffffffu && sum) {
+                atomicAdd(&R.bins[cell], sum);
+                ++c.atomics;
+            }
+        }
+        const unsigned long long m = __ballot(defer);
+        if (m) {
+            if (defer) ring[(tail + (uint32_t)__popcll(m & below)) & (kRing - 1)] = Deferred{r.dir.x, r.dir.y, r.dir.z, pw};
+            tail += (uint32_t)__popcll(m);
+            if (tail - head >= 64) drain(64);  // leaves fewer than 64, so the next round fits
         }
     };
     for (;;) {
@@ -243,47 +341,59 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
                 count_status(c, kMiss);
         }
         if (__ballot(have) == 0) break;
-        if (!have) continue;
-        // trace the acquired ray
-        float t;
-        if (C.shape.sphere) {
-            t = -sqrtf(1.0f - rho2);
-        } else {
-            const float rm = 1.0f + C.shape.c[0];
-            const float g2 = fmaf(rm, rm, -rho2);
-            const float x0 = g2 > 0.0f ? -sqrtf(g2) : fmaf(tm - ta, 0.5f, ta);
-            t = newton(C.shape, o, C.ki, ta, tm, true, x0);
-        }
-        f3 p = axpy(C.ki, t, o);
+        // From here every lane runs each round; lanes without a ray only take
+        // part in the emissions.
         Ray ray;
-        ray.dir = C.ki;
-        ray.e0 = C.e0;
-        ray.e1 = C.e1;
-        ray.j = Jones{c32{1.0f, 0.0f}, c32{0.0f, 0.0f}, c32{0.0f, 0.0f}, c32{1.0f, 0.0f}};
-        Split sp = split(ray, outward_normal(C.shape, p), C.n_ext, C.n_int);
+        Split sp;
+        sp.ok = false;
+        sp.has_t = false;
+        f3 p = F3(0.0f, 0.0f, 0.0f);
         uint32_t status = kLimit, hits = 0;
         float unresolved = 0.0f;
-        if (!sp.ok) {
-            status = kNumeric;
-            unresolved = 1.0f;
-        } else {
-            emit(sp.r);
-            if (!sp.has_t) {
+        bool alive = false, inner = false;
+        if (have) {
+            float t;
+            if (C.shape.sphere) {
+                t = -sqrtf(1.0f - rho2);
+            } else {
+                const float rm = 1.0f + C.shape.c[0];
+                const float g2 = fmaf(rm, rm, -rho2);
+                const float x0 = g2 > 0.0f ? -sqrtf(g2) : fmaf(tm - ta, 0.5f, ta);
+                t = newton(C.shape, o, C.ki, ta, tm, true, x0);
+            }
+            p = axpy(C.ki, t, o);
+            ray.dir = C.ki;
+            ray.e0 = C.e0;
+            ray.e1 = C.e1;
+            ray.j = Jones{c32{1.0f, 0.0f}, c32{0.0f, 0.0f}, c32{0.0f, 0.0f}, c32{1.0f, 0.0f}};
+            sp = split(ray, outward_normal(C.shape, p), C.n_ext, C.n_int);
+            if (!sp.ok) {
+                status = kNumeric;
+                unresolved = 1.0f;
+            } else if (!sp.has_t) {
                 status = kExhausted;
             } else {
                 ray = sp.t;
-                for (uint32_t hit = 0; hit < C.max_hits; ++hit) {
-                    const float pw = ray.j.power();
-                    if (pw == 0.0f) {
-                        status = kExhausted;
-                        break;
-                    }
-                    if (pw <= C.tail) {
-                        status = kTail;
-                        break;
-                    }
+                alive = true;
+                inner = true;
+            }
+        }
+        emit(have && sp.ok, sp.r);  // external reflection
+        for (uint32_t hit = 0; hit < C.max_hits; ++hit) {
+            if (__ballot(alive) == 0) break;
+            bool out = false;
+            if (alive) {
+                const float pw = ray.j.power();
+                if (pw == 0.0f) {
+                    status = kExhausted;
+                    alive = false;
+                } else if (pw <= C.tail) {
+                    status = kTail;
+                    alive = false;
+                } else {
                     const float b = dot(p, ray.dir);
-                    float tx;
+                    float tx = 0.0f;
+                    bool found = true;
                     if (C.shape.sphere) {
                         const float cc = dot(p, p) - 1.0f;
                         float disc = fmaf(b, b, -cc);
@@ -295,34 +405,45 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
                         if (disc < 0.0f) disc = 0.0f;
                         const float tb = -b + sqrtf(disc);
                         const float ts = 0x1p-16f;
-                        if (!(ts < tb) || !(surface(C.shape, axpy(ray.dir, ts, p)) < 0.0f)) {
-                            status = kUnresolved;
-                            break;
+                        if (!(ts < tb) || !(surface(C.shape, axpy(ray.dir, ts, p)) < 0.0f))
+                            found = false;
+                        else
+                            tx = newton(C.shape, p, ray.dir, ts, tb, false, -2.0f * b);
+                    }
+                    if (!found) {
+                        status = kUnresolved;
+                        alive = false;
+                    } else {
+                        const f3 q = axpy(ray.dir, tx, p);
+                        const f3 nout = outward_normal(C.shape, q);
+                        sp = split(ray, nout * -1.0f, C.n_int, C.n_ext);
+                        if (!sp.ok) {
+                            status = kNumeric;
+                            alive = false;
+                        } else {
+                            ++hits;
+                            out = sp.has_t;
+                            ray = sp.r;
+                            p = q;
                         }
-                        tx = newton(C.shape, p, ray.dir, ts, tb, false, -2.0f * b);
                     }
-                    const f3 q = axpy(ray.dir, tx, p);
-                    const f3 nout = outward_normal(C.shape, q);
-                    sp = split(ray, nout * -1.0f, C.n_int, C.n_ext);
-                    if (!sp.ok) {
-                        status = kNumeric;
-                        break;
-                    }
-                    ++hits;
-                    if (sp.has_t) emit(sp.t);
-                    ray = sp.r;
-                    p = q;
                 }
-                unresolved = ray.j.power();
-                if (status == kLimit && unresolved == 0.0f) status = kExhausted;
             }
+            emit(out, sp.t);  // transmitted branch of this encounter
         }
-        c.escaped += rq;
-        c.unresolved += quantize((double)unresolved, R.qbits);
-        c.hits += hits;
-        count_status(c, status);
+        if (inner) {
+            unresolved = ray.j.power();
+            if (status == kLimit && unresolved == 0.0f) status = kExhausted;
+        }
+        if (have) {
+            c.escaped += rq;
+            c.unresolved += quantize((double)unresolved, R.qbits);
+            c.hits += hits;
+            count_status(c, status);
+        }
         rq = 0;
     }
+    if (tail != head) drain(tail - head);  // the tile's remaining deferred emissions
 }
 
 // Persistent tile loop of one record kind. LDS_EDGES: the record's u edges are
@@ -334,6 +455,8 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_cached;
     __shared__ uint32_t s_next;
+    // per wave: the ring of deferred drop emissions (256 threads = 4 waves)
+    __shared__ Deferred s_ring[KIND == kKindDrop ? 4 * kRing : 1];
     if (threadIdx.x == 0) s_cached = 0xffffffffu;
     for (;;) {
         __syncthreads();
@@ -370,25 +493,25 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
             if constexpr (KIND == kKindHex)
                 trace_tile<KIND, AREA>(R, first, count, [&](int k) { return s_edges[k]; }, c);
             else
-                drop_tile(R, first, count, &s_next, [&](int k) { return s_edges[k]; }, c);
+                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [&](int k) { return s_edges[k]; }, c);
         } else {
             const double* e = R.edges;
             if constexpr (KIND == kKindHex)
                 trace_tile<KIND, AREA>(R, first, count, [&](int k) { return e[k]; }, c);
             else
-                drop_tile(R, first, count, &s_next, [&](int k) { return e[k]; }, c);
+                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [&](int k) { return e[k]; }, c);
         }
         flush_counters(c, R.stats);
     }
 }
 
-// Waves per SIMD the drop kernel is compiled for. It is latency bound (serial
-// Newton chains): uncapped it needs 201 VGPRs and runs 2 waves per SIMD without
-// spills; capped for 3 waves (168 VGPRs, 100 B/lane of spills) it is fastest
-// (4.31 G rays/s on 24 records of the oblate workload, against 4.26 at 4 waves
-// and 3.65 at 2).
+// Waves per SIMD the drop kernel is compiled for. Uncapped it runs 2 waves per
+// SIMD; the serial Newton chains and the cell atomics' latency want more waves
+// than the register file holds without spills. Measured on 48 records of the
+// oblate workload: 2 waves 668 ms, 3 waves (168 VGPRs, 204 B/lane of spills)
+// 540 ms, 4 waves (128 VGPRs, 420 B/lane) 508 ms.
 #ifndef ICEPHASE_DROP_WAVES
-#define ICEPHASE_DROP_WAVES 3
+#define ICEPHASE_DROP_WAVES 4
 #endif
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_DROP_WAVES, 8)))
     icephase_trace_drop(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)

@@ -23,7 +23,7 @@ P = C.POINTER
 
 SHAPE_SPHERE, SHAPE_DROP, SHAPE_HEX = 0, 1, 2
 REC_DIFFRACTION = 1
-ABI_VERSION = 2
+ABI_VERSION = 3
 RUN_TRACE, RUN_CDF, RUN_GRAPH, RUN_NO_AUDIT = 1, 2, 4, 8
 
 STATUS_NAMES = ("exhausted", "tail_tolerance_reached", "interaction_limit", "ambiguous_boundary",
@@ -53,7 +53,8 @@ class HistStats(C.Structure):
     _fields_ = [("q_forward", u64), ("q_backward", u64), ("samples", u64), ("invalid", u64), ("snapped", u64),
                 ("status", u64 * 9), ("q_escaped", u64), ("q_unresolved", u64), ("internal_hits", u64),
                 ("rays", u64), ("qbits", u32), ("pad", u32), ("ray_weight_mm2", f64),
-                ("projected_area_mm2", f64), ("q_diffraction", u64), ("diffraction_captured", f64)]
+                ("projected_area_mm2", f64), ("q_diffraction", u64), ("diffraction_captured", f64),
+                ("cell_atomics", u64)]
 
 
 class WaterState(C.Structure):

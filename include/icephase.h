@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define IPH_ABI_VERSION 2
+#define IPH_ABI_VERSION 3
 
 enum iph_status {
     IPH_OK = 0,
@@ -107,6 +107,9 @@ typedef struct iph_hist_stats {
     uint64_t q_diffraction;          /* quanta of the Fraunhofer lobe (IPH_REC_DIFFRACTION) */
     double diffraction_captured;     /* lobe power inside the computed rows / projected area,
                                         before the lobe is scaled to carry exactly the area */
+    uint64_t cell_atomics;           /* u64 atomicAdds the trace kernels issued into histogram
+                                        cells (drop records merge equal cells of adjacent lanes,
+                                        so this is at most the number of binned samples) */
 } iph_hist_stats;
 
 typedef struct iph_cdf_policy {

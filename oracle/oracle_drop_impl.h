@@ -134,13 +134,11 @@ static REAL FN(newton_)(const FN(rec_)* R, V o, V d, REAL a, REAL b, int a_pos, 
         const REAL scale = FABS(x) > (REAL)1 ? FABS(x) : (REAL)1;
         if (b - a <= tol * scale) return x;
         const REAL cand = x - fx / slope;
-        if (cand > a && cand < b) {
-            const REAL step = FABS(cand - x);
-            x = cand;
-            if (step <= tol * scale) return x;
-        } else {
-            x = FMA(b - a, (REAL)0.5, a);
-        }
+        const int inside = cand > a && cand < b;
+        /* a step below the tolerance ends the solve even when rounding left a
+         * bracket end on the root and the candidate is not strictly inside */
+        if (FABS(cand - x) <= tol * scale) return inside ? cand : x;
+        x = inside ? cand : FMA(b - a, (REAL)0.5, a);
     }
     return x;
 }

@@ -33,7 +33,7 @@ def test_header_symbols_are_exported():
 
 def test_abi_version_and_struct_sizes():
     lib = ip.lib()
-    assert lib.iph_abi_version() == 2
+    assert lib.iph_abi_version() == 3
     import ctypes
     assert lib.iph_struct_size(0) == ctypes.sizeof(icephase.Record) == 264
     assert lib.iph_struct_size(1) == ctypes.sizeof(icephase.HistStats)
