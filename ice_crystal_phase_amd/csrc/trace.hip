@@ -234,8 +234,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
                 ++c.samples;
                 const d3 d = D3((double)r.dir.x, (double)r.dir.y, (double)r.dir.z);
                 if (finite64(d)) {
-                    const CellHit h =
-                        bin_fast(R.nt, R.np, dot(R.frame.e0, d), dot(R.frame.e1, d), dot(R.frame.ki, d));
+                    const CellHit h = bin_fast_approx(R.nt, R.np, R.frame, d);
                     if (h.kind == 0) {
                         cell = h.index;
                         sum = q;
