@@ -76,7 +76,7 @@ BASE_COEFFS = (-0.0458, -0.0335, -0.1211, -0.0227)
 # fused multiply-add counts as two operations (as the FP32 peak counts it); the
 # compiler fuses nothing else (FMA contraction off).
 CENSUS = {
-    "drop": {"miss": 74.8, "entry": 617.5, "hit": 565.4, "emit": 46.5},
+    "drop": {"miss": 74.8, "entry": 574.6, "hit": 523.7, "emit": 46.5},
     "sphere": {"miss": 12.0, "entry": 269.0, "hit": 273.0, "emit": 46.5},
 }
 PEAK_TFLOPS_FP32 = 157.3   # MI355X published FP32 vector peak

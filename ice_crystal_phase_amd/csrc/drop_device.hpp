@@ -138,9 +138,15 @@ __device__ inline float surface_slope(const Shape& s, f3 p, f3 d, float& slope)
 // can leave a bracket end on the root, and a candidate on or past it must not
 // send the solve back to bisecting the rest of the bracket (that cost up to 20
 // iterations on a few percent of exits, and a wave waits for its slowest lane).
+// Two Newton steps in a row that contract quadratically predict the next one:
+// about K step^2 with K = step / prev^2. When that prediction is below an
+// eighth of the tolerance the candidate is accepted without the confirming
+// evaluation (on the headline oblate drops this saves about one iteration per
+// solve; directions stay as close to an FP64 trace as with the confirmation).
 __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, bool a_pos, float x)
 {
     const float tol = 0x1p-21f;
+    float prev = 0.0f;  // length of the previous Newton step, 0 after a bisection
     if (!(x > a && x < b)) x = fmaf(b - a, 0.5f, a);
     for (int it = 0; it < 48; ++it) {
         const f3 p = axpy(d, x, o);
@@ -155,7 +161,12 @@ __device__ inline float newton(const Shape& s, f3 o, f3 d, float a, float b, boo
         if (b - a <= tol * scale) return x;
         const float cand = x - fx / slope;
         const bool inside = cand > a && cand < b;
-        if (fabsf(cand - x) <= tol * scale) return inside ? cand : x;
+        const float step = fabsf(cand - x);
+        if (step <= tol * scale) return inside ? cand : x;
+        if (inside && prev > 0.0f && step <= prev * 0.25f &&
+            step * step * step <= 0.125f * tol * scale * prev * prev)
+            return cand;
+        prev = inside ? step : 0.0f;
         x = inside ? cand : fmaf(b - a, 0.5f, a);
     }
     return x;

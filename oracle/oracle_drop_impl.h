@@ -124,6 +124,7 @@ static inline REAL FN(surf_slope_)(const FN(rec_)* R, V p, V d, REAL* slope)
 static REAL FN(newton_)(const FN(rec_)* R, V o, V d, REAL a, REAL b, int a_pos, REAL x)
 {
     const REAL tol = TOL;
+    REAL prev = (REAL)0; /* length of the previous Newton step, 0 after a bisection */
     if (!(x > a && x < b)) x = FMA(b - a, (REAL)0.5, a);
     for (int it = 0; it < 48; ++it) {
         const V p = FN(axpy_)(d, x, o);
@@ -135,9 +136,16 @@ static REAL FN(newton_)(const FN(rec_)* R, V o, V d, REAL a, REAL b, int a_pos, 
         if (b - a <= tol * scale) return x;
         const REAL cand = x - fx / slope;
         const int inside = cand > a && cand < b;
+        const REAL step = FABS(cand - x);
         /* a step below the tolerance ends the solve even when rounding left a
          * bracket end on the root and the candidate is not strictly inside */
-        if (FABS(cand - x) <= tol * scale) return inside ? cand : x;
+        if (step <= tol * scale) return inside ? cand : x;
+        /* two quadratically contracting steps predict the next, K step^2 with
+         * K = step / prev^2: accept the candidate when that is below tol / 8 */
+        if (inside && prev > (REAL)0 && step <= prev * (REAL)0.25 &&
+            step * step * step <= (REAL)0.125 * tol * scale * prev * prev)
+            return cand;
+        prev = inside ? step : (REAL)0;
         x = inside ? cand : FMA(b - a, (REAL)0.5, a);
     }
     return x;
