@@ -7,8 +7,8 @@
 //   linear emission chain, audit  hex_trace/include/ice_crystal/hex_trace_core.hpp:80-170
 //   counter RNG                   hex_trace/include/ice_crystal/math.hpp:136-146
 // The chain is fused with binning: every emission goes straight to the
-// histogram through a sink functor instead of a CSR sample buffer, so each ray
-// is traced once (the reference traces it twice, count then replay).
+// histogram instead of a CSR sample buffer, so each ray is traced once (the
+// reference traces it twice, count then replay).
 #pragma once
 
 #include "phase_bin.hpp"
@@ -358,107 +358,135 @@ struct RayResult {
     uint32_t status, internal_hits, tir, outputs;
 };
 
-// Trace one ray from a given entry point on a lit face (the deterministic
-// splitting chain of hex_trace/include/ice_crystal/hex_trace_core.hpp:102-162:
+// One ray's splitting chain (hex_trace/include/ice_crystal/hex_trace_core.hpp:102-162:
 // emit the exterior reflection, then the transmission at every internal
-// encounter, continuing the internal reflection); sink(dir, power) is called
-// for every escaping branch.
-template <class Sink>
-__device__ inline RayResult trace_from(const Config& c, const Entry& e, Sink& sink)
+// encounter, continuing the internal reflection), advanced one interface per
+// call so that a wave can run its rays' chains in lockstep and bin their
+// emissions together. Each call leaves the emission of its interface, if any,
+// in (dir, pw) with `sink` set; the operations applied to a ray, and their
+// order, are those of the reference chain.
+struct Chain {
+    Ray ray;  // the internally reflected ray being followed
+    d3 p;     // its point on face `cur`
+    unsigned cur;
+    RayResult out;
+    bool alive;  // more encounters to trace
+    bool sink;   // (dir, pw) holds an emission of the last call
+    d3 dir;
+    double pw;
+};
+
+// The escaping branch r of an interface: a non-finite, negative, or zero power
+// with a nonzero field ends the ray (numeric); a zero power emits nothing.
+__device__ inline bool take_emission(Chain& ch, const Ray& r)
 {
-    RayResult out{0, 1, 0, kInvalid, 0, 0, 0};
-    if (!e.ok || e.face >= 8) return out;
-    if (!c.prism.on_face(e.pos, e.face)) {
-        out.status = kAmbiguous;
-        return out;
+    const double pw = r.j.power();
+    if (!finite64(pw) || pw < 0 || (!pw && r.j.nonzero())) {
+        ch.out.status = kNumeric;
+        ch.alive = false;
+        return false;
     }
-    if (!(dot(Prism::normal(e.face), c.k) < 0)) return out;
-    Ray inc{c.k, c.f, Jones{cplx{1, 0}, cplx{0, 0}, cplx{0, 0}, cplx{1, 0}}};
-    Split sp = split(inc, Prism::normal(e.face), c.ni, c.nt);
-    if (!sp.ok) {
-        out.status = kNumeric;
-        return out;
+    if (pw > 0) {
+        ch.sink = true;
+        ch.dir = r.dir;
+        ch.pw = pw;
+        ch.out.escaped += pw;
+        ++ch.out.outputs;
     }
-    out.max_balance = fabs(sp.balance);
-    auto emit = [&](const Ray& r) -> uint32_t {
-        const double pw = r.j.power();
-        if (!finite64(pw) || pw < 0 || (!pw && r.j.nonzero())) return kNumeric;
-        if (pw == 0) return kExhausted;
-        sink(r.dir, pw);
-        out.escaped += pw;
-        ++out.outputs;
-        return kExhausted;
-    };
-    uint32_t em = emit(sp.r);
-    if (em != kExhausted) {
-        out.status = em;
-        return out;
-    }
-    if (!sp.has_t) {
-        out.tir = 1;
-        out.unresolved = 0;
-        out.status = kExhausted;
-        return out;
-    }
-    Ray ray = sp.t;
-    d3 p = e.pos;
-    unsigned cur = e.face;
-    for (uint32_t enc = 0; enc < c.max_hits; ++enc) {
-        const double power = ray.j.power();
-        out.unresolved = power;
-        if (!finite64(power) || power < 0 || (!power && ray.j.nonzero())) {
-            out.status = kNumeric;
-            return out;
-        }
-        if (power == 0) {
-            out.status = kExhausted;
-            return out;
-        }
-        if (power <= c.tail) {
-            out.status = kTail;
-            return out;
-        }
-        const Exit hit = next_face(c.prism, p, ray.dir, cur);
-        if (hit.status != 0) {
-            out.status = hit.status == 2 ? kAmbiguous : kNumeric;
-            return out;
-        }
-        sp = split(ray, -Prism::normal(hit.face), c.nt, c.ni);
-        if (!sp.ok) {
-            out.status = kNumeric;
-            return out;
-        }
-        ++out.internal_hits;
-        if (sp.tir) ++out.tir;
-        out.max_balance = gt_max(out.max_balance, fabs(sp.balance));
-        if (sp.has_t) {
-            em = emit(sp.t);
-            if (em != kExhausted) {
-                out.status = em;
-                return out;
-            }
-        }
-        ray = sp.r;
-        p = hit.pos;
-        cur = hit.face;
-    }
-    const double rem = ray.j.power();
-    out.unresolved = rem;
-    if (!finite64(rem) || rem < 0 || (!rem && ray.j.nonzero())) {
-        out.status = kNumeric;
-        return out;
-    }
-    out.status = rem == 0 ? kExhausted : (rem <= c.tail ? kTail : kLimit);
-    return out;
+    return true;
 }
 
-// Trace sample `id`: entry point drawn from the counter RNG over the lit faces.
-template <class Sink>
-__device__ inline RayResult trace(const Config& c, uint64_t id, Sink& sink)
+// Entry at a point on a lit face: the exterior reflection is this call's emission.
+__device__ inline void enter(Chain& ch, const Config& c, const Entry& e)
 {
-    const Entry e = pick_entry(c.prism, c.k, c.area, open_uniform(c.seed, id, 0), open_uniform(c.seed, id, 1),
-                               open_uniform(c.seed, id, 2));
-    return trace_from(c, e, sink);
+    ch.out = RayResult{0, 1, 0, kInvalid, 0, 0, 0};
+    ch.alive = false;
+    ch.sink = false;
+    if (!e.ok || e.face >= 8) return;
+    if (!c.prism.on_face(e.pos, e.face)) {
+        ch.out.status = kAmbiguous;
+        return;
+    }
+    if (!(dot(Prism::normal(e.face), c.k) < 0)) return;
+    const Ray inc{c.k, c.f, Jones{cplx{1, 0}, cplx{0, 0}, cplx{0, 0}, cplx{1, 0}}};
+    const Split sp = split(inc, Prism::normal(e.face), c.ni, c.nt);
+    if (!sp.ok) {
+        ch.out.status = kNumeric;
+        return;
+    }
+    ch.out.max_balance = fabs(sp.balance);
+    if (!take_emission(ch, sp.r)) return;
+    if (!sp.has_t) {
+        ch.out.tir = 1;
+        ch.out.unresolved = 0;
+        ch.out.status = kExhausted;
+        return;
+    }
+    ch.ray = sp.t;
+    ch.p = e.pos;
+    ch.cur = e.face;
+    ch.alive = true;
+}
+
+// One internal encounter of a live chain: its transmission is this call's emission.
+__device__ inline void encounter(Chain& ch, const Config& c)
+{
+    ch.sink = false;
+    const double power = ch.ray.j.power();
+    ch.out.unresolved = power;
+    if (!finite64(power) || power < 0 || (!power && ch.ray.j.nonzero())) {
+        ch.out.status = kNumeric;
+        ch.alive = false;
+        return;
+    }
+    if (power == 0) {
+        ch.out.status = kExhausted;
+        ch.alive = false;
+        return;
+    }
+    if (power <= c.tail) {
+        ch.out.status = kTail;
+        ch.alive = false;
+        return;
+    }
+    const Exit hit = next_face(c.prism, ch.p, ch.ray.dir, ch.cur);
+    if (hit.status != 0) {
+        ch.out.status = hit.status == 2 ? kAmbiguous : kNumeric;
+        ch.alive = false;
+        return;
+    }
+    const Split sp = split(ch.ray, -Prism::normal(hit.face), c.nt, c.ni);
+    if (!sp.ok) {
+        ch.out.status = kNumeric;
+        ch.alive = false;
+        return;
+    }
+    ++ch.out.internal_hits;
+    if (sp.tir) ++ch.out.tir;
+    ch.out.max_balance = gt_max(ch.out.max_balance, fabs(sp.balance));
+    if (sp.has_t && !take_emission(ch, sp.t)) return;
+    ch.ray = sp.r;
+    ch.p = hit.pos;
+    ch.cur = hit.face;
+}
+
+// A chain still live after max_hits encounters: the remainder's status.
+__device__ inline void finish(Chain& ch, const Config& c)
+{
+    const double rem = ch.ray.j.power();
+    ch.out.unresolved = rem;
+    if (!finite64(rem) || rem < 0 || (!rem && ch.ray.j.nonzero()))
+        ch.out.status = kNumeric;
+    else
+        ch.out.status = rem == 0 ? kExhausted : (rem <= c.tail ? kTail : kLimit);
+    ch.alive = false;
+}
+
+// Entry point of sample `id`, drawn from the counter RNG over the lit faces.
+__device__ inline Entry sample_entry(const Config& c, uint64_t id)
+{
+    return pick_entry(c.prism, c.k, c.area, open_uniform(c.seed, id, 0), open_uniform(c.seed, id, 1),
+                      open_uniform(c.seed, id, 2));
 }
 
 }  // namespace hex

@@ -115,35 +115,6 @@ struct CellTable {
     }
 };
 
-template <class EdgeLoad>
-__device__ inline void deposit(const DevRecord& R, d3 dir, double power, EdgeLoad edge, const CellTable& table,
-                               LaneCounters& c, unsigned long long& ray_quanta)
-{
-    const unsigned long long q = quantize(power * R.qfactor, R.qbits);
-    ray_quanta += q;
-    ++c.samples;
-    if (R.rotated) {
-        // body -> ensemble (hex_phase/include/ice_crystal/phase_math.hpp:10, same
-        // operation order: column images summed left to right)
-        const double* m = R.rot;
-        dir = D3((m[0] * dir.x + m[1] * dir.y) + m[2] * dir.z, (m[3] * dir.x + m[4] * dir.y) + m[5] * dir.z,
-                 (m[6] * dir.x + m[7] * dir.y) + m[8] * dir.z);
-    }
-    const CellHit h = bin_direction(R.frame, R.nt, R.np, dir, edge);
-    if (h.kind == 2) {
-        ++c.invalid;
-        return;
-    }
-    c.snapped += h.snapped;
-    if (h.kind == 1) {
-        c.fwd += q;
-    } else if (h.kind == -1) {
-        c.bwd += q;
-    } else if (q) {
-        table.add(R.bins, h.index, q, c);
-    }
-}
-
 // Largest of v over the wave, merged into a non-negative FP64 counter by its
 // bit pattern (non-negative doubles order like their u64 bits).
 __device__ inline void wave_max_into(double v, unsigned long long* slot)
@@ -153,46 +124,162 @@ __device__ inline void wave_max_into(double v, unsigned long long* slot)
     if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
 }
 
-// Hex tile of an area-quadrature record: ray i enters at R.area[i] and carries
-// the weight qf relative to the record's quantum.
+// A hex emission whose cell the FP32 decision leaves open: its FP64 direction
+// (after the node rotation) and quantum, binned later by the FP64 rules.
+struct HexDeferred {
+    double x, y, z;
+    unsigned long long q;
+};
+constexpr uint32_t kHexRing = 256;  // deferred hex emissions per wave (a power of two, at least 2 x 64)
+
+// phase_bin's FP64 rules for a deferred hex emission (its unit checks already
+// passed). Not inlined: the hex tile calls it between rounds of the splitting
+// chain, whose live registers the FP64 binning would otherwise add to.
 template <class EdgeLoad>
-__device__ inline void area_tile(const DevRecord& R, uint64_t first, uint64_t count, EdgeLoad edge,
-                                 const CellTable& table, LaneCounters& c)
+__device__ __attribute__((noinline)) CellHit bin_deferred_hex(const BinFrame& g, uint32_t nt, uint32_t np, double dx,
+                                                              double dy, double dz, EdgeLoad edge)
 {
-    double ray_balance = 0, face_balance = 0;
-    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
-        const AreaRay a = R.area[first + i];
-        const hex::Entry e{a.pos, a.face, true};
-        unsigned long long rq = 0;
-        auto sink = [&](d3 dir, double pw) { deposit(R, dir, pw * a.qf, edge, table, c, rq); };
-        const hex::RayResult r = hex::trace_from(R.hex, e, sink);
-        c.escaped += rq;
-        c.unresolved += quantize(r.unresolved * a.qf * R.qfactor, R.qbits);
-        c.hits += r.internal_hits;
-        count_status(c, r.status);
-        ray_balance = gt_max(ray_balance, fabs(r.escaped + r.unresolved - 1));
-        face_balance = gt_max(face_balance, r.max_balance);
-    }
-    wave_max_into(ray_balance, &R.stats[kStatRayBalance]);
-    wave_max_into(face_balance, &R.stats[kStatFaceBalance]);
+    const d3 d = D3(dx, dy, dz);
+    const double x = dot(g.e0, d), y = dot(g.e1, d), z = dot(g.ki, d);
+    const double nrm = sqrt(x * x + y * y + z * z);
+    return bin_exact(nt, np, x, y, z, nrm, edge);
 }
 
-template <int KIND, bool AREA, class EdgeLoad>
-__device__ inline void trace_tile(const DevRecord& R, uint64_t first, uint64_t count, EdgeLoad edge,
-                                  const CellTable& table, LaneCounters& c)
+// Cell bookkeeping of a binned hex emission: pole quanta and audit counters in
+// lane registers, cell quanta into the workgroup's cell table.
+__device__ inline void record_hex(const DevRecord& R, const CellHit& h, unsigned long long q, const CellTable& table,
+                                  LaneCounters& c)
 {
-    if constexpr (KIND == kKindHex && AREA) {
-        area_tile(R, first, count, edge, table, c);
-    } else if constexpr (KIND == kKindHex) {
-        for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
-            unsigned long long rq = 0;
-            auto sink = [&](d3 dir, double pw) { deposit(R, dir, pw, edge, table, c, rq); };
-            const hex::RayResult r = hex::trace(R.hex, first + i, sink);
-            c.escaped += rq;
-            c.unresolved += quantize(r.unresolved * R.qfactor, R.qbits);
-            c.hits += r.internal_hits;
-            count_status(c, r.status);
+    if (h.kind == 2) {
+        ++c.invalid;
+        return;
+    }
+    c.snapped += h.snapped;
+    if (h.kind == 1)
+        c.fwd += q;
+    else if (h.kind == -1)
+        c.bwd += q;
+    else if (q)
+        table.add(R.bins, h.index, q, c);
+}
+
+// Hex tile. Each lane traces whole rays; the wave runs its 64 rays' chains in
+// lockstep, one interface per round (hex::enter, then hex::encounter up to
+// max_hits times), so that every round's emissions are binned where the whole
+// wave runs. phase_bin of an emission (phase_math.hpp:37-81 after the node
+// rotation, phase_math.hpp:10): the unit checks on the spot, then the FP32
+// decision (bin_fast_approx), which bins ~96 % of them; the rest go to the
+// wave's LDS ring and are binned by the FP64 rules 64 at a time, between rays
+// (inside a ray's rounds only when the ring could not take another round).
+// The FP64 binning then runs with every lane busy and outside the splitting
+// chain, whose registers it would otherwise add to. Integer quanta make the histogram
+// independent of when an emission is binned. AREA: ray i enters at R.area[i]
+// and carries the weight qf relative to the record's quantum (atlas nodes).
+template <bool AREA, class EdgeLoad>
+__device__ inline void hex_tile(const DevRecord& R, uint64_t first, uint64_t count, HexDeferred* ring, EdgeLoad edge,
+                                const CellTable& table, LaneCounters& c)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    double ray_balance = 0, face_balance = 0;
+    unsigned long long rq = 0;
+    // ring positions: changed only where every lane of the wave runs
+    uint32_t head = 0, tail = 0;
+    auto drain = [&](uint32_t n) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < n) {
+            const HexDeferred e = ring[(head + lane) & (kHexRing - 1)];
+            record_hex(R, bin_deferred_hex(R.frame, R.nt, R.np, e.x, e.y, e.z, edge), e.q, table, c);
         }
+        head += n;
+    };
+    // One emission per lane where `sink`; called by the whole wave.
+    auto emit = [&](bool sink, d3 d, double power, double qf) {
+        bool defer = false;
+        unsigned long long q = 0;
+        if (sink) {
+            if constexpr (AREA)
+                q = quantize(power * qf * R.qfactor, R.qbits);
+            else
+                q = quantize(power * R.qfactor, R.qbits);
+            rq += q;
+            ++c.samples;
+            if (R.rotated) {
+                // body -> ensemble (hex_phase/include/ice_crystal/phase_math.hpp:10, same
+                // operation order: column images summed left to right)
+                const double* m = R.rot;
+                d = D3((m[0] * d.x + m[1] * d.y) + m[2] * d.z, (m[3] * d.x + m[4] * d.y) + m[5] * d.z,
+                       (m[6] * d.x + m[7] * d.y) + m[8] * d.z);
+            }
+            bool valid = is_unit64(d);
+            if (valid) {
+                const double x = dot(R.frame.e0, d), y = dot(R.frame.e1, d), z = dot(R.frame.ki, d);
+                const double nrm = sqrt(x * x + y * y + z * z);
+                valid = nrm > 0 && finite64(nrm) && fabs(nrm - 1) <= 512 * DBL_EPSILON;
+            }
+            if (!valid) {
+                ++c.invalid;
+            } else {
+                const CellHit h = bin_fast_approx(R.nt, R.np, R.frame, d);
+                if (h.kind == 0) {
+                    if (q) table.add(R.bins, h.index, q, c);
+                } else {
+                    defer = true;
+                }
+            }
+        }
+        const unsigned long long m = __ballot(defer);
+        if (m) {
+            if (defer) ring[(tail + (uint32_t)__popcll(m & below)) & (kHexRing - 1)] = HexDeferred{d.x, d.y, d.z, q};
+            tail += (uint32_t)__popcll(m);
+            if (tail - head > kHexRing - 64) drain(64);  // so that the next round fits
+        }
+    };
+    for (uint64_t base = 0; base < count; base += blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        const bool have = i < count;
+        hex::Chain ch;
+        ch.alive = false;
+        ch.sink = false;
+        double qf = 1.0;
+        rq = 0;
+        if (have) {
+            hex::Entry e;
+            if constexpr (AREA) {
+                const AreaRay a = R.area[first + i];
+                e = hex::Entry{a.pos, a.face, true};
+                qf = a.qf;
+            } else {
+                e = hex::sample_entry(R.hex, first + i);
+            }
+            hex::enter(ch, R.hex, e);
+        }
+        emit(ch.sink, ch.dir, ch.pw, qf);  // exterior reflection
+        for (uint32_t k = 0; k < R.hex.max_hits; ++k) {
+            if (__ballot(ch.alive) == 0) break;
+            ch.sink = false;
+            if (ch.alive) hex::encounter(ch, R.hex);
+            emit(ch.sink, ch.dir, ch.pw, qf);  // transmission of this encounter
+        }
+        if (ch.alive) hex::finish(ch, R.hex);
+        while (tail - head >= 64) drain(64);
+        if (have) {
+            c.escaped += rq;
+            if constexpr (AREA) {
+                c.unresolved += quantize(ch.out.unresolved * qf * R.qfactor, R.qbits);
+                ray_balance = gt_max(ray_balance, fabs(ch.out.escaped + ch.out.unresolved - 1));
+                face_balance = gt_max(face_balance, ch.out.max_balance);
+            } else {
+                c.unresolved += quantize(ch.out.unresolved * R.qfactor, R.qbits);
+            }
+            c.hits += ch.out.internal_hits;
+            count_status(c, ch.out.status);
+        }
+    }
+    if (tail != head) drain(tail - head);  // the tile's remaining deferred emissions
+    if constexpr (AREA) {
+        wave_max_into(ray_balance, &R.stats[kStatRayBalance]);
+        wave_max_into(face_balance, &R.stats[kStatFaceBalance]);
     }
 }
 
@@ -516,6 +603,8 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
     // hex: the workgroup's cell table (flushed into the record's histogram after every tile)
     __shared__ uint32_t s_keys[KIND == kKindHex ? kCellSlots : 1];
     __shared__ unsigned long long s_sums[KIND == kKindHex ? kCellSlots : 1];
+    // hex: per wave, the ring of deferred emissions (256 threads = 4 waves)
+    __shared__ HexDeferred s_hring[KIND == kKindHex ? 4 * kHexRing : 1];
     const CellTable table{s_keys, s_sums};
     if constexpr (KIND == kKindHex) table.clear();
     if (threadIdx.x == 0) s_cached = 0xffffffffu;
@@ -552,15 +641,15 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
                 __syncthreads();
             }
             if constexpr (KIND == kKindHex)
-                trace_tile<KIND, AREA>(R, first, count, [&](int k) { return s_edges[k]; }, table, c);
+                hex_tile<AREA>(R, first, count, s_hring + (threadIdx.x >> 6) * kHexRing, [&](int k) { return s_edges[k]; }, table, c);
             else
                 drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [&](int k) { return s_edges[k]; }, c);
         } else {
             const double* e = R.edges;
             if constexpr (KIND == kKindHex)
-                trace_tile<KIND, AREA>(R, first, count, [&](int k) { return e[k]; }, table, c);
+                hex_tile<AREA>(R, first, count, s_hring + (threadIdx.x >> 6) * kHexRing, [e](int k) { return e[k]; }, table, c);
             else
-                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [&](int k) { return e[k]; }, c);
+                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [e](int k) { return e[k]; }, c);
         }
         if constexpr (KIND == kKindHex) {
             __syncthreads();  // every lane's quanta are in the table
@@ -578,23 +667,29 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
 #ifndef ICEPHASE_DROP_WAVES
 #define ICEPHASE_DROP_WAVES 4
 #endif
+// Waves per SIMD the hex kernels are compiled for: two, with the splitting
+// chain's registers (the incoming ray and both outgoing rays, 17 FP64 values
+// each) and a few spills.
+#ifndef ICEPHASE_HEX_WAVES
+#define ICEPHASE_HEX_WAVES 2
+#endif
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_DROP_WAVES, 8)))
     icephase_trace_drop(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindDrop, true>(recs, map, queue);
 }
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_HEX_WAVES, 8)))
     icephase_trace_hex(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindHex, true>(recs, map, queue);
 }
 // hex records whose rays enter at the points of an area quadrature (atlas nodes)
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_HEX_WAVES, 8)))
     icephase_trace_hex_area(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindHex, true, true>(recs, map, queue);
 }
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_HEX_WAVES, 8)))
     icephase_trace_hex_area_wide(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindHex, false, true>(recs, map, queue);
@@ -604,7 +699,7 @@ extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_pe
 {
     trace_loop<kKindDrop, false>(recs, map, queue);
 }
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_HEX_WAVES, 8)))
     icephase_trace_hex_wide(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     trace_loop<kKindHex, false>(recs, map, queue);
