@@ -26,7 +26,9 @@
 //   sur_forward     fused 4-layer forward per 128-row tile (activations in LDS),
 //                   logits, and the pre-activations Z_l when training;
 //   sur_lse_*       log-sum-exp per record (fixed-order two-level reduction);
-//   sur_dlogits     dL/df = p - P and the loss (over the folded rows);
+//   sur_dlogits     dL/df = p - P (+ the weighted total-variation term, whose
+//                   record sum sur_tv_partial forms first) and the loss (over
+//                   the folded rows);
 //   sur_probs/fold  folded rows <-> full-grid cells;
 //   sur_backward    fused data gradients layers 4 -> 1 per tile (dZ_l to HBM);
 //   sur_wgrad       weight gradients dW_l = dZ_l^T H_{l-1}, split over rows,
@@ -322,11 +324,44 @@ __device__ inline float log_mult(const Static& st, uint32_t q)
     return st.canon[q] == st.mirr[q] ? 0.0f : 0.69314718f;
 }
 
-// dL/df = p - P over the folded rows (record 0); loss partials sum P (lse - f
-// + log mult), which is the full-grid cross entropy -sum P_cell log p_cell.
+// Half the sign of p - P: the subgradient of the total variation
+// 1/2 sum |p - P| with respect to p.
+__device__ inline float tv_sign(float p, float P) { return p > P ? 0.5f : (p < P ? -0.5f : 0.0f); }
+
+// Total-variation term of the loss, first pass: per block, the sum over folded
+// rows of g p with g = tv_sign(p, P); the record's sum G enters every row's
+// gradient p (g - G). A folded row stands for a mirror pair of cells whose
+// targets are equal (the tracer's records are mirror symmetric), so its p, P
+// and g are the pair's, and these sums are the full grid's.
+__global__ void __launch_bounds__(256) sur_tv_partial(Static st, const float* __restrict__ logits,
+                                                      const float* __restrict__ lse, const float* __restrict__ target,
+                                                      float* __restrict__ part)
+{
+    __shared__ float sh[256];
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    float v = 0;
+    if (r < st.Mpad) {
+        const float f = logits[r];
+        const float P = r < st.Mh ? target[r] : 0.0f;
+        const float p = f > -INFINITY ? __expf(f - lse[0]) : 0.0f;
+        v = tv_sign(p, P) * p;
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// dL/df over the folded rows (record 0) of L = CE + tvw TV: p - P, plus
+// tvw p (g - G) when tvw > 0 (G from sur_tv_partial, summed). Loss partials
+// sum P (lse - f + log mult), the full-grid cross entropy -sum P_cell log p_cell.
 __global__ void __launch_bounds__(256) sur_dlogits(Static st, const float* __restrict__ logits,
                                                    const float* __restrict__ lse, const float* __restrict__ target,
-                                                   float* __restrict__ dF, float* __restrict__ loss_part)
+                                                   float tvw, const float* __restrict__ tvG, float* __restrict__ dF,
+                                                   float* __restrict__ loss_part)
 {
     __shared__ float sh[256];
     const uint32_t r = blockIdx.x * 256 + threadIdx.x;
@@ -335,7 +370,9 @@ __global__ void __launch_bounds__(256) sur_dlogits(Static st, const float* __res
         const float f = logits[r];
         const float P = r < st.Mh ? target[r] : 0.0f;
         const float p = f > -INFINITY ? __expf(f - lse[0]) : 0.0f;
-        dF[r] = p - P;
+        float d = p - P;
+        if (tvw > 0.0f) d = fmaf(tvw * p, tv_sign(p, P) - tvG[0], d);
+        dF[r] = d;
         if (P > 0) l = P * ((lse[0] - f) + log_mult(st, r % st.nq));
     }
     sh[threadIdx.x] = l;
@@ -872,11 +909,16 @@ hipError_t lse(const Static& st, const float* logits, uint32_t nrec, float2* par
     return hipGetLastError();
 }
 
-hipError_t dlogits(const Static& st, const float* logits, const float* lse_v, const float* target, float* dF,
-                   float* loss_part, float* loss, hipStream_t s)
+hipError_t dlogits(const Static& st, const float* logits, const float* lse_v, const float* target, float tvw,
+                   float* tv_part, float* tv_G, float* dF, float* loss_part, float* loss, hipStream_t s)
 {
     const uint32_t blocks = (st.Mpad + 255) / 256;
-    hipLaunchKernelGGL(sur_dlogits, dim3(blocks), dim3(256), 0, s, st, logits, lse_v, target, dF, loss_part);
+    if (tvw > 0.0f) {
+        hipLaunchKernelGGL(sur_tv_partial, dim3(blocks), dim3(256), 0, s, st, logits, lse_v, target, tv_part);
+        hipLaunchKernelGGL(sur_sum, dim3(1), dim3(256), 0, s, tv_part, blocks, tv_G);
+    }
+    hipLaunchKernelGGL(sur_dlogits, dim3(blocks), dim3(256), 0, s, st, logits, lse_v, target, tvw, tv_G, dF,
+                       loss_part);
     hipLaunchKernelGGL(sur_sum, dim3(1), dim3(256), 0, s, loss_part, blocks, loss);
     return hipGetLastError();
 }

@@ -80,8 +80,8 @@ hipError_t forward(const Net& net, const Static& st, const float* d_conds, uint3
                    __bf16* Z, hipStream_t s);
 hipError_t lse(const Static& st, const float* logits, uint32_t nrec, float2* part, float* out, hipStream_t s);
 // target: folded-grid probabilities [Mh]; the loss is the full-grid cross entropy
-hipError_t dlogits(const Static& st, const float* logits, const float* lse, const float* target, float* dF,
-                   float* loss_part, float* loss, hipStream_t s);
+hipError_t dlogits(const Static& st, const float* logits, const float* lse, const float* target, float tvw,
+                   float* tv_part, float* tv_G, float* dF, float* loss_part, float* loss, hipStream_t s);
 // full-grid cell probabilities [nrec][M] (FP64 and/or FP32) from folded logits
 hipError_t probs(const Static& st, const float* logits, const float* lse, uint32_t nrec, double* mass, float* p,
                  hipStream_t s);

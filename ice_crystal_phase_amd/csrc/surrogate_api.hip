@@ -70,6 +70,8 @@ struct iph_sur {
     DevBuf<uint32_t> canon, mirr;
     DevBuf<float> pphi, ptheta, feat, tfeat, logw, logits, dF, lse, loss_part, loss, conds, part, part2, part_w5, part_b5, part_db;
     DevBuf<float2> lse_part;
+    DevBuf<float> tv_part, tv_G;  // total-variation term: block sums of g p, their record sum
+    float tv_weight = 0.0f;       // loss = cross entropy + tv_weight * total variation
     DevBuf<double> mass;
     // training set: probabilities [dataset][M], normalized conditions [dataset][4]
     DevBuf<float> data, data_conds;
@@ -245,6 +247,8 @@ int iph_sur_create(iph_ctx* ctx, const iph_sur_config* cfg, iph_sur** out)
     IPH_HIP(ctx, s->lse.alloc(cfg->max_records));
     IPH_HIP(ctx, s->lse_part.alloc((size_t)cfg->max_records * kLseParts));
     IPH_HIP(ctx, s->loss_part.alloc((st.Mpad + 255) / 256));
+    IPH_HIP(ctx, s->tv_part.alloc((st.Mpad + 255) / 256));
+    IPH_HIP(ctx, s->tv_G.alloc(1));
     IPH_HIP(ctx, s->loss.alloc(std::max<uint32_t>(cfg->max_records, 1024)));
     IPH_HIP(ctx, s->conds.alloc((size_t)4 * std::max<uint32_t>(cfg->max_records, 1024)));
     IPH_HIP(ctx, s->part.alloc((size_t)st.splits * 256 * (64 + 3 * 256)));
@@ -324,6 +328,10 @@ int iph_sur_set_params(iph_sur* s, const float* packed)
         if (!std::isfinite(v)) return sur_fail(s, "Parameters must be finite.");
     IPH_HIP(ctx, hipMemcpy(s->params.p, dev.data(), kParamsPad * 4, hipMemcpyHostToDevice));
     IPH_HIP(ctx, pack(s->params.p, s->w.p, s->wt.p, s->st, s->pphi.p, s->ptheta.p, ctx->stream));
+    // new parameters start a new optimization: Adam moments and step count from zero
+    IPH_HIP(ctx, hipMemsetAsync(s->adam_m.p, 0, kParamsPad * 4, ctx->stream));
+    IPH_HIP(ctx, hipMemsetAsync(s->adam_v.p, 0, kParamsPad * 4, ctx->stream));
+    s->step = 0;
     IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return IPH_OK;
 }
@@ -343,7 +351,8 @@ static int record_gradient(iph_sur* s, uint32_t slot, uint32_t k, bool accumulat
     const float* d_target = s->data.p + (size_t)slot * st.Mh;
     IPH_HIP(ctx, forward(n, st, cond, 1, true, s->logits.p, s->Z.p, q));
     IPH_HIP(ctx, lse(st, s->logits.p, 1, s->lse_part.p, s->lse.p, q));
-    IPH_HIP(ctx, dlogits(st, s->logits.p, s->lse.p, d_target, s->dF.p, s->loss_part.p, s->loss.p + k, q));
+    IPH_HIP(ctx, dlogits(st, s->logits.p, s->lse.p, d_target, s->tv_weight, s->tv_part.p, s->tv_G.p, s->dF.p,
+                         s->loss_part.p, s->loss.p + k, q));
     IPH_HIP(ctx, backward(n, st, s->dF.p, s->Z.p, s->dZ.p, s->part_w5.p, s->part_b5.p, s->part_db.p, accumulate, q));
     const size_t zl = (size_t)st.Mpad * 256;
     for (int l = 3; l >= 0; --l)
@@ -406,6 +415,15 @@ int iph_sur_set_target(iph_sur* s, uint32_t slot, const float* host_probs, const
     IPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     IPH_HIP(ctx, hipMemcpy(s->data_conds.p + 4 * (size_t)slot, c, sizeof c, hipMemcpyHostToDevice));
     s->filled[slot] = 1;
+    return IPH_OK;
+}
+
+int iph_sur_set_loss(iph_sur* s, float tv_weight)
+{
+    if (!s) return sur_fail(s, "Invalid arguments.");
+    if (!(std::isfinite(tv_weight) && tv_weight >= 0))
+        return sur_fail(s, "The total-variation weight must be finite and non-negative.");
+    s->tv_weight = tv_weight;
     return IPH_OK;
 }
 

@@ -198,6 +198,7 @@ def lib() -> C.CDLL:
     L.iph_sur_set_params.argtypes = [C.c_void_p, C.c_void_p]
     L.iph_sur_set_record.argtypes = [C.c_void_p, u32, C.c_void_p, u32, P(C.c_float)]
     L.iph_sur_set_target.argtypes = [C.c_void_p, u32, C.c_void_p, P(C.c_float)]
+    L.iph_sur_set_loss.argtypes = [C.c_void_p, C.c_float]
     L.iph_sur_gradient.argtypes = [C.c_void_p, u32, P(C.c_float), C.c_void_p]
     L.iph_sur_train_step.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_float, C.c_void_p]
     L.iph_sur_predict.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_void_p]
@@ -266,7 +267,7 @@ EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_des
             "iph_build_cdf", "iph_build_cdf_noalloc", "iph_atomic_probe", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
             "iph_drop_projected_area", "iph_hex_projected_area", "iph_batch_probabilities", "iph_sur_create",
             "iph_sur_destroy", "iph_sur_param_count", "iph_sur_get_params", "iph_sur_set_params",
-            "iph_sur_set_record", "iph_sur_set_target", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",
+            "iph_sur_set_record", "iph_sur_set_target", "iph_sur_set_loss", "iph_sur_gradient", "iph_sur_train_step", "iph_sur_predict",
             "iph_sur_compare", "iph_sur_records", "iph_sur_timing", "iph_ice_index", "iph_atlas_node_rays",
             "iph_atlas_trace", "iph_write_npy", "iph_atlas_mass_create", "iph_atlas_mass_destroy",
             "iph_atlas_compose", "iph_coherent_create", "iph_coherent_destroy", "iph_coherent_trace",

@@ -113,6 +113,10 @@ class Surrogate:
         c = (C.c_float * 4)(*[float(x) for x in cond])
         icp._check(icp.lib().iph_sur_set_target(self._p, slot, probs.ctypes.data, c), self.ctx.ptr)
 
+    def set_loss(self, tv_weight: float) -> None:
+        """Train on cross entropy + tv_weight * total variation (0: cross entropy alone)."""
+        icp._check(icp.lib().iph_sur_set_loss(self._p, float(tv_weight)), self.ctx.ptr)
+
     def gradient(self, slot: int):
         """Loss and packed gradient of one training slot."""
         loss = C.c_float()

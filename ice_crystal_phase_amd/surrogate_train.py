@@ -40,6 +40,8 @@ def main(argv=None) -> int:
     ap.add_argument("--eval-grid", type=int, default=8192)
     ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
     ap.add_argument("--out", default=None, help="write the trained parameters (.npy)")
+    ap.add_argument("--tv-weight", type=float, default=0.0,
+                    help="train on cross entropy + this weight x total variation (0: cross entropy alone)")
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--gate", type=int, default=0,
                     help="validation records (seed + 2) for the served domain; 0 = no gate")
@@ -61,7 +63,9 @@ def main(argv=None) -> int:
                                             seed=a.seed) as sur:
         if a.init:
             sur.set_params(np.load(a.init, allow_pickle=False))
-        out = {"records": a.records, "grid": a.grid, "cells": [a.theta, a.phi]}
+        if a.tv_weight:
+            sur.set_loss(a.tv_weight)
+        out = {"records": a.records, "grid": a.grid, "cells": [a.theta, a.phi], "tv_weight": a.tv_weight}
         if train:
             t0 = time.time()
             trace_ms = S.fill_training_set(sur, conds, a.grid, log=log)

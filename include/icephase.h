@@ -271,14 +271,22 @@ void iph_sur_destroy(iph_sur* s);
  * w5 [256], b5 [1] (row-major [out][in]). */
 uint32_t iph_sur_param_count(void);
 int iph_sur_get_params(iph_sur* s, float* packed);
+/* Replaces the parameters and starts a new optimization (Adam moments and step
+ * count reset). */
 int iph_sur_set_params(iph_sur* s, const float* packed);
 /* Training-set slot from a traced record of a batch of the same context, or
  * from host probabilities (theta_count*phi_count, summing to one). cond: alpha
  * deg, wavelength nm, radius mm, aspect. */
 int iph_sur_set_record(iph_sur* s, uint32_t slot, iph_batch* b, uint32_t rec, const float cond[4]);
 int iph_sur_set_target(iph_sur* s, uint32_t slot, const float* host_probs, const float cond[4]);
+/* Training loss of the gradients below: cross entropy + tv_weight * total
+ * variation 1/2 sum |p - P| (default 0: cross entropy alone; the records the
+ * network learns are mirror symmetric, which the folded total variation
+ * assumes). No reference counterpart (the reference has no learned part). */
+int iph_sur_set_loss(iph_sur* s, float tv_weight);
 /* Cross-entropy loss of one slot and its parameter gradient (packed, host;
- * may be NULL). */
+ * may be NULL); with a total-variation weight the gradient is that of the
+ * weighted sum, the loss stays the cross entropy. */
 int iph_sur_gradient(iph_sur* s, uint32_t slot, float* loss, float* packed_grad);
 /* One Adam step (beta 0.9/0.999) on the mean gradient of n slots; loss[n]
  * receives their losses (may be NULL). */

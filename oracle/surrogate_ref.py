@@ -7,8 +7,9 @@ learning belongs to another project), so this restates the network the
 library defines (ice_crystal_phase_amd/csrc/surrogate.hip): inputs
 [t, cos(2 pi k (v - 1/4)) (k=1..16), sin(pi f t), cos(pi f t) (f in
 THETA_FREQ), c0..c3], four SiLU layers of
-256, one log density per cell, softmax over the cells of f + log du_i, and the
-cross-entropy loss -sum P log p. Pinned by central finite differences
+256, one log density per cell, softmax over the cells of f + log du_i, the
+cross-entropy loss -sum P log p and the training objective's optional
+total-variation term. Pinned by central finite differences
 (tests/test_surrogate_oracle.py), which fix every term of the gradient.
 """
 from __future__ import annotations
@@ -92,8 +93,12 @@ def probs(p, nt, np_, cond) -> np.ndarray:
     return e / e.sum()
 
 
-def loss_and_grad(p, nt, np_, cond, target):
-    """Cross entropy -sum P log softmax(f) and its gradient (packed layout)."""
+def loss_and_grad(p, nt, np_, cond, target, tv_weight: float = 0.0):
+    """Cross entropy -sum P log softmax(f) and the gradient (packed layout) of
+    the training objective cross entropy + tv_weight * total variation
+    1/2 sum |q - P|, q = softmax(f): dF = q - P + tv_weight q (g - G) with
+    g = sign(q - P) / 2 and G = sum g q (the subgradient of the total variation
+    through the softmax)."""
     X = features(nt, np_, cond)
     lw = log_du(nt, np_)
     f, zs, hs = forward(p, X, lw)
@@ -101,7 +106,11 @@ def loss_and_grad(p, nt, np_, cond, target):
     lse = m + np.log(np.exp(f - m).sum())
     P = np.asarray(target, np.float64).reshape(-1)
     loss = float(np.sum(P[P > 0] * (lse - f[P > 0])))
-    dF = np.exp(f - lse) - P
+    q = np.exp(f - lse)
+    dF = q - P
+    if tv_weight:
+        g = 0.5 * np.sign(q - P)
+        dF = dF + tv_weight * q * (g - np.sum(g * q))
     W, b, w5, b5 = unpack(p)
     gW, gb = [None] * 4, [None] * 4
     gw5 = hs[4].T @ dF
@@ -118,3 +127,11 @@ def loss_and_grad(p, nt, np_, cond, target):
         out += [gW[l].reshape(-1), gb[l]]
     out += [gw5, np.array([gb5])]
     return loss, np.concatenate(out)
+
+
+def objective(p, nt, np_, cond, target, tv_weight: float = 0.0) -> float:
+    """Cross entropy + tv_weight * total variation of the network's cell
+    probabilities against the target (the value loss_and_grad differentiates)."""
+    loss, _ = loss_and_grad(p, nt, np_, cond, target)
+    q = probs(p, nt, np_, cond)
+    return loss + tv_weight * 0.5 * float(np.abs(q - np.asarray(target, np.float64).reshape(-1)).sum())

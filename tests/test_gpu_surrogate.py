@@ -71,6 +71,50 @@ def test_gradient_matches_fp64_oracle(ctx):
         assert abs(q.sum() - 1) < 1e-5
 
 
+def test_tv_gradient_matches_fp64_oracle(ctx):
+    """Gradient of cross entropy + 2 x total variation against the FP64 oracle on
+    a mirror-symmetric target (the folded rows' total variation is the full
+    grid's for such targets, as for every traced record); the default loss is
+    restored by a zero weight, bit for bit."""
+    print("case: surrogate TV gradient 45x90", flush=True)
+    nt, np_ = 45, 90
+    rng = np.random.default_rng(12)
+    tgt = (rng.random((nt, np_)) ** 4)
+    mirror = (np_ // 2 - 1 - np.arange(np_)) % np_
+    tgt = 0.5 * (tgt + tgt[:, mirror])
+    tgt = (tgt / tgt.sum()).astype(np.float32).reshape(-1)
+    cond = (-20.0, 610.0, 1.3, 0.9)
+    with S.Surrogate(ctx, nt, np_, max_records=2, dataset=1, seed=3) as sur:
+        p = sur.params().astype(np.float64)
+        sur.set_target(0, tgt, cond)
+        _, g_ce = sur.gradient(0)
+        sur.set_loss(2.0)
+        loss, g = sur.gradient(0)
+        l_ref, g_ref = ref.loss_and_grad(p, nt, np_, np.float32(cond).astype(np.float64), tgt.astype(np.float64),
+                                         tv_weight=2.0)
+        _, g_ref0 = ref.loss_and_grad(p, nt, np_, np.float32(cond).astype(np.float64), tgt.astype(np.float64))
+        assert abs(loss / l_ref - 1) < 2e-3, (loss, l_ref)
+        assert np.linalg.norm(g_ref - g_ref0) > 0.1 * np.linalg.norm(g_ref0)  # the term matters here
+        for name, a, b in _segments():
+            if name == "b5":
+                assert abs(g_ref[a]) < 1e-6 and abs(g[a]) < 1e-4
+                continue
+            x, y = g[a:b].astype(np.float64), g_ref[a:b]
+            rel = np.linalg.norm(x - y) / np.linalg.norm(y)
+            cos = float(x @ y / (np.linalg.norm(x) * np.linalg.norm(y)))
+            print(f"{name}: rel {rel:.2e} cos {cos:.6f}", flush=True)
+            tol, ctol = (2e-2, 0.9998) if name.startswith("W") or name == "w5" else (1e-1, 0.995)
+            assert rel < tol and cos > ctol, name
+        total = np.linalg.norm(g.astype(np.float64) - g_ref) / np.linalg.norm(g_ref)
+        print(f"whole gradient: rel {total:.2e}", flush=True)
+        assert total < 1e-2
+        sur.set_loss(0.0)
+        _, g0 = sur.gradient(0)
+        assert np.array_equal(g0, g_ce)
+        with pytest.raises(ip.IcePhaseError):
+            sur.set_loss(-1.0)
+
+
 def test_training_set_from_traced_record(ctx, oracle):
     """A traced record copied into a training slot holds its normalized cell
     probabilities: the gradient of that slot equals the gradient of the same

@@ -71,6 +71,52 @@ def test_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
+def test_tv_gradient_matches_finite_differences():
+    """The gradient of cross entropy + w * total variation: central differences
+    of the objective (the total variation is piecewise linear in q, smooth away
+    from q = P, which a random target keeps far from every cell)."""
+    nt, np_ = 6, 8
+    rng = np.random.default_rng(9)
+    target = rng.random(nt * np_) ** 3
+    target /= target.sum()
+    cond = (25.0, 640.0, 0.7, 1.0)
+    p = _params(3)
+    q = ref.probs(p, nt, np_, cond)
+    assert np.min(np.abs(q - target)) > 1e-5
+    for w in (1.0, 7.5):
+        _, g = ref.loss_and_grad(p, nt, np_, cond, target, tv_weight=w)
+        _, g0 = ref.loss_and_grad(p, nt, np_, cond, target)
+        assert np.linalg.norm(g - g0) > 1e-3 * np.linalg.norm(g0)  # the term contributes
+        n1 = 256 * ref.NIN
+        idx = np.concatenate([rng.choice(n1, 8, replace=False), n1 + 256 + rng.choice(65536 + 256, 8, replace=False),
+                              ref.param_count() - 257 + rng.choice(257, 6, replace=False)])
+        h = 1e-6
+        for i in idx:
+            pp, pm = p.copy(), p.copy()
+            pp[i] += h
+            pm[i] -= h
+            fd = (ref.objective(pp, nt, np_, cond, target, w) - ref.objective(pm, nt, np_, cond, target, w)) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (w, i, fd, g[i])
+
+
+def test_tv_gradient_is_shift_invariant():
+    """Closed form: the softmax is invariant to a common shift of the logits, so
+    the objective's derivative along the output bias is sum_k dF_k = 1 - sum P
+    (= 0 for a normalized target) with or without the total-variation term,
+    whose part sum_k q_k (g_k - G) = G - G vanishes only if G is subtracted."""
+    nt, np_ = 5, 6
+    rng = np.random.default_rng(4)
+    target = rng.random(nt * np_)
+    target /= target.sum()
+    p = _params(4)
+    cond = (0.0, 550.0, 1.0, 1.0)
+    _, g = ref.loss_and_grad(p, nt, np_, cond, target, tv_weight=3.0)
+    assert abs(g[-1]) < 1e-12
+    q = ref.probs(p, nt, np_, cond)
+    sg = 0.5 * np.sign(q - target)
+    assert abs(np.sum(q * sg)) > 1e-3  # the subtracted G is not trivially zero here
+
+
 def test_probabilities_normalized_and_loss_is_cross_entropy():
     nt, np_ = 5, 7
     p = _params(2)
