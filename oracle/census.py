@@ -60,7 +60,7 @@ def main(argv=None) -> int:
     import ice_crystal_phase_amd as ip
     recs, _, _ = bench.workload(192, 26, 900, 1800)
     n = ip.water_optics(550.0).refractive_index
-    sphere = ip.sphere_record(n, 2048, 900, 1800, radius_mm=1.0, inclination_deg=0.0, max_internal_hits=8)
+    sphere = ip.sphere_record(n, 2048, 900, 1800, radius_mm=1.0, inclination_deg=0.0, max_internal_hits=9)
     out = {"drop": census_of(recs, 900, 1800, args.rows, args.every),
            "sphere": census_of([sphere], 900, 1800, max(args.rows, 16), 1)}
     print(json.dumps(out, indent=2))

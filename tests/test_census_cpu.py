@@ -39,7 +39,7 @@ def test_census_constants_are_reproduced(workload):
 def test_sphere_census_is_exact(workload):
     import ice_crystal_phase_amd as ip
     n = ip.water_optics(550.0).refractive_index
-    rec = ip.sphere_record(n, 2048, 900, 1800, radius_mm=1.0, inclination_deg=0.0, max_internal_hits=8)
+    rec = ip.sphere_record(n, 2048, 900, 1800, radius_mm=1.0, inclination_deg=0.0, max_internal_hits=9)
     got = census.census_of([rec], 900, 1800, rows=4)
     want = bench.CENSUS["sphere"]
     # closed-form entry and exit: the same operations for every ray and encounter
