@@ -198,6 +198,10 @@ def run_hex(ctx, rays_log2: int = 28):
         tr, cdf, _ = b.timing()
         st = b.stats(0)
     return {"rays": 1 << rays_log2, "rays_per_s": (1 << rays_log2) / (tr / 1e3), "trace_ms": tr, "cdf_ms": cdf,
+            # every emission adds into the workgroup's LDS cell table; the tables reach the
+            # histogram by flush atomics (plus any cell that found no slot)
+            "emissions": int(st.samples), "lds_cell_adds_per_s": st.samples / (tr / 1e3),
+            "hbm_cell_atomics": int(st.cell_atomics),
             "unresolved_fraction": st.q_unresolved / float(st.q_escaped + st.q_unresolved),
             "status_counts": list(map(int, st.status[:8])), "diffraction_captured": st.diffraction_captured,
             "config": "hex column n=1.31 a=50um c/a=2 ki=(1,-0.4,0.3), 2^%d rays, <=12 interactions, "
