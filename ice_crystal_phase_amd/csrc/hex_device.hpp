@@ -189,13 +189,20 @@ struct Exit {
     int status;  // 0 regular, 2 ambiguous, 3 invalid
 };
 
+// CHAIN: (p, start, d) come from a link of the chain that already passed these
+// input checks on the same values (the entry point's on_face in hex::enter or
+// the previous next_face's status 0; the direction's unit check inside the
+// split that produced it), so their outcome is known and they are skipped.
+template <bool CHAIN = false>
 __device__ inline Exit next_face(const Prism& h, d3 p, d3 d, unsigned start)
 {
     Exit hit{D3(0, 0, 0), 0, 8, 3};
-    if (!is_unit64(d) || start >= 8) return hit;
-    if (!h.on_face(p, start)) {
-        hit.status = 2;
-        return hit;
+    if constexpr (!CHAIN) {
+        if (!is_unit64(d) || start >= 8) return hit;
+        if (!h.on_face(p, start)) {
+            hit.status = 2;
+            return hit;
+        }
     }
     if (dot(Prism::normal(start), d) > 128 * DBL_EPSILON) return hit;
     double best = DBL_MAX, second = DBL_MAX;
@@ -260,7 +267,12 @@ __device__ inline Jones rows(Jones j, cplx s, cplx p)
     return j;
 }
 
-// n points into the incident medium; flux-normalized transmission.
+// n points into the incident medium; flux-normalized transmission. CHAIN: the
+// incoming ray is the output of an earlier split, whose success required
+// basis_ok and finite Jones entries of these same values, and n is a face
+// normal (unit, checked once per record for all eight faces), so those input
+// checks are skipped; the refractive indices are still checked.
+template <bool CHAIN = false>
 __device__ inline Split split(const Ray& in, d3 n, double ni, double nt)
 {
     Split b;
@@ -268,9 +280,10 @@ __device__ inline Split split(const Ray& in, d3 n, double ni, double nt)
     b.has_t = false;
     b.tir = false;
     b.balance = 0;
-    if (!basis_ok(in.f, in.dir) || !is_unit64(n) || !in.j.finite() ||
-        !(ni > 0 && nt > 0 && finite64(ni) && finite64(nt)))
-        return b;
+    if constexpr (!CHAIN) {
+        if (!basis_ok(in.f, in.dir) || !is_unit64(n) || !in.j.finite()) return b;
+    }
+    if (!(ni > 0 && nt > 0 && finite64(ni) && finite64(nt))) return b;
     double ci = -dot(in.dir, n);
     if (ci < 0 || ci > 1 + 128 * DBL_EPSILON) return b;
     if (ci > 1) ci = 1;
@@ -449,13 +462,13 @@ __device__ inline void encounter(Chain& ch, const Config& c)
         ch.alive = false;
         return;
     }
-    const Exit hit = next_face(c.prism, ch.p, ch.ray.dir, ch.cur);
+    const Exit hit = next_face<true>(c.prism, ch.p, ch.ray.dir, ch.cur);
     if (hit.status != 0) {
         ch.out.status = hit.status == 2 ? kAmbiguous : kNumeric;
         ch.alive = false;
         return;
     }
-    const Split sp = split(ch.ray, -Prism::normal(hit.face), c.nt, c.ni);
+    const Split sp = split<true>(ch.ray, -Prism::normal(hit.face), c.nt, c.ni);
     if (!sp.ok) {
         ch.out.status = kNumeric;
         ch.alive = false;

@@ -266,6 +266,13 @@ bool prepare_record(const iph_record& in, uint64_t tile_rays, DevRecord& R, doub
             err = "Invalid hex prism geometry.";
             return false;
         }
+        // the splitting chain skips the unit check of the (negated) face normals:
+        // they hold for all eight faces, which this confirms with the same operations
+        for (unsigned f = 0; f < 8; ++f)
+            if (!is_unit64(hex::Prism::normal(f)) || !is_unit64(-hex::Prism::normal(f))) {
+                err = "Internal error: a prism face normal fails its unit check.";
+                return false;
+            }
         d3 k_ens;
         if (!unit_of(k_in, k_ens)) {
             err = "Incident direction must be finite and nonzero.";
