@@ -197,7 +197,17 @@ def run_hex(ctx, rays_log2: int = 28):
         b.run(flags)
         tr, cdf, _ = b.timing()
         st = b.stats(0)
+        # the same record with every quantum added straight to HBM (no workgroup cell
+        # table): the emissions of a prism at one orientation fall in a few dozen cells,
+        # whose serialized atomics that kernel waits on; the histogram is the same bits
+        ref = b.bins(0).copy()
+        b.set_cell_table(False)
+        b.run(ip.RUN_TRACE | ip.RUN_GRAPH)
+        tr_direct = b.timing()[0]
+        if not np.array_equal(b.bins(0), ref):
+            raise RuntimeError("hex record: the direct-atomics histogram differs from the cell-table one")
     return {"rays": 1 << rays_log2, "rays_per_s": (1 << rays_log2) / (tr / 1e3), "trace_ms": tr, "cdf_ms": cdf,
+            "direct_atomics_trace_ms": tr_direct,
             # every emission adds into the workgroup's LDS cell table; the tables reach the
             # histogram by flush atomics (plus any cell that found no slot)
             "emissions": int(st.samples), "lds_cell_adds_per_s": st.samples / (tr / 1e3),

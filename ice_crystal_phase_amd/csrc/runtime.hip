@@ -20,7 +20,7 @@
 namespace icephase {
 hipError_t launch_zero(const DevRecord* d_recs, uint32_t nrec, uint32_t* d_queue, hipStream_t s);
 hipError_t launch_trace(int kind, bool wide, const DevRecord* d_recs, TileMap map, uint32_t* d_queue,
-                        uint32_t blocks, uint32_t threads, hipStream_t s);
+                        uint32_t blocks, uint32_t threads, hipStream_t s, bool direct);
 hipError_t launch_bin_dirs(const BinFrame& g, uint32_t nt, uint32_t np, const double* d_edges, uint64_t n,
                            const double* d_dirs, uint32_t* d_index, int32_t* d_kind, hipStream_t s);
 hipError_t launch_cdf(const CdfJob* d_jobs, uint32_t njobs, uint32_t max_onp, bool any_coarse, uint32_t reduce_blocks,
@@ -207,6 +207,7 @@ struct iph_batch {
     bool any_coarse = false;
     uint32_t blocks[2] = {0, 0};
     uint64_t tile_rays = 0;
+    bool cell_table = true;  // hex kernels: workgroup LDS cell table (false: every quantum straight to HBM)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // second stream (default priority) for the hex kernel of a mixed batch, with
     // the fork / join events that order it after the zero kernel and before the CDF
@@ -589,6 +590,20 @@ int iph_batch_set_schedule(iph_batch* b, uint64_t tile_rays, uint32_t blocks)
     return IPH_OK;
 }
 
+int iph_batch_set_cell_table(iph_batch* b, uint32_t enable)
+{
+    if (!b) return fail(nullptr, IPH_E_INVALID, "batch must not be NULL.");
+    if (enable > 1) return fail(b->ctx, IPH_E_INVALID, "cell table switch must be 0 or 1.");
+    b->cell_table = enable != 0;
+    // the captured graphs hold the kernel of the previous choice
+    for (auto& g : b->graphs)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    return IPH_OK;
+}
+
 static int enqueue(iph_batch* b, uint32_t flags, bool timed)
 {
     iph_ctx* ctx = b->ctx;
@@ -604,15 +619,16 @@ static int enqueue(iph_batch* b, uint32_t flags, bool timed)
             IPH_HIP(ctx, hipEventRecord(b->fork, s));
             IPH_HIP(ctx, hipStreamWaitEvent(b->side, b->fork, 0));
             IPH_HIP(ctx, launch_trace(1, b->wide[1], b->d_recs.p, b->map[1], b->d_queue.p + 1, b->blocks[1], 256,
-                                      b->side));
-            IPH_HIP(ctx, launch_trace(0, b->wide[0], b->d_recs.p, b->map[0], b->d_queue.p, b->blocks[0], 256, s));
+                                      b->side, !b->cell_table));
+            IPH_HIP(ctx, launch_trace(0, b->wide[0], b->d_recs.p, b->map[0], b->d_queue.p, b->blocks[0], 256, s,
+                                      false));
             IPH_HIP(ctx, hipEventRecord(b->join, b->side));
             IPH_HIP(ctx, hipStreamWaitEvent(s, b->join, 0));
         } else {
             for (int k = 0; k < 2; ++k)
                 if (b->map[k].total)
                     IPH_HIP(ctx, launch_trace(k, b->wide[k], b->d_recs.p, b->map[k], b->d_queue.p + k, b->blocks[k],
-                                              256, s));
+                                              256, s, k == 1 && !b->cell_table));
         }
         for (uint32_t r : b->lobe_recs)
             IPH_HIP(ctx, launch_lobe(b->d_recs.p, r, b->host[r].lobe.rows, b->host[r].np, s));

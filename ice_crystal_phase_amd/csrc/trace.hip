@@ -67,16 +67,24 @@ __device__ inline void count_status(LaneCounters& c, uint32_t s)
 // open-addressing table (key = cell + 1, 0 = empty; slots claimed by atomicCAS,
 // sums by LDS u64 atomicAdd) and adds the table into the record's histogram
 // once per tile. A cell that finds no slot within kCellProbes goes straight to
-// HBM. Integer sums are associative, so the histogram is the same bits.
+// HBM. Integer sums are associative, so the histogram is the same bits either
+// way. The DIRECT instantiation adds every quantum straight to HBM (the
+// measured alternative, and the overflow path for every cell).
 constexpr uint32_t kCellSlots = 2048;  // a power of two
 constexpr uint32_t kCellProbes = 16;
 
+template <bool DIRECT = false>
 struct CellTable {
     uint32_t* keys;
     unsigned long long* sums;
 
     __device__ void add(unsigned long long* bins, uint32_t cell, unsigned long long q, LaneCounters& c) const
     {
+        if constexpr (DIRECT) {
+            atomicAdd(&bins[cell], q);
+            ++c.atomics;
+            return;
+        }
         const uint32_t key = cell + 1u;
         uint32_t h = (cell * 2654435761u) >> 21;  // top 11 bits: kCellSlots slots
         for (uint32_t p = 0; p < kCellProbes; ++p) {
@@ -147,7 +155,8 @@ __device__ __attribute__((noinline)) CellHit bin_deferred_hex(const BinFrame& g,
 
 // Cell bookkeeping of a binned hex emission: pole quanta and audit counters in
 // lane registers, cell quanta into the workgroup's cell table.
-__device__ inline void record_hex(const DevRecord& R, const CellHit& h, unsigned long long q, const CellTable& table,
+template <class Table>
+__device__ inline void record_hex(const DevRecord& R, const CellHit& h, unsigned long long q, const Table& table,
                                   LaneCounters& c)
 {
     if (h.kind == 2) {
@@ -175,9 +184,9 @@ __device__ inline void record_hex(const DevRecord& R, const CellHit& h, unsigned
 // chain, whose registers it would otherwise add to. Integer quanta make the histogram
 // independent of when an emission is binned. AREA: ray i enters at R.area[i]
 // and carries the weight qf relative to the record's quantum (atlas nodes).
-template <bool AREA, class EdgeLoad>
+template <bool AREA, class EdgeLoad, class Table>
 __device__ inline void hex_tile(const DevRecord& R, uint64_t first, uint64_t count, HexDeferred* ring, EdgeLoad edge,
-                                const CellTable& table, LaneCounters& c)
+                                const Table& table, LaneCounters& c)
 {
     const uint32_t lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -591,7 +600,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
 
 // Persistent tile loop of one record kind. LDS_EDGES: the record's u edges are
 // staged in LDS (nt < kLdsEdges); otherwise they are read from global memory.
-template <int KIND, bool LDS_EDGES, bool AREA = false>
+template <int KIND, bool LDS_EDGES, bool AREA = false, bool DIRECT = false>
 __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
 {
     __shared__ double s_edges[LDS_EDGES ? kLdsEdges : 1];
@@ -605,8 +614,8 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
     __shared__ unsigned long long s_sums[KIND == kKindHex ? kCellSlots : 1];
     // hex: per wave, the ring of deferred emissions (256 threads = 4 waves)
     __shared__ HexDeferred s_hring[KIND == kKindHex ? 4 * kHexRing : 1];
-    const CellTable table{s_keys, s_sums};
-    if constexpr (KIND == kKindHex) table.clear();
+    const CellTable<DIRECT> table{s_keys, s_sums};
+    if constexpr (KIND == kKindHex && !DIRECT) table.clear();
     if (threadIdx.x == 0) s_cached = 0xffffffffu;
     for (;;) {
         __syncthreads();
@@ -651,7 +660,7 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
             else
                 drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [e](int k) { return e[k]; }, c);
         }
-        if constexpr (KIND == kKindHex) {
+        if constexpr (KIND == kKindHex && !DIRECT) {
             __syncthreads();  // every lane's quanta are in the table
             table.flush(R.bins, c);
         }
@@ -704,6 +713,17 @@ extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_pe
 {
     trace_loop<kKindHex, false>(recs, map, queue);
 }
+// the hex kernels with every quantum added straight to HBM (no cell table)
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_HEX_WAVES, 8)))
+    icephase_trace_hex_direct(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
+{
+    trace_loop<kKindHex, true, false, true>(recs, map, queue);
+}
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ICEPHASE_HEX_WAVES, 8)))
+    icephase_trace_hex_wide_direct(const DevRecord* __restrict__ recs, TileMap map, uint32_t* __restrict__ queue)
+{
+    trace_loop<kKindHex, false, false, true>(recs, map, queue);
+}
 
 // Bin caller directions (C-ABI iph_bin_directions).
 extern "C" __global__ void __launch_bounds__(256)
@@ -745,11 +765,17 @@ hipError_t launch_zero(const DevRecord* d_recs, uint32_t nrec, uint32_t* d_queue
     return hipGetLastError();
 }
 
-// kind: kKindDrop or kKindHex; wide: edges too large for LDS.
+// kind: kKindDrop or kKindHex; wide: edges too large for LDS; direct: hex
+// quanta straight to HBM (no workgroup cell table).
 hipError_t launch_trace(int kind, bool wide, const DevRecord* d_recs, TileMap map, uint32_t* d_queue, uint32_t blocks,
-                        uint32_t threads, hipStream_t s)
+                        uint32_t threads, hipStream_t s, bool direct)
 {
-    if (kind == kKindHex) {
+    if (kind == kKindHex && direct) {
+        if (wide)
+            hipLaunchKernelGGL(icephase_trace_hex_wide_direct, dim3(blocks), dim3(threads), 0, s, d_recs, map, d_queue);
+        else
+            hipLaunchKernelGGL(icephase_trace_hex_direct, dim3(blocks), dim3(threads), 0, s, d_recs, map, d_queue);
+    } else if (kind == kKindHex) {
         if (wide)
             hipLaunchKernelGGL(icephase_trace_hex_wide, dim3(blocks), dim3(threads), 0, s, d_recs, map, d_queue);
         else

@@ -174,6 +174,7 @@ def lib() -> C.CDLL:
     L.iph_batch_destroy.restype = None
     L.iph_batch_run.argtypes = [C.c_void_p, u32]
     L.iph_batch_set_schedule.argtypes = [C.c_void_p, u64, u32]
+    L.iph_batch_set_cell_table.argtypes = [C.c_void_p, u32]
     L.iph_batch_timing.argtypes = [C.c_void_p, P(f64)]
     L.iph_batch_stats.argtypes = [C.c_void_p, u32, P(HistStats)]
     L.iph_batch_bins.argtypes = [C.c_void_p, u32, C.c_void_p]
@@ -263,7 +264,7 @@ def _live(handle: C.c_void_p, what: str) -> C.c_void_p:
 
 EXPORTED = ("iph_abi_version", "iph_struct_size", "iph_ctx_create", "iph_ctx_destroy", "iph_last_error",
             "iph_u_edges", "iph_bin_directions", "iph_batch_create", "iph_batch_destroy", "iph_batch_run",
-            "iph_batch_set_schedule", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
+            "iph_batch_set_schedule", "iph_batch_set_cell_table", "iph_batch_timing", "iph_batch_stats", "iph_batch_bins", "iph_batch_cdf",
             "iph_build_cdf", "iph_build_cdf_noalloc", "iph_atomic_probe", "iph_write_record", "iph_water_optics", "iph_water_index", "iph_drop_shape",
             "iph_drop_projected_area", "iph_hex_projected_area", "iph_batch_probabilities", "iph_sur_create",
             "iph_sur_destroy", "iph_sur_param_count", "iph_sur_get_params", "iph_sur_set_params",
@@ -675,6 +676,10 @@ class Batch:
 
     def set_schedule(self, tile_rays: int = 0, blocks: int = 0) -> None:
         _check(lib().iph_batch_set_schedule(_live(self._p, "batch"), tile_rays, blocks), self.ctx.ptr)
+
+    def set_cell_table(self, enable: bool) -> None:
+        """Hex kernels: workgroup LDS cell table (True, default) or every quantum straight to HBM."""
+        _check(lib().iph_batch_set_cell_table(_live(self._p, "batch"), int(enable)), self.ctx.ptr)
 
     def timing(self) -> tuple[float, float, float]:
         ms = (f64 * 3)()

@@ -187,6 +187,14 @@ int iph_batch_run(iph_batch* b, uint32_t flags);
 /* Tile size in rays and workgroups of the trace launch (0 = library choice).
  * The autotuner in the package sets these. */
 int iph_batch_set_schedule(iph_batch* b, uint64_t tile_rays, uint32_t blocks);
+/* enable = 1 (default): the hex kernels add their quanta into a
+ * workgroup-private LDS cell table (2048 slots) flushed into the histogram once
+ * per tile, and a cell that finds no slot within 16 probes goes to the
+ * histogram directly; 0: every quantum goes to the histogram directly (u64 HBM
+ * atomics). The histogram is the same bits either way. No reference
+ * counterpart (the reference accumulates samples after tracing,
+ * hex_phase/device/phase_accumulate.cu:10-28). */
+int iph_batch_set_cell_table(iph_batch* b, uint32_t enable);
 /* Device time of the last run, from HIP events on the batch stream:
  * ms[0] zero+trace launches, ms[1] CDF kernels, ms[2] whole run. */
 int iph_batch_timing(const iph_batch* b, double* ms /* 3 */);

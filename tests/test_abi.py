@@ -143,6 +143,10 @@ def test_atlas_entry_points_reject_bad_arguments_without_a_device():
     assert count.value > 1
 
 
+def test_cell_table_switch_rejects_a_missing_batch():
+    assert icephase.lib().iph_batch_set_cell_table(None, 1) == -1
+
+
 def test_write_npy_matches_numpy(tmp_path):
     data = np.arange(2 * 3 * 4, dtype=np.float64).reshape(2, 3, 4) / 7
     shape = (ctypes.c_uint64 * 3)(2, 3, 4)

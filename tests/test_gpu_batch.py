@@ -55,6 +55,33 @@ def test_schedules_that_wrap_the_tile_queue_are_refused(ctx):
             assert np.array_equal(b.bins(i), ref[i])
 
 
+def test_hex_cell_table_and_direct_atomics_give_the_same_histogram(ctx, oracle):
+    """The hex kernels add their quanta into a workgroup LDS cell table flushed
+    once per tile, or (switched off) every quantum straight to HBM, the path a
+    cell that finds no slot takes: both give the oracle's histogram bit for bit,
+    eagerly and from the captured graph, and the direct kernel counts one HBM
+    atomic per binned emission."""
+    print("case: hex column 2^15 rays 900x1800, cell table on / off", flush=True)
+    nt, np_, rays = 900, 1800, 1 << 15
+    rec = ip.hex_record(0.05, 0.2, 1.31, rays, incident=(1.0, -0.4, 0.3), seed=12345, theta=nt, phi=np_,
+                        max_internal_hits=12, tail=1e-12)
+    s = oracle.hex_settings(radius_mm=0.05, length_mm=0.2, ki=(1.0, -0.4, 0.3), n_int=1.31, tail=1e-12,
+                            seed=12345, total=rays, max_hits=12)
+    want, _ = oracle.hex_hist(s, nt, np_)
+    atomics = {}
+    with ip.Batch(ctx, [rec]) as b:
+        for enable, flags in ((True, ip.RUN_TRACE), (False, ip.RUN_TRACE), (False, ip.RUN_TRACE | ip.RUN_GRAPH),
+                              (True, ip.RUN_TRACE | ip.RUN_GRAPH)):
+            b.set_cell_table(enable)
+            b.run(flags)
+            got = b.bins()
+            assert np.array_equal(got, want), (enable, flags, int(np.count_nonzero(got != want)))
+            atomics[enable] = int(b.stats().cell_atomics)
+        st = b.stats()
+    print(f"HBM cell atomics, table on / off: {atomics[True]} / {atomics[False]}", flush=True)
+    assert 0 < atomics[True] < atomics[False] <= st.samples
+
+
 def test_atomic_probe_counts_every_increment(ctx):
     """The atomic calibration lands every increment (the library checks the slab
     sum against the count and fails otherwise) and refuses bad arguments."""
