@@ -4,11 +4,14 @@
 // record into tiles of consecutive ray ids; workgroups pull tiles from an
 // atomic queue until the queue is empty (the exit condition every wave
 // reaches), so uneven records and uneven ray lengths balance across the 256
-// CUs. Each lane traces whole rays and sends every emission to the record's
+// CUs. Each lane traces whole rays and adds every emission to the record's
 // histogram as a u64 fixed-point quantum (rint(power * 2^Q)): integer sums are
 // associative, so the histogram is bitwise independent of tile order, wave
-// scheduling and atomic arrival order. Pole quanta and audit counters stay in
-// lane registers and are reduced per wave before one atomic per counter.
+// scheduling and atomic arrival order. Drop waves merge the emissions of a
+// round that fall in one cell (adjacent lanes) before the HBM atomic; hex
+// workgroups add into an LDS cell table flushed once per tile. Pole quanta and
+// audit counters stay in lane registers and are reduced per wave before one
+// atomic per counter.
 //
 // The record's u edges are staged in LDS when they fit (nt <= kLdsEdges-1) so
 // the row lookup (phase_bin.hpp) reads LDS, not L2.
