@@ -673,9 +673,10 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
 
 // Waves per SIMD the drop kernel is compiled for. Uncapped it runs 2 waves per
 // SIMD; the serial Newton chains and the cell atomics' latency want more waves
-// than the register file holds without spills. Measured on 48 records of the
-// oblate workload: 2 waves 668 ms, 3 waves (168 VGPRs, 204 B/lane of spills)
-// 540 ms, 4 waves (128 VGPRs, 420 B/lane) 508 ms.
+// than the register file holds without spills. Measured on the first 48 records
+// of the oblate workload at this build: 3 waves (168 VGPRs, 220 B/lane of
+// spills) 504 ms, 4 waves (128 VGPRs, 396 B/lane) 472 ms, 5 waves (96 VGPRs,
+// 528 B/lane) 716 ms; an earlier build ran 2 waves (uncapped) in 668 ms.
 #ifndef ICEPHASE_DROP_WAVES
 #define ICEPHASE_DROP_WAVES 4
 #endif
