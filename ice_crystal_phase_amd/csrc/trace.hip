@@ -34,23 +34,40 @@ struct LaneCounters {
     uint32_t atomics;  // cell atomicAdds issued (per tile, so 32 bits per lane suffice)
 };
 
-__device__ inline void flush_counters(LaneCounters& c, unsigned long long* stats)
+// Counts the drop tile keeps per wave rather than per lane: events of a round
+// counted where the whole wave runs (popcounts of ballots), so they take no
+// lane registers through the tracing loops.
+struct WaveTally {
+    unsigned long long samples, atomics;
+    uint32_t status[9];
+};
+
+__device__ inline void flush_counters(LaneCounters& c, unsigned long long* stats, const WaveTally& w)
 {
     const unsigned long long v[9] = {c.fwd,     c.bwd,     c.samples,    c.invalid,         c.snapped,
                                      c.escaped, c.unresolved, c.hits, (unsigned long long)c.atomics};
+    const unsigned long long wv[9] = {0, 0, w.samples, 0, 0, 0, 0, 0, w.atomics};
     const uint32_t slot[9] = {kStatForward, kStatBackward, kStatSamples,    kStatInvalid,    kStatSnapped,
                               kStatEscaped, kStatUnresolved, kStatInternalHits, kStatCellAtomics};
     const bool lead = (threadIdx.x & 63) == 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const unsigned long long s = wave_sum(v[k]);
+        const unsigned long long s = wave_sum(v[k]) + wv[k];
         if (lead && s) atomicAdd(&stats[slot[k]], s);
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const unsigned long long s = wave_sum((unsigned long long)c.status[k]);
+        const unsigned long long s = wave_sum((unsigned long long)c.status[k]) + w.status[k];
         if (lead && s) atomicAdd(&stats[kStatStatus0 + k], s);
     }
+}
+
+// Status s of every lane where `on`, tallied for the wave (the whole wave calls it).
+__device__ inline void tally_status(WaveTally& w, bool on, uint32_t s)
+{
+    s = s <= 8 ? s : 7;
+#pragma unroll
+    for (uint32_t k = 0; k < 9; ++k) w.status[k] += (uint32_t)__popcll(__ballot(on && s == k));
 }
 
 // Predicated adds keep the status array in registers (no dynamic indexing).
@@ -355,7 +372,7 @@ __device__ inline void bin_deferred(const DevRecord& R, const Deferred& e, EdgeL
 // quanta make the histogram independent of when an emission is binned.
 template <class EdgeLoad>
 __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t count, uint32_t* s_next,
-                                 Deferred* ring, EdgeLoad edge, LaneCounters& c)
+                                 Deferred* ring, EdgeLoad edge, LaneCounters& c, WaveTally& w)
 {
     using namespace drop;
     const Config& C = R.drop;
@@ -378,7 +395,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
     // cells, always as runs of adjacent lanes): a segmented sum over each run of
     // equal cells leaves one u64 atomicAdd per run, issued by its first lane.
     auto emit = [&](bool valid, const Ray& r) {
-        bool defer = false;
+        bool defer = false, counted = false;
         float pw = 0.0f;
         uint32_t cell = 0xffffffffu;  // no cell: no quantum to add
         unsigned long long sum = 0;
@@ -387,7 +404,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
             if (pw > 0.0f) {
                 const unsigned long long q = quantize((double)pw, R.qbits);
                 rq += q;
-                ++c.samples;
+                counted = true;
                 const d3 d = D3((double)r.dir.x, (double)r.dir.y, (double)r.dir.z);
                 if (finite64(d)) {
                     const CellHit h = bin_fast_approx(R.nt, R.np, R.frame, d);
@@ -415,10 +432,10 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
                 const uint32_t vrun = __shfl_down(run, off, 64);  // every lane shuffles
                 if (lane + off < 64 && vrun == run) sum += v;
             }
-            if (first_of_run && cell != 0xffffffffu && sum) {
-                atomicAdd(&R.bins[cell], sum);
-                ++c.atomics;
-            }
+            const bool add = first_of_run && cell != 0xffffffffu && sum;
+            if (add) atomicAdd(&R.bins[cell], sum);
+            w.atomics += (uint32_t)__popcll(__ballot(add));
+            w.samples += (uint32_t)__popcll(__ballot(counted));
         }
         const unsigned long long m = __ballot(defer);
         if (m) {
@@ -432,7 +449,10 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
         float u = 0.0f, v = 0.0f, rho2 = 0.0f, ta = 0.0f, tm = 0.0f;
         f3 o = F3(0.0f, 0.0f, 0.0f);
         bool need = !exhausted;
+        bool missed = false;  // this lane's id of the last round missed the drop
         for (;;) {
+            tally_status(w, missed, kMiss);
+            missed = false;
             const unsigned long long m = __ballot(need);
             if (m == 0) break;
             const uint32_t n = (uint32_t)__popcll(m);
@@ -453,7 +473,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
             v = ((float)(2u * iy + 1u) / (float)C.hgt - 1.0f) * C.h;
             rho2 = fmaf(v, v, u * u);
             if (!(rho2 < rmax2)) {
-                count_status(c, kMiss);
+                missed = true;
                 continue;
             }
             o = axpy(C.e1, v, C.e0 * u);
@@ -493,7 +513,7 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
             if (have)
                 need = false;
             else
-                count_status(c, kMiss);
+                missed = true;
         }
         if (__ballot(have) == 0) break;
         // From here every lane runs each round; lanes without a ray only take
@@ -594,8 +614,8 @@ __device__ inline void drop_tile(const DevRecord& R, uint64_t first, uint64_t co
             c.escaped += rq;
             c.unresolved += quantize((double)unresolved, R.qbits);
             c.hits += hits;
-            count_status(c, status);
         }
+        tally_status(w, have, status);
         rq = 0;
     }
     if (tail != head) drain(tail - head);  // the tile's remaining deferred emissions
@@ -644,6 +664,7 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
         const uint64_t first = (t - map.tile_start[lo]) * R.tile_rays;
         const uint64_t count = R.rays - first < R.tile_rays ? R.rays - first : R.tile_rays;
         LaneCounters c{};
+        WaveTally w{};
         if constexpr (LDS_EDGES) {
             if (s_cached != rec) {
                 __syncthreads();
@@ -655,19 +676,21 @@ __device__ inline void trace_loop(const DevRecord* __restrict__ recs, TileMap ma
             if constexpr (KIND == kKindHex)
                 hex_tile<AREA>(R, first, count, s_hring + (threadIdx.x >> 6) * kHexRing, [&](int k) { return s_edges[k]; }, table, c);
             else
-                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [&](int k) { return s_edges[k]; }, c);
+                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [&](int k) { return s_edges[k]; }, c,
+                          w);
         } else {
             const double* e = R.edges;
             if constexpr (KIND == kKindHex)
                 hex_tile<AREA>(R, first, count, s_hring + (threadIdx.x >> 6) * kHexRing, [e](int k) { return e[k]; }, table, c);
             else
-                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [e](int k) { return e[k]; }, c);
+                drop_tile(R, first, count, &s_next, s_ring + (threadIdx.x >> 6) * kRing, [e](int k) { return e[k]; }, c,
+                          w);
         }
         if constexpr (KIND == kKindHex && !DIRECT) {
             __syncthreads();  // every lane's quanta are in the table
             table.flush(R.bins, c);
         }
-        flush_counters(c, R.stats);
+        flush_counters(c, R.stats, w);
     }
 }
 
