@@ -365,7 +365,9 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
     const double col = active ? G(J.cols)[j] : 0.0;
     const double a0 = active ? G(J.phi)[j] : 0.0, a1 = active ? G(J.phi)[j + 1] : 0.0;
     const double da = a1 - a0;
-    Neumaier l1{0, 0}, lost{0, 0}, rec{0, 0}, ax{0, 0};
+    // audit sums (report only): plain FP64 per thread over at most a few hundred
+    // cells, then fixed-order wave and block sums
+    double l1 = 0, lost = 0, rec = 0, ax = 0;
     double maxcell = 0, lostcells = 0, malformed = 0;
     if (ph == 0) {
         s_carry[c] = 0;
@@ -426,14 +428,14 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
                     const double p = mv[k] / Mc;
                     malformed += (!(q >= 0 && q <= DBL_MAX) || next < prev) ? 1 : 0;
                     const double err = fabs(p - q);
-                    l1.add(err);
+                    l1 += err;
                     maxcell = fmax(maxcell, err);
                     if (p > 0 && q == 0) {
-                        lost.add(p);
+                        lost += p;
                         lostcells += 1;
                     }
-                    rec.add(q);
-                    ax.add(q * (1 - G(J.cedges)[i] - G(J.cedges)[i + 1]));
+                    rec += q;
+                    ax += q * (1 - G(J.cedges)[i] - G(J.cedges)[i + 1]);
                 }
             }
         }
@@ -452,7 +454,7 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
         __syncthreads();
     }
     // block reduction of the audit terms
-    double v[8] = {l1.value(), lost.value(), rec.value(), ax.value(), maxcell, lostcells, malformed, 0};
+    double v[8] = {l1, lost, rec, ax, maxcell, lostcells, malformed, 0};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         for (int off = 32; off > 0; off >>= 1) {
