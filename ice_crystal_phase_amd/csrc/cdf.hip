@@ -31,7 +31,8 @@ __device__ inline gptr<T> G(T* p)
 }
 
 constexpr int kRedThreads = 256;
-constexpr uint32_t kSumChunk = 16;  // rows a column-sum lane has in flight
+constexpr uint32_t kSumChunk = 16;    // rows a column-sum lane has in flight
+constexpr uint32_t kMomentBatch = 8;  // cells a moments thread has in flight
 
 struct Neumaier {
     double s, c;
@@ -61,6 +62,64 @@ __device__ inline double stored_mass(const CdfJob& J, uint32_t i, uint32_t j)
     if (J.coarse) return G(J.coarse)[(uint64_t)i * J.onp + j];
     return cell_mass(J, i, j);
 }
+
+// Reader of a job's stored cells with the source resolved once: FP64 rows (the
+// coarse buffer, or the FP64 masses) or u64 quanta with the pole shares added in
+// the end rows (the operations of cell_mass, with each share computed once).
+struct StoredReader {
+    gptr<const double> f;
+    gptr<const unsigned long long> q;
+    uint64_t stride;
+    uint32_t last;  // last fine row
+    double first_share, last_share;
+
+    __device__ static StoredReader of(const CdfJob& J)
+    {
+        StoredReader r{nullptr, nullptr, J.np, J.nt - 1, 0.0, 0.0};
+        if (J.coarse) {
+            r.f = G((const double*)J.coarse);
+            r.stride = J.onp;
+        } else if (!J.bins) {
+            r.f = G(J.mass);
+        } else {
+            r.q = G(J.bins);
+            r.first_share = (double)G(J.stats)[kStatForward] / (double)J.np;
+            r.last_share = (double)G(J.stats)[kStatBackward] / (double)J.np;
+        }
+        return r;
+    }
+    // The fine cells (cell_mass) even where the job stores a coarsened grid.
+    __device__ static StoredReader fine(const CdfJob& J)
+    {
+        StoredReader r{nullptr, nullptr, J.np, J.nt - 1, 0.0, 0.0};
+        if (!J.bins) {
+            r.f = G(J.mass);
+        } else {
+            r.q = G(J.bins);
+            r.first_share = (double)G(J.stats)[kStatForward] / (double)J.np;
+            r.last_share = (double)G(J.stats)[kStatBackward] / (double)J.np;
+        }
+        return r;
+    }
+    // The cell's 64-bit word, whatever the source. Reading a batch of words and
+    // converting them afterwards keeps every load of the batch in flight at once;
+    // a conversion per load (the source test and the u64 -> double between the
+    // loads) makes the compiler wait for each load before issuing the next.
+    __device__ unsigned long long word(uint32_t i, uint32_t j) const
+    {
+        const gptr<const unsigned long long> w = q ? q : (gptr<const unsigned long long>)f;
+        return w[(uint64_t)i * stride + j];
+    }
+    // The mass of row i from its word (the operations of cell_mass).
+    __device__ double mass(unsigned long long w, uint32_t i) const
+    {
+        if (f) return __longlong_as_double((long long)w);
+        double m = (double)w;
+        if (i == 0) m = m + first_share;
+        if (i == last) m = m + last_share;
+        return m;
+    }
+};
 
 __device__ inline double block_sum(double v, double* sh)
 {
@@ -116,18 +175,30 @@ extern "C" __global__ void __launch_bounds__(kRedThreads) icephase_cdf_moments(c
     const CdfJob J = jobs[blockIdx.y];
     Neumaier mf{0, 0}, af{0, 0}, ms{0, 0}, as{0, 0};
     double neg = 0, largest = 0;
-    // rows over blocks, columns over threads: one row's reads are contiguous
+    // rows over blocks, columns over threads: one row's reads are contiguous, and
+    // each thread has kMomentBatch of them in flight
+    const StoredReader fine = StoredReader::fine(J);
     for (uint32_t i = blockIdx.x; i < J.nt; i += gridDim.x) {
         const double axial = 1 - G(J.edges)[i] - G(J.edges)[i + 1];
-        for (uint32_t j = threadIdx.x; j < J.np; j += blockDim.x) {
-            const double m = cell_mass(J, i, j);
-            if (!(m >= 0 && m <= DBL_MAX)) {
-                neg += 1;
-                continue;
+        for (uint32_t j0 = threadIdx.x; j0 < J.np; j0 += blockDim.x * kMomentBatch) {
+            unsigned long long wv[kMomentBatch];
+#pragma unroll
+            for (uint32_t b = 0; b < kMomentBatch; ++b) {
+                const uint32_t j = j0 + b * blockDim.x;
+                wv[b] = j < J.np ? fine.word(i, j) : 0ull;
             }
-            mf.add(m);
-            af.add(m * axial);
-            largest = fmax(largest, m);
+#pragma unroll
+            for (uint32_t b = 0; b < kMomentBatch; ++b) {
+                if (j0 + b * blockDim.x >= J.np) continue;
+                const double m = fine.mass(wv[b], i);
+                if (!(m >= 0 && m <= DBL_MAX)) {
+                    neg += 1;
+                    continue;
+                }
+                mf.add(m);
+                af.add(m * axial);
+                largest = fmax(largest, m);
+            }
         }
     }
     if (!J.coarse) {
@@ -231,18 +302,19 @@ extern "C" __global__ void __launch_bounds__(kRedThreads) icephase_cdf_colsum(co
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= J.onp) return;
     const double largest = G(J.scalars)[kCdfLargest];
+    const StoredReader src = StoredReader::of(J);
     double s = 0;
     uint32_t under = 0;
     for (uint32_t i0 = 0; i0 < J.ont; i0 += kSumChunk) {
         const uint32_t rows = J.ont - i0 < kSumChunk ? J.ont - i0 : kSumChunk;
         // the chunk's reads first, all in flight together; then the ordered sum
-        double mv[kSumChunk];
+        unsigned long long wv[kSumChunk];
 #pragma unroll
-        for (uint32_t r = 0; r < kSumChunk; ++r) mv[r] = r < rows ? stored_mass(J, i0 + r, j) : 0.0;
+        for (uint32_t r = 0; r < kSumChunk; ++r) wv[r] = r < rows ? src.word(i0 + r, j) : 0ull;
 #pragma unroll
         for (uint32_t r = 0; r < kSumChunk; ++r) {
             if (r < rows) {
-                const double m = mv[r];
+                const double m = src.mass(wv[r], i0 + r);
                 const double w = m / largest;
                 under += (m > 0 && !(w > 0)) ? 1u : 0u;
                 s += w;
@@ -297,41 +369,6 @@ extern "C" __global__ void __launch_bounds__(256) icephase_cdf_marginal(const Cd
     }
 }
 
-// Reader of a job's stored cells with the source resolved once: FP64 rows (the
-// coarse buffer, or the FP64 masses) or u64 quanta with the pole shares added in
-// the end rows (the operations of cell_mass, with each share computed once).
-struct StoredReader {
-    gptr<const double> f;
-    gptr<const unsigned long long> q;
-    uint64_t stride;
-    uint32_t last;  // last fine row
-    double first_share, last_share;
-
-    __device__ static StoredReader of(const CdfJob& J)
-    {
-        StoredReader r{nullptr, nullptr, J.np, J.nt - 1, 0.0, 0.0};
-        if (J.coarse) {
-            r.f = G((const double*)J.coarse);
-            r.stride = J.onp;
-        } else if (!J.bins) {
-            r.f = G(J.mass);
-        } else {
-            r.q = G(J.bins);
-            r.first_share = (double)G(J.stats)[kStatForward] / (double)J.np;
-            r.last_share = (double)G(J.stats)[kStatBackward] / (double)J.np;
-        }
-        return r;
-    }
-    __device__ double at(uint32_t i, uint32_t j) const
-    {
-        if (f) return f[(uint64_t)i * stride + j];
-        double m = (double)q[(uint64_t)i * stride + j];
-        if (i == 0) m = m + first_share;
-        if (i == last) m = m + last_share;
-        return m;
-    }
-};
-
 // Pass 4: the ordered prefix again (same operations -> same values), normalized
 // into phi-major theta, with the audit of each cell. A block owns a strip of 64
 // stored columns and walks it in tiles of 32 rows staged in LDS: its 256 threads
@@ -351,6 +388,7 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
     __shared__ double s_carry[kThetaCols];               // prefix after the previous tile
     __shared__ double n_carry[2][kThetaCols];            // normalized value of the previous tile's last row
     __shared__ double s_red[kThetaPhases][8];
+    __shared__ double s_ce[kThetaRows + 1];  // the tile's stored u edges
     const CdfJob J = jobs[blockIdx.y];
     const StoredReader src = StoredReader::of(J);
     const uint32_t c = threadIdx.x % kThetaCols, ph = threadIdx.x / kThetaCols;
@@ -380,12 +418,19 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
     uint32_t tile = 0;
     for (uint32_t i0 = 0; i0 < nt; i0 += kThetaRows, ++tile) {
         const uint32_t rows = nt - i0 < kThetaRows ? nt - i0 : kThetaRows;
+        if (threadIdx.x <= rows) s_ce[threadIdx.x] = G(J.cedges)[i0 + threadIdx.x];
         // stored masses of the tile, w = m / largest
         double mv[kThetaRows / kThetaPhases];
+        unsigned long long wv[kThetaRows / kThetaPhases];
 #pragma unroll
         for (uint32_t k = 0; k < kThetaRows / kThetaPhases; ++k) {
             const uint32_t r = ph + kThetaPhases * k;
-            mv[k] = (active && r < rows) ? src.at(i0 + r, j) : 0.0;
+            wv[k] = (active && r < rows) ? src.word(i0 + r, j) : 0ull;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kThetaRows / kThetaPhases; ++k) {
+            const uint32_t r = ph + kThetaPhases * k;
+            mv[k] = (active && r < rows) ? src.mass(wv[k], i0 + r) : 0.0;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kThetaRows / kThetaPhases; ++k) sv[ph + kThetaPhases * k][c] = mv[k] / largest;
@@ -410,7 +455,7 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
                 if (i + 1 == nt)
                     next = 1.0;
                 else
-                    next = col > 0 ? sv[r][c] / col : G(J.cedges)[i + 1];
+                    next = col > 0 ? sv[r][c] / col : s_ce[r + 1];
                 sv[r][c] = next;
             }
         }
@@ -421,7 +466,6 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
             for (uint32_t k = 0; k < kThetaRows / kThetaPhases; ++k) {
                 const uint32_t r = ph + kThetaPhases * k;
                 if (r < rows) {
-                    const uint32_t i = i0 + r;
                     const double next = sv[r][c];
                     const double prev = r ? sv[r - 1][c] : n_carry[tile & 1][c];
                     const double q = da * (next - prev);
@@ -435,7 +479,7 @@ extern "C" __global__ void __launch_bounds__(kThetaThreads) icephase_cdf_theta(c
                         lostcells += 1;
                     }
                     rec += q;
-                    ax += q * (1 - G(J.cedges)[i] - G(J.cedges)[i + 1]);
+                    ax += q * (1 - s_ce[r] - s_ce[r + 1]);
                 }
             }
         }

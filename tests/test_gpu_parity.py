@@ -164,6 +164,37 @@ def test_build_cdf_sparse_hdr_and_coarsening(ctx, oracle):
         assert rep.l1_error < 2e-14
 
 
+def _cdf_masses(nt, np_, seed):
+    rng = np.random.default_rng(seed)
+    m = rng.random((nt, np_)) ** 6
+    m[:, np_ // 3] = 0.0                # an empty column takes the u edges
+    m[nt // 2, :] *= 1e4                # a bright row
+    m[:, -1] = 0.0
+    m[nt // 4, -1] = 1e-20              # a column too faint to move the marginal: d = 0, its mass lost
+    m[rng.random((nt, np_)) < 0.2] = 0  # empty cells inside columns: dtheta = 0
+    return m.reshape(-1)
+
+
+@pytest.mark.parametrize("nt,np_", [(1024, 45), (900, 7), (3, 13), (1025, 24), (1100, 40)],
+                         ids=["1024x45", "900x7", "3x13", "1025x24", "1100x40"])
+def test_cdf_tiles_empty_and_lost_columns_match_the_oracle(ctx, oracle, nt, np_):
+    """Row counts on and off the theta pass's 32-row tiles, ragged 64-column
+    strips, an empty column (u edges), cells absorbed by the ordered prefix
+    (dtheta = 0) and a column too faint to move the marginal (dphi = 0): the
+    oracle's CDFs bitwise, its report within 1e-12 and its lost-cell count."""
+    print(f"case: CDF {nt}x{np_}", flush=True)
+    mass = _cdf_masses(nt, np_, seed=nt * 31 + np_)
+    phi, theta, rep, rc = ctx.build_cdf(mass, nt, np_)
+    orc, ophi, otheta, orep = oracle.build_cdf(mass, nt, np_)
+    assert rc == 0 and orc == 0
+    assert np.array_equal(phi, ophi)
+    assert np.array_equal(theta, otheta)
+    for f in ("g_source", "g_stored", "cdf_mass", "l1_error", "lost_mass", "max_cell_error"):
+        assert abs(getattr(rep, f) - getattr(orep, f)) <= 1e-12, f
+    assert rep.lost_cells == orep.lost_cells and rep.lost_cells > 0
+    assert rep.lost_mass > 0
+
+
 def test_build_cdf_rejections(ctx):
     nt, np_ = 8, 16
     with pytest.raises(ip.AuditError):
