@@ -441,68 +441,111 @@ struct QueryPass {
     const uint64_t* hoff;   // fill pass: hit offsets per direction
     unsigned int* cursor;   // fill pass: per-direction fill counters
     QHit* out;              // fill pass: hit storage
-    struct Staged* stage;   // count pass: hits in discovery order (up to cap)
+    struct Staged* stage;   // count pass: hits in per-wave runs of slots, gaps marked (up to cap)
     unsigned long long* staged;
     uint64_t cap;
     uint32_t fill;
 };
 
 // A hit found by the count pass with its direction, kept so that the hits need
-// not be recomputed to be stored.
+// not be recomputed to be stored; dir == kStageGap marks a slot left unused.
 struct Staged {
     uint32_t dir;
     QHit hit;
 };
+constexpr uint32_t kStageGap = 0xffffffffu;
+constexpr uint32_t kStageChunk = 256;  // staging slots a wave reserves at a time
 
 extern "C" __global__ void __launch_bounds__(kThreads) icephase_coh_query(QueryPass Q)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint32_t lane = threadIdx.x & 63;
-    // every lane of a wave walks the same iterations (the bound is per wave),
-    // so the wave's first candidate is searched once and broadcast
-    for (uint64_t wk = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wk < Q.total; wk += stride) {
-        const uint64_t k = wk + lane;
-        uint32_t base = 0;
-        if (lane == 0) {
-            uint32_t lo = 0, hi = Q.npatch;  // the patch whose candidate range holds wk
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (Q.coff[mid] <= wk)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            base = lo;
+    // each wave walks one contiguous run of candidates, 64 at a time: the patch of
+    // its first candidate is searched once (by lane 0, broadcast), after which the
+    // patch cursor only moves forward (from the last lane's patch of the previous
+    // step), so no step repeats the search over all patches
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t per = ((Q.total + nwaves - 1) / nwaves + 63) & ~63ull;
+    const uint64_t begin = wave * per;
+    const uint64_t end = begin + per < Q.total ? begin + per : Q.total;
+    if (begin >= end) return;  // the whole wave
+    uint32_t base = 0;
+    if (lane == 0) {
+        uint32_t lo = 0, hi = Q.npatch;  // the patch whose candidate range holds begin
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (Q.coff[mid] <= begin)
+                lo = mid;
+            else
+                hi = mid;
         }
-        base = __shfl(base, 0, 64);
-        if (k >= Q.total) continue;
+        base = lo;
+    }
+    base = __shfl(base, 0, 64);
+    // count pass: the wave stages its hits in a run of slots it reserved with one
+    // atomic (kStageChunk or more at a time; one counter shared by every wave would
+    // take an atomic on one address per step with a hit); the slots it leaves
+    // unused are marked as gaps, which the scatter skips
+    unsigned long long chunk = 0;
+    uint32_t used = 0, size = 0;
+    for (uint64_t wk = begin; wk < end; wk += 64) {
+        const uint64_t k = wk + lane;
         uint32_t p = base;  // consecutive candidates: step forward to the patch of k
-        while (p + 1 < Q.npatch && Q.coff[p + 1] <= k) ++p;
-        const Extent e = Q.ext[p];
-        const uint64_t r = k - Q.coff[p];
-        const uint32_t row = e.row0 + (uint32_t)(r / e.cols);
-        uint32_t col = e.col0 + (uint32_t)(r % e.cols);
-        if (col >= Q.np) col -= Q.np;
-        const uint64_t di = (uint64_t)row * Q.np + col;
-        const v3 w = Q.dirs[di];
-        const PatchGeom& g = Q.G[p];
-        if (!ray_meets_box(w, g.lo, g.hi)) continue;
-        v3 c[4];
-        for (int i = 0; i < 4; ++i) c[i] = g.c[i];
-        const PHits h = bilinear_intersect(c, w);
-        const uint32_t pid = g.id;
+        if (k < end)
+            while (p + 1 < Q.npatch && Q.coff[p + 1] <= k) ++p;
+        // lane 63 holds the step's last patch; if its k is past the end, this is the last step
+        base = __shfl(p, 63, 64);
+        PHits h;
+        h.count = 0;
+        h.flags = 0;
+        uint64_t di = 0;
+        uint32_t pid = 0;
+        if (k < end) {
+            const Extent e = Q.ext[p];
+            const uint32_t r = (uint32_t)(k - Q.coff[p]);  // below 2^31: extent counts are clamped there
+            const uint32_t row = e.row0 + r / e.cols;
+            uint32_t col = e.col0 + r % e.cols;
+            if (col >= Q.np) col -= Q.np;
+            di = (uint64_t)row * Q.np + col;
+            const v3 w = Q.dirs[di];
+            const PatchGeom& g = Q.G[p];
+            if (ray_meets_box(w, g.lo, g.hi)) {
+                v3 c[4];
+                for (int i = 0; i < 4; ++i) c[i] = g.c[i];
+                h = bilinear_intersect(c, w);
+                pid = g.id;
+            }
+        }
         if (!Q.fill) {
             if (h.flags) atomicOr(&Q.notes[di], h.flags);
             if (h.flags & (kQueryUnresolved | kQueryInvalid)) atomicMin(&Q.first[di], pid);
-            if (h.count) {
-                atomicAdd(&Q.hits[di], h.count);
-                const unsigned long long slot = atomicAdd(Q.staged, (unsigned long long)h.count);
+            if (h.count) atomicAdd(&Q.hits[di], h.count);
+            // the wave's inclusive prefix of hit counts
+            uint32_t incl = h.count;
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                const uint32_t t = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += t;
+            }
+            const uint32_t wave_hits = __shfl(incl, 63, 64);
+            if (wave_hits) {
+                if (used + wave_hits > size) {
+                    for (uint32_t z = used + lane; z < size; z += 64)
+                        if (chunk + z < Q.cap) Q.stage[chunk + z].dir = kStageGap;
+                    const uint32_t want = wave_hits > kStageChunk ? wave_hits : kStageChunk;
+                    unsigned long long first = 0;
+                    if (lane == 0) first = atomicAdd(Q.staged, (unsigned long long)want);
+                    chunk = __shfl(first, 0, 64);
+                    used = 0;
+                    size = want;
+                }
+                const unsigned long long slot = chunk + used + (incl - h.count);
                 for (uint32_t j = 0; j < h.count; ++j) {
                     if (slot + j >= Q.cap) break;  // over capacity: the host recomputes with the fill pass
                     Staged& z = Q.stage[slot + j];
                     z.dir = (uint32_t)di;
                     z.hit = QHit{p, pid, j, h.h[j].flags, h.h[j].u, h.h[j].v, h.h[j].t, h.h[j].res};
                 }
+                used += wave_hits;
             }
         } else {
             for (uint32_t j = 0; j < h.count; ++j) {
@@ -521,6 +564,9 @@ extern "C" __global__ void __launch_bounds__(kThreads) icephase_coh_query(QueryP
             }
         }
     }
+    if (!Q.fill)
+        for (uint32_t z = used + lane; z < size; z += 64)
+            if (chunk + z < Q.cap) Q.stage[chunk + z].dir = kStageGap;
 }
 
 // Staged hits to their direction's segment (slots by per-direction counters;
@@ -532,6 +578,7 @@ extern "C" __global__ void __launch_bounds__(kThreads)
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const Staged z = stage[k];
+    if (z.dir == kStageGap) return;
     const uint32_t at = atomicAdd(&cursor[z.dir], 1u);
     if (at >= cnt[z.dir]) return;  // the finish pass flags the mismatch
     out[hoff[z.dir] + at] = z.hit;
@@ -1495,29 +1542,33 @@ int iph_coherent_evaluate(iph_coherent* h)
         Q.hits = h->qcnt.p;
         Q.notes = h->qnotes.p;
         Q.first = h->qfirst.p;
+        const uint32_t qblocks = (uint32_t)std::min<uint64_t>(blocks_for(total ? total : 1, kThreads),
+                                                              (uint64_t)ctx->cus * 16);
         // staging for the count pass: four hits per direction to start with,
-        // grown to the last record's count
-        IPH_HIP(ctx, grow(h->stage, std::max<uint64_t>(std::max<uint64_t>(4 * ndir, h->nhits), 1u << 20)));
+        // grown to the last record's count, plus one chunk per wave (the most its
+        // gaps can take)
+        const uint64_t gaps = (uint64_t)qblocks * (kThreads / 64) * kStageChunk;
+        IPH_HIP(ctx, grow(h->stage, std::max<uint64_t>(std::max<uint64_t>(4 * ndir, h->nhits), 1u << 20) + gaps));
         IPH_HIP(ctx, grow(h->staged, 1));
         IPH_HIP(ctx, hipMemsetAsync(h->staged.p, 0, sizeof(unsigned long long), s));
         Q.stage = h->stage.p;
         Q.staged = h->staged.p;
         Q.cap = h->stage.n;
-        const uint32_t qblocks = (uint32_t)std::min<uint64_t>(blocks_for(total ? total : 1, kThreads),
-                                                              (uint64_t)ctx->cus * 16);
         if (total) hipLaunchKernelGGL(icephase_coh_query, dim3(qblocks), dim3(kThreads), 0, s, Q);
         IPH_HIP(ctx, hipGetLastError());
         IPH_HIP(ctx, scan(h, h->qcnt.p, ndir, h->hoff.p));
         uint64_t nh = 0;
+        unsigned long long reserved = 0;  // staging slots the waves reserved (hits and gaps)
         IPH_HIP(ctx, hipMemcpyAsync(&nh, h->hoff.p + ndir, sizeof nh, hipMemcpyDeviceToHost, s));
+        IPH_HIP(ctx, hipMemcpyAsync(&reserved, h->staged.p, sizeof reserved, hipMemcpyDeviceToHost, s));
         IPH_HIP(ctx, hipStreamSynchronize(s));
         h->nhits = nh;
         st.hits = nh;
         IPH_HIP(ctx, grow(h->H, nh ? nh : 1));
-        if (nh <= Q.cap) {
-            if (nh)
-                hipLaunchKernelGGL(icephase_coh_scatter, dim3(blocks_for(nh, kThreads)), dim3(kThreads), 0, s,
-                                   h->stage.p, nh, h->hoff.p, h->qcnt.p, h->qcursor.p, h->H.p);
+        if (reserved <= Q.cap) {
+            if (reserved)
+                hipLaunchKernelGGL(icephase_coh_scatter, dim3(blocks_for(reserved, kThreads)), dim3(kThreads), 0, s,
+                                   h->stage.p, (uint64_t)reserved, h->hoff.p, h->qcnt.p, h->qcursor.p, h->H.p);
         } else {
             Q.fill = 1;
             Q.hoff = h->hoff.p;
